@@ -1,0 +1,159 @@
+"""Benchmark: time to relative residual 1e-8 of the matrix-free MGPCG on one
+MI355X, mixed precision vs plain FP64, plus level-0 matvec GDOF/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ellipsoids256]
+
+A step is one complete solve (x0 = 0 to FP64 true relative residual 1e-8) of the
+workload's right-hand side, resident in HBM; the timed region covers K steps
+bracketed by device synchronisation.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import vulkanfem_amd as V  # noqa: E402
+from vulkanfem_amd import problems  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def matvec_bytes(n_nodes, n_vox, s):
+    # x read once, y written once (3 components of s bytes), bit-packed occupancy and fixed mask
+    return 6 * s * n_nodes + math.ceil(n_vox / 8) + math.ceil(n_nodes / 8)
+
+
+def cpu_baseline(p, nlev, iterations, sample_iters, threads):
+    """The oracle (C restatement of the reference CPU path, additive cycle, FP64)
+    on the host cores: setup plus `sample_iters` PCG iterations, scaled to the
+    iteration count of the same algorithm's full solve."""
+    from oracle import oracle as O
+
+    O.set_threads(threads)
+    ke = O.ke0_closed_form(1.0, 0.3)
+    t0 = time.perf_counter()
+    s = O.OracleSystem(p.voxels, ke, p.fixed, nlev)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.pcg(p.loads[0], tol=1e-8, max_iter=sample_iters, precond=O.REF_ADDITIVE)
+    t_it = (time.perf_counter() - t0) / (sample_iters + 1)  # + the initial cycle
+    s.close()
+    return {"value": t_it * (iterations + 1), "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle setup ({t_setup:.1f} s, not counted) + {sample_iters} FP64 PCG iterations with the "
+                      f"reference additive V-cycle on the {p.name} workload, {t_it:.3f} s/iteration x "
+                      f"{iterations + 1} (initial cycle + the iteration count of the full GPU solve of the same "
+                      f"algorithm)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ellipsoids256")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=4)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    nlev = problems.WORKLOAD_LEVELS[args.workload]
+    p = problems.workload(args.workload)
+    ke = problems.ke0(1.0, 0.3)
+    log(f"workload {p.name}: ndof={p.ndof} levels={nlev} info={p.info}")
+    t0 = time.perf_counter()
+    A = V.assemble_system_matrix(p.voxels, ke, p.fixed, nlev)
+    t_setup = time.perf_counter() - t0
+    n_nodes = A.n // 3
+    n_vox = int(np.prod(p.voxels.shape))
+    log(f"setup {t_setup:.2f} s, levels {A.num_levels}")
+    A.set_rhs(p.loads[0])
+    prec = V.FP32 if args.precision == "fp32" else V.BF16
+    mixed = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=0.6, true_res_every=20)
+
+    for _ in range(args.warmup):
+        st = A.solve_resident(**mixed)
+        log("warmup", st)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(A.solve_resident(**mixed))
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    u_mixed = A.get_solution()
+    log("mixed", stats[-1], f"{t_mixed:.4f} s/solve")
+
+    out_extra = {}
+    if not args.no_fp64:
+        fp64 = dict(tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.6)
+        A.solve_resident(**fp64)
+        t0 = time.perf_counter()
+        st64 = A.solve_resident(**fp64)
+        t_fp64 = time.perf_counter() - t0
+        log("fp64", st64, f"{t_fp64:.4f} s/solve")
+        # FP64 reference solution for the displacement error (same operator, tol 1e-12)
+        A.solve_resident(tol=1e-12, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.6)
+        u_ref = A.get_solution()
+        err = float(np.linalg.norm(u_mixed - u_ref) / np.linalg.norm(u_ref))
+        # the reference's own additive cycle in FP64 (iteration count for the CPU baseline)
+        st_ref = A.solve_resident(tol=1e-8, precision=V.FP64, cycle=V.CYCLE_REFERENCE)
+        out_extra.update({"fp64_time_s": round(t_fp64, 5), "fp64_iterations": st64["iterations"],
+                          "speedup_vs_fp64": round(t_fp64 / t_mixed, 3), "disp_rel_err_vs_fp64": err,
+                          "ref_cycle_fp64_iterations": st_ref["iterations"],
+                          "ref_cycle_fp64_time_s": round(st_ref["solve_ms"] / 1e3, 5)})
+    # level-0 matvec: HIP-event timing on the library's stream
+    t_mv32 = A.time_kernel(0, V.FP32, reps=20) / 1e3
+    t_mv64 = A.time_kernel(0, V.FP64, reps=20) / 1e3
+    t_it = A.time_kernel(2, prec, reps=5) / 1e3
+    b32 = matvec_bytes(n_nodes, n_vox, 4)
+    achieved = b32 / t_mv32 / 1e9
+    out_extra.update({"matvec_gdofs_fp32": round(A.n / t_mv32 / 1e9, 2), "matvec_gdofs_fp64": round(A.n / t_mv64 / 1e9, 2),
+                      "matvec_ms_fp32": round(t_mv32 * 1e3, 4), "matvec_ms_fp64": round(t_mv64 * 1e3, 4),
+                      "iteration_ms": round(t_it * 1e3, 4), "iterations": stats[-1]["iterations"],
+                      "rel_residual": stats[-1]["rel_residual"], "gdof_iter_per_s":
+                          round(A.n * stats[-1]["iterations"] / t_mixed / 1e9, 2),
+                      "setup_s": round(t_setup, 3), "ndof": A.n})
+    cpu = None
+    if not args.no_cpu and "ref_cycle_fp64_iterations" in out_extra:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(p, nlev, out_extra["ref_cycle_fp64_iterations"], args.cpu_iters, threads)
+    rec = {
+        "metric": "time to rel-res 1e-8 (s) + matvec GDOF/s, 256^3 voxels, mixed vs plain FP64",
+        "value": round(t_mixed, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32" if prec == V.FP32 else "f64+bf16",
+        "data": "synthetic",
+        "config": {"workload": p.name, "voxels": list(p.voxels.shape[::-1]), "ndof": A.n, "levels": A.num_levels,
+                   "cycle": "damped V(1,1), omega 0.6", "tol": 1e-8, "true_res_every": 20},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "k_matvec_fine<float> (level-0 matvec)", "bytes_per_launch": b32},
+        "cpu_baseline": cpu,
+    }
+    rec.update(out_extra)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

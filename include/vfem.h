@@ -1,0 +1,164 @@
+/*
+ * vfem.h -- C-ABI of the MI355X matrix-free multigrid-preconditioned CG solver
+ * for voxel linear elasticity.
+ *
+ * Drop-in boundary for the GPU solver path of hirademirkol/VulkanFEM.  Every
+ * entry point names the reference interface it replaces (file:line).  Plain
+ * pointers and sizes only; all buffers are caller-owned host memory unless the
+ * name says "device".  Every call is synchronous and returns a vfem_status
+ * (0 = success); nothing throws across this boundary.  One vfem_system per host
+ * thread; the library owns all device memory it allocates.
+ *
+ * Conventions (identical to the reference):
+ *   - voxels: int32, value 1 = solid, x fastest: voxels[x + nx*(y + ny*z)]
+ *     (src/inout.cpp:16-36, include/utils.hpp:85-86).
+ *   - compact node ids: corners of solid voxels numbered in ascending grid-linear
+ *     order x + y*(nx+1) + z*(nx+1)*(ny+1) (src/FEM.cpp:88-99,179-184).
+ *   - DOF vectors: 3 per compact node, dof = 3*node + component
+ *     (src/FEM.cpp:580-582).  A fixed node pins all three DOFs
+ *     (include/MatrixFreeSparse.hpp:210-213).
+ *   - ke: 24x24 row-major element stiffness; only its upper triangle is read
+ *     (src/FEM.cpp:200-210).  Local DOF 3*L + c, local node L = x + 2y + 4z.
+ */
+#ifndef VFEM_H
+#define VFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vfem_system vfem_system;
+
+enum vfem_status {
+    VFEM_OK = 0,
+    VFEM_ERR_ARGUMENT = 1,      /* invalid argument (null pointer, size, option) */
+    VFEM_ERR_EMPTY = 2,         /* the voxel model has no solid voxel */
+    VFEM_ERR_DEVICE = 3,        /* a HIP call failed (see vfem_last_error) */
+    VFEM_ERR_SINGULAR = 4,      /* the coarsest-level matrix is not positive definite */
+    VFEM_ERR_NOT_CONVERGED = 5, /* max_iter reached before the tolerance */
+    VFEM_ERR_BREAKDOWN = 6,     /* CG breakdown (p.Ap <= 0 or a non-finite scalar) */
+    VFEM_ERR_UNSUPPORTED = 7    /* option combination not implemented */
+};
+
+/* arithmetic of the multigrid cycle and of the CG matvec */
+enum vfem_precision {
+    VFEM_FP64 = 0,  /* everything FP64 ("plain FP64 MGPCG") */
+    VFEM_FP32 = 1,  /* V-cycle and CG matvec in FP32; CG dots, scalars, x/r updates FP64 */
+    VFEM_BF16 = 2   /* as VFEM_FP32 with the V-cycle's level-0 matvec on BF16 MFMA */
+};
+
+/* multigrid cycle */
+enum vfem_cycle {
+    VFEM_CYCLE_REFERENCE = 0, /* additive cycle of MultigridPreconditioner.hpp:158-193 (omega = 1) */
+    VFEM_CYCLE_DAMPED = 1     /* multiplicative V(1,1) cycle with damped Jacobi (omega) */
+};
+
+typedef struct {
+    double tol;              /* relative residual |r| < tol |f| (FEMGPU.cpp:416,492); default 1e-8 */
+    int32_t max_iter;        /* default 30000 (FEMGPU.cpp:407) */
+    int32_t precision;       /* enum vfem_precision */
+    int32_t cycle;           /* enum vfem_cycle */
+    double omega;            /* damped Jacobi weight (VFEM_CYCLE_DAMPED), default 0.6 */
+    int32_t true_res_every;  /* mixed precision: FP64 true residual every k iterations, default 20 */
+    int32_t check_every;     /* iterations between host convergence polls, default 8 */
+    int32_t use_graph;       /* capture one CG iteration in a hipGraph (default 1) */
+    int32_t warm_start;      /* 1: start from the u passed in instead of zero */
+} vfem_options;
+
+typedef struct {
+    int32_t iterations;      /* completed CG iterations, the converged step excluded (FEMGPU.cpp:471,494) */
+    int32_t converged;
+    double rel_residual;     /* true FP64 |f - A u| / |f| at exit */
+    double rec_residual;     /* recursively updated |r| / |f| at exit */
+    double solve_ms;         /* device time of the solve (hipEvents) */
+    double setup_ms;         /* host + device setup time of vfem_assemble */
+    int32_t true_res_count;  /* FP64 true-residual recomputations done */
+    int32_t status;
+} vfem_stats;
+
+/* Fill *o with the defaults listed above. */
+void vfem_default_options(vfem_options *o);
+
+/* Replaces assembleSystemMatrix<double>(voxelModel, dims, Ke, fixedNodes, numLevels,
+ * skipLevels) (include/FEM.hpp:33-34, src/FEM.cpp:157-517): numbers the nodes,
+ * builds the multigrid hierarchy (levels may stop early, FEM.cpp:465-470) and
+ * uploads it to the current HIP device.  fixed_nodes: compact node ids.
+ * skip_levels: 0 or 1 (FEM.cpp:266-272). */
+int vfem_assemble(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, const double *ke,
+                  const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
+                  vfem_system **out);
+
+/* Releases everything vfem_assemble allocated. */
+void vfem_destroy(vfem_system *sys);
+
+/* MatrixFreeSparse<double>::rows() (include/MatrixFreeSparse.hpp:57): 3 * #nodes. */
+int64_t vfem_num_dofs(const vfem_system *sys);
+
+/* MatrixFreeSparse::numLevels after the early stop (include/MatrixFreeSparse.hpp:86). */
+int32_t vfem_num_levels(const vfem_system *sys);
+
+/* Per-level sizes: elements, used nodes, fixed nodes of level `level`. */
+int vfem_level_info(const vfem_system *sys, int32_t level, int64_t *n_elems, int64_t *n_nodes, int64_t *n_fixed);
+
+/* Replaces applyBoundaryConditions (include/FEM.hpp:63-64, src/FEM.cpp:568-584,
+ * MATRIX_FREE branch): f[3*node + c] = loads[3*k + c] for node = loaded_nodes[k]. */
+int vfem_apply_boundary_conditions(double *f, int64_t n_dofs, const int64_t *loaded_nodes, const double *loads,
+                                   int64_t n_loaded);
+
+/* Replaces solveWithKompute<double>(systemMatrix, f, u) (include/FEMGPU.hpp:11-12,
+ * src/FEMGPU.cpp:32-554) with a device-resident solve: f, u are host arrays of
+ * vfem_num_dofs doubles.  opts may be NULL (defaults); stats may be NULL. */
+int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options *opts, vfem_stats *stats);
+
+/* Multi-RHS block: nrhs load cases on one geometry, f and u are nrhs x ndof row-major
+ * (no reference counterpart; each column equals a single-RHS solve). */
+int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrhs, const vfem_options *opts,
+                       vfem_stats *stats);
+
+/* Resident-data split of vfem_solve for timing without PCIe transfers:
+ * vfem_set_rhs uploads f, vfem_solve_resident solves on the device and keeps u
+ * there, vfem_get_solution downloads it. */
+int vfem_set_rhs(vfem_system *sys, const double *f);
+int vfem_solve_resident(vfem_system *sys, const vfem_options *opts, vfem_stats *stats);
+int vfem_get_solution(vfem_system *sys, double *u);
+
+/* Replaces the matrix-free product y = A x with fixed rows zeroed
+ * (include/MatrixFreeSparse.hpp:191-215; GPU: shaders/MatxVec.comp +
+ * shaders/FixedNodes.comp).  precision selects the kernel (vfem_precision). */
+int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision);
+
+/* Replaces MultigridPreconditioner::solve (include/MultigridPreconditioner.hpp:158-193;
+ * GPU: FEMGPU.cpp:500-534): z = M r with the cycle/precision of opts. */
+int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_options *opts);
+
+/* Replaces shaders/VecDotVec.comp: *out = a . b over vfem_num_dofs entries (FP64). */
+int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
+
+/* Kernel timing hook (no reference counterpart): runs `reps` launches of one
+ * kernel on the library's stream between two hipEvents and returns the mean
+ * duration.  which: 0 = level-0 matvec, 1 = one V-cycle, 2 = one CG iteration. */
+int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);
+
+/* Host-only inspection of the hierarchy vfem_assemble builds (no device call; no
+ * reference counterpart -- a test hook for the setup logic of FEM.cpp:157-517):
+ * *n_levels_out = levels after the early stop; for `level`, the used-node count,
+ * the fixed-node count and, when the pointers are non-null, the inverse diagonal
+ * (3 per node, compact order, FEM.cpp:104-151,463) and the fixed-node mask
+ * (1 byte per node).  free_dofs_out (non-null, level = last) receives the count
+ * of free coarsest DOFs (FEM.cpp:473-496). */
+int vfem_inspect_hierarchy(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, const double *ke,
+                           const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
+                           int32_t level, int32_t *n_levels_out, int64_t *n_nodes, int64_t *n_fixed_out,
+                           double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out);
+
+/* Name of a status code, and the last error message of the calling thread. */
+const char *vfem_status_string(int status);
+const char *vfem_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFEM_H */

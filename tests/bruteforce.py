@@ -1,0 +1,143 @@
+"""Brute-force restatements used to pin the oracle: explicit sparse matrices
+built with numpy/scipy from the definitions (global triplet assembly, explicit
+trilinear prolongation matrices), sharing no code with oracle/vfem_oracle.c or
+the HIP library.  Small grids only."""
+import numpy as np
+import scipy.sparse as sp
+
+from vulkanfem_amd import problems
+
+
+def sym_upper(ke):
+    return np.triu(ke) + np.triu(ke, 1).T
+
+
+def assemble(occ, ke, scale=1.0):
+    """Global K over the elements occ == 1 (local node L = x + 2y + 4z)."""
+    ids = problems.compact_ids(occ)
+    ez, ey, ex = np.nonzero(np.asarray(occ) == 1)
+    nodes = np.stack([ids[ez + ((L >> 2) & 1), ey + ((L >> 1) & 1), ex + (L & 1)] for L in range(8)], axis=1)
+    dofs = (3 * nodes[:, :, None] + np.arange(3)[None, None, :]).reshape(-1, 24)
+    rows = np.repeat(dofs, 24, axis=1).reshape(-1)
+    cols = np.tile(dofs, (1, 24)).reshape(-1)
+    vals = np.tile(scale * ke.reshape(-1), dofs.shape[0])
+    n = 3 * int(ids.max() + 1)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def coarsen(occ, div=2):
+    """Coarse elements: floor(fine / div), present if any child is (FEM.cpp:274-283)."""
+    occ = np.asarray(occ) == 1
+    nz, ny, nx = occ.shape
+    cz, cy, cx = (nz + div - 1) // div, (ny + div - 1) // div, (nx + div - 1) // div
+    pad = np.zeros((cz * div, cy * div, cx * div), bool)
+    pad[:nz, :ny, :nx] = occ
+    return pad.reshape(cz, div, cy, div, cx, div).any(axis=(1, 3, 5)).astype(np.int32)
+
+
+def node_coords(occ):
+    ids = problems.compact_ids(occ)
+    z, y, x = np.nonzero(ids >= 0)
+    order = np.argsort(ids[z, y, x])
+    return np.stack([x[order], y[order], z[order]], axis=1), ids
+
+
+def prolongation(fine_occ, coarse_occ, div=2):
+    """Explicit trilinear P (fine DOFs x coarse DOFs) on used nodes:
+    P[f, N] = prod_d (1 - |f_d - div N_d| / div) for |f - div N|_inf < div."""
+    fc, _ = node_coords(fine_occ)
+    _, cids = node_coords(coarse_occ)
+    rows, cols, vals = [], [], []
+    for i, (x, y, z) in enumerate(fc):
+        cand = []
+        for p in (x, y, z):
+            lo = p // div
+            cand.append([lo] if p % div == 0 else [lo, lo + 1])
+        for Z in cand[2]:
+            for Y in cand[1]:
+                for X in cand[0]:
+                    if Z >= cids.shape[0] or Y >= cids.shape[1] or X >= cids.shape[2]:
+                        continue
+                    j = cids[Z, Y, X]
+                    if j < 0:
+                        continue
+                    w = (1 - abs(x - div * X) / div) * (1 - abs(y - div * Y) / div) * (1 - abs(z - div * Z) / div)
+                    if w > 0:
+                        rows.append(i)
+                        cols.append(j)
+                        vals.append(w)
+    Pn = sp.csr_matrix((vals, (rows, cols)), shape=(fc.shape[0], int(cids.max() + 1)))
+    return sp.kron(Pn, sp.identity(3), format="csr")
+
+
+def fixed_dof_mask(nnodes, fixed_nodes):
+    m = np.zeros(3 * nnodes, bool)
+    for c in range(3):
+        m[3 * np.asarray(fixed_nodes, np.int64) + c] = True
+    return m
+
+
+def coarse_fixed(fine_occ, coarse_occ, fixed_fine, div=2):
+    """Coarse nodes with a fixed fine node strictly inside their support."""
+    fc, _ = node_coords(fine_occ)
+    _, cids = node_coords(coarse_occ)
+    out = set()
+    for f in np.asarray(fixed_fine, np.int64):
+        x, y, z = fc[f]
+        cand = []
+        for p in (x, y, z):
+            lo = p // div
+            cand.append([lo] if p % div == 0 else [lo, lo + 1])
+        for Z in cand[2]:
+            for Y in cand[1]:
+                for X in cand[0]:
+                    if Z < cids.shape[0] and Y < cids.shape[1] and X < cids.shape[2] and cids[Z, Y, X] >= 0:
+                        out.add(int(cids[Z, Y, X]))
+    return np.array(sorted(out), np.int64)
+
+
+class Hierarchy:
+    """Explicit-matrix hierarchy: occupancies, P_l, fixed sets, the level
+    operators (level 0: K; level l >= 1: elements of level l-1 taken full with
+    2^(l-1) Ke, Galerkin-projected -- the reference's unique-K rule,
+    FEM.cpp:364-395,463) and the reference coarsest matrix Kc (FEM.cpp:513)."""
+
+    def __init__(self, voxels, ke, fixed, nlev):
+        ke = sym_upper(ke)
+        self.occ = [np.asarray(voxels, np.int32)]
+        self.fixed = [np.asarray(fixed, np.int64)]
+        self.P = []
+        self.A = [assemble(self.occ[0], ke)]
+        for l in range(1, nlev):
+            c = coarsen(self.occ[-1])
+            P = prolongation(self.occ[-1], c)
+            self.P.append(P)
+            fin = self.A[0] if l == 1 else assemble(self.occ[-1], ke, 2.0 ** (l - 1))
+            self.A.append((P.T @ fin @ P).tocsr())
+            self.fixed.append(coarse_fixed(self.occ[-1], c, self.fixed[-1]))
+            self.occ.append(c)
+        self.nlev = nlev
+        self.kc_full = assemble(self.occ[-1], ke, 2.0 ** (nlev - 1)).toarray()
+        nn = self.kc_full.shape[0] // 3
+        self.free = np.nonzero(~fixed_dof_mask(nn, self.fixed[-1]))[0]
+
+    def invdiag(self, l):
+        return 1.0 / self.A[l].diagonal()
+
+    def kc(self):
+        return self.kc_full[np.ix_(self.free, self.free)]
+
+    def vcycle_additive(self, r):
+        """z = Pi [sum_l P_{0<-l} 2 D_l^-1 R_{l<-0} r + P_{0<-L-1} Kc^-1 R r], R = P^T
+        (MultigridPreconditioner.hpp:158-193 with mu cancelled)."""
+        L = self.nlev
+        rs = [r]
+        for l in range(L - 1):
+            rs.append(self.P[l].T @ rs[-1])
+        xc = np.zeros_like(rs[-1])
+        xc[self.free] = np.linalg.solve(self.kc(), rs[-1][self.free])
+        x = xc
+        for l in range(L - 2, -1, -1):
+            x = self.P[l] @ x + 2 * self.invdiag(l) * rs[l]
+        x[fixed_dof_mask(len(x) // 3, self.fixed[0])] = 0
+        return x

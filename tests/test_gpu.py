@@ -1,0 +1,182 @@
+"""GPU parity: the HIP library, called through the C-ABI, against the CPU oracle
+on the same seeded inputs.
+
+Tolerances: FP64 kernels agree with the oracle to rounding (1e-12 relative);
+FP32 kernels to 1e-5 relative; full solves must reach the requested true
+relative residual and agree with the FP64 oracle/direct solution to 1e-6
+relative L2 displacement error."""
+import numpy as np
+import pytest
+
+import vulkanfem_amd as V
+from oracle import oracle as O
+from vulkanfem_amd import problems
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+
+
+def geometry(n, cnt, a0, a1, seed):
+    vox, _ = problems.ellipsoid_union(n, cnt, a0, a1, seed)
+    ids = problems.compact_ids(vox)
+    return vox, np.sort(ids[0][ids[0] >= 0])
+
+
+CASES = {
+    "box": (np.ones((16, 16, 32), np.int32), None, 3),
+    "ell14": geometry(14, 8, 2.0, 6.0, 32) + (3,),
+    "ell24": geometry(24, 10, 3.0, 9.0, 7) + (3,),
+    "ell33": geometry(33, 12, 3.0, 11.0, 3) + (4,),
+}
+
+
+@pytest.fixture(scope="module")
+def systems(ke0, goldens):
+    out = {}
+    for name, (vox, fixed, nl) in CASES.items():
+        if fixed is None:
+            fixed = goldens["box_fixed"]
+        say("assemble", name, vox.shape)
+        out[name] = (V.assemble_system_matrix(vox, ke0, fixed, nl), O.OracleSystem(vox, ke0, fixed, nl), vox, fixed)
+    return out
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_matvec_parity(systems, name):
+    A, ora, _, _ = systems[name]
+    x = np.random.default_rng(1).standard_normal(A.n)
+    say("matvec", name)
+    ref = ora.matvec(x)
+    assert rel(A.matvec(x, V.FP64), ref) < 1e-13
+    assert rel(A.matvec(x, V.FP32), ref) < 2e-6
+
+
+def test_matvec_golden(goldens, ke0):
+    for k in range(3):
+        say("matvec golden", k)
+        A = V.assemble_system_matrix(goldens[f"mv{k}_voxels"], ke0, goldens[f"mv{k}_fixed"], 2)
+        assert rel(A.matvec(goldens[f"mv{k}_x"]), goldens[f"mv{k}_y"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_precondition_parity(systems, name):
+    A, ora, _, _ = systems[name]
+    r = np.random.default_rng(2).standard_normal(A.n)
+    say("precondition ref", name)
+    z_ref = ora.vcycle(r, O.REF_ADDITIVE)
+    assert rel(V.MultigridPreconditioner(A).solve(r), z_ref) < 1e-12
+    assert rel(V.MultigridPreconditioner(A, precision=V.FP32).solve(r), z_ref) < 1e-5
+    say("precondition damped", name)
+    mask = np.repeat(ora.fixed(0), 3)
+    r[mask] = 0
+    z_d = ora.vcycle(r, O.DAMPED, 0.6)
+    assert rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r), z_d) < 1e-12
+    assert rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.FP32).solve(r), z_d) < 1e-5
+
+
+def test_dot(systems):
+    A, _, _, _ = systems["ell24"]
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    assert abs(A.dot(a, b) - a @ b) < 1e-12 * np.abs(a * b).sum()
+
+
+def test_box_solve_matches_direct_solution(systems, goldens):
+    A, ora, _, _ = systems["box"]
+    f, u_direct = goldens["box_f"], goldens["box_u"]
+    for cycle in (V.CYCLE_REFERENCE, V.CYCLE_DAMPED):
+        for prec in (V.FP64, V.FP32):
+            say("box solve", cycle, prec)
+            u, st = A.solve(f, tol=1e-10, precision=prec, cycle=cycle)
+            assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
+            assert rel(u, u_direct) < 1e-6, (cycle, prec, rel(u, u_direct))
+    # FP64 reference cycle: same iteration count as the oracle (Eigen semantics)
+    _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.REF_ADDITIVE)
+    _, st = A.solve(f, tol=1e-8)
+    assert st["iterations"] == it_ora
+    _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.DAMPED, omega=0.6)
+    _, st = A.solve(f, tol=1e-8, cycle=V.CYCLE_DAMPED)
+    assert st["iterations"] == it_ora
+
+
+@pytest.mark.parametrize("name", ["ell24", "ell33"])
+def test_solve_parity(systems, name):
+    A, ora, vox, fixed = systems[name]
+    ids = problems.compact_ids(vox)
+    zs = np.nonzero((ids >= 0).any(axis=(1, 2)))[0]
+    top = ids[zs.max()]
+    f = np.zeros(A.n)
+    f[3 * top[top >= 0] + 2] = -1.0 / np.count_nonzero(top >= 0)
+    u_ref, _, _ = ora.pcg(f, tol=1e-13, precond=O.DAMPED)
+    for cycle in (V.CYCLE_REFERENCE, V.CYCLE_DAMPED):
+        for prec in (V.FP64, V.FP32):
+            say("solve", name, cycle, prec)
+            u, st = A.solve(f, tol=1e-10, precision=prec, cycle=cycle)
+            assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
+            assert rel(u, u_ref) < 1e-6
+
+
+def test_edge_cases(ke0):
+    box = np.ones((4, 4, 8), np.int32)
+    ids = problems.compact_ids(box)
+    fixed = np.sort(ids[:, :, 0].reshape(-1))
+    f = np.zeros(3 * int(ids.max() + 1))
+    f[3 * ids[:, :, 8].reshape(-1) + 2] = -1.0
+    say("edge: single level")
+    A1 = V.assemble_system_matrix(box, ke0, fixed, 1)
+    assert A1.num_levels == 1
+    u, st = A1.solve(f, tol=1e-10)
+    assert st["iterations"] == 0 and st["rel_residual"] < 1e-10
+    say("edge: zero rhs")
+    A2 = V.assemble_system_matrix(box, ke0, fixed, 2)
+    u, st = A2.solve(np.zeros_like(f))
+    assert st["iterations"] == 0 and not u.any() and st["converged"] == 1
+    say("edge: iteration cap")
+    u, st = A2.solve(f, tol=1e-15, max_iter=3)
+    assert st["iterations"] == 3 and st["converged"] == 0
+    assert st["status"] == V.api.ERR_NOT_CONVERGED
+    say("edge: warm start from the solution")
+    u, st = A2.solve(f, tol=1e-10)
+    u2, st2 = A2.solve(f, u=u.copy(), tol=1e-8, warm_start=1)
+    assert st2["iterations"] == 0 and rel(u2, u) < 1e-12
+    say("edge: early level stop")
+    A3 = V.assemble_system_matrix(box, ke0, fixed, 6)
+    assert A3.num_levels == 2  # (8,4,4) -> (4,2,2): a dim <= 2 stops the hierarchy (FEM.cpp:465-470)
+    say("edge: bad options")
+    with pytest.raises(V.VfemError):
+        A2.solve(f, precision=7)
+
+
+def test_skip_level_solve(ke0):
+    vox, fixed = geometry(24, 10, 3.0, 9.0, 7)
+    say("skip level")
+    A = V.assemble_system_matrix(vox, ke0, fixed, 3, skip_levels=1)
+    ora = O.OracleSystem(vox, ke0, fixed, 3, skip=1)
+    r = np.random.default_rng(4).standard_normal(A.n)
+    assert rel(V.MultigridPreconditioner(A).solve(r), ora.vcycle(r, O.REF_ADDITIVE)) < 1e-12
+    f = np.zeros(A.n)
+    f[2::3] = -1.0 / (A.n // 3)
+    u, st = A.solve(f, tol=1e-10)
+    u_ref, it, _ = ora.pcg(f, tol=1e-10)
+    assert st["converged"] == 1 and rel(u, u_ref) < 1e-8 and st["iterations"] == it
+    with pytest.raises(V.VfemError) as e:
+        A.solve(f, cycle=V.CYCLE_DAMPED)
+    assert e.value.status == V.api.ERR_UNSUPPORTED
+
+
+def test_no_graph_path_matches_graph(systems):
+    A, _, _, _ = systems["ell24"]
+    f = np.zeros(A.n)
+    f[2::3] = -1.0 / (A.n // 3)
+    say("graph vs stream")
+    u1, s1 = A.solve(f, tol=1e-9, use_graph=1)
+    u2, s2 = A.solve(f, tol=1e-9, use_graph=0)
+    assert s1["iterations"] == s2["iterations"]
+    assert np.array_equal(u1, u2)  # deterministic reductions: bitwise identical

@@ -1,0 +1,111 @@
+"""Host-side checks (CPU only): the C-ABI library loads and exports every symbol
+of include/vfem.h, the product's host hierarchy setup equals the oracle's, and
+the problem generators are deterministic with the documented sizes."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import vulkanfem_amd as V
+from oracle import oracle as O
+from vulkanfem_amd import problems
+
+
+def test_library_exports_every_header_symbol():
+    names = V.header_functions()
+    assert "vfem_solve" in names and "vfem_assemble" in names and len(names) >= 18
+    L = ctypes.CDLL(V.api.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_defaults():
+    L = V.lib()
+    assert L.vfem_status_string(0) == b"ok"
+    assert L.vfem_status_string(5) == b"not converged"
+    o = V.options()
+    assert o.tol == 1e-8 and o.max_iter == 30000 and o.omega == 0.6 and o.true_res_every == 20
+
+
+def _cases():
+    out = []
+    for n, cnt, a0, a1, seed in [(12, 6, 2.0, 6.0, 22), (14, 8, 2.0, 6.0, 32), (15, 8, 2.0, 7.0, 43)]:
+        vox, _ = problems.ellipsoid_union(n, cnt, a0, a1, seed)
+        ids = problems.compact_ids(vox)
+        out.append((vox, np.sort(ids[0][ids[0] >= 0])))
+    return out
+
+
+@pytest.mark.parametrize("case", range(3))
+@pytest.mark.parametrize("skip", [0, 1])
+def test_host_hierarchy_matches_oracle(ke0, case, skip):
+    vox, fixed = _cases()[case]
+    s = O.OracleSystem(vox, ke0, fixed, 4, skip=skip)
+    h0 = V.inspect_hierarchy(vox, ke0, fixed, 4, skip, 0)
+    assert h0["levels"] == s.num_levels
+    for l in range(s.num_levels):
+        h = V.inspect_hierarchy(vox, ke0, fixed, 4, skip, l)
+        assert h["nodes"] == s.nodes(l)
+        assert np.array_equal(h["fixed"], s.fixed(l))
+        np.testing.assert_allclose(h["invdiag"], s.invdiag(l), rtol=1e-13)
+    assert h0["free_dofs"] == s.free_dofs().size
+
+
+def test_host_hierarchy_rejects_bad_input(ke0):
+    vox = np.zeros((4, 4, 4), np.int32)
+    with pytest.raises(V.VfemError) as e:
+        V.inspect_hierarchy(vox, ke0, [], 2, 0, 0)
+    assert e.value.status == V.api.ERR_EMPTY
+    vox[:] = 1
+    with pytest.raises(V.VfemError) as e:
+        V.inspect_hierarchy(vox, ke0, [10 ** 6], 2, 0, 0)
+    assert e.value.status == V.api.ERR_ARGUMENT
+    with pytest.raises(V.VfemError):
+        V.inspect_hierarchy(vox, ke0, [0], 2, 2, 0)  # skip_levels must be 0 or 1
+    # no fixed node: the coarsest matrix is singular
+    with pytest.raises(V.VfemError) as e:
+        V.inspect_hierarchy(vox, ke0, [], 2, 0, 0)
+    assert e.value.status == V.api.ERR_SINGULAR
+
+
+def test_ke0_generator_matches_reference_data(ke0):
+    assert np.abs(problems.ke0(1.0, 0.3) - ke0).max() < 1e-15
+    assert np.abs(problems.ke0(3.0, 0.25) - O.ke0_closed_form(3.0, 0.25)).max() < 1e-14
+
+
+def test_cantilever_generator():
+    p = problems.cantilever_box(32, 16, 16)
+    assert p.ndof == 28611 and p.fixed.size == 289
+    f = p.loads[0]
+    assert abs(f.sum() + 1.0) < 1e-12 and np.count_nonzero(f) == 289
+    assert np.all(f[2::3][f[2::3] != 0] < 0)
+
+
+def test_ellipsoid_generator_is_deterministic():
+    a, ia = problems.ellipsoid_union(40, 10, 3.0, 10.0, 7)
+    b, ib = problems.ellipsoid_union(40, 10, 3.0, 10.0, 7)
+    c, _ = problems.ellipsoid_union(40, 10, 3.0, 10.0, 8)
+    assert np.array_equal(a, b) and ia == ib and not np.array_equal(a, c)
+    p = problems.ellipsoid_problem(40, 10, 3.0, 10.0, 7)
+    f = p.loads[0]
+    assert abs(f.sum() + 1.0) < 1e-12
+    ids = problems.compact_ids(p.voxels)
+    assert np.array_equal(p.fixed, np.sort(ids[0][ids[0] >= 0]))
+
+
+def test_patch_load_cases():
+    vox, _ = problems.ellipsoid_union(24, 6, 3.0, 8.0, 5)
+    F = problems.patch_load_cases(vox, 4, seed=9)
+    G = problems.patch_load_cases(vox, 4, seed=9)
+    assert np.array_equal(F, G) and F.shape[0] == 4
+    for k in range(4):
+        tot = F[k].reshape(-1, 3).sum(axis=0)
+        assert abs(np.linalg.norm(tot) - 1.0) < 1e-12
+
+
+def test_apply_boundary_conditions():
+    f = np.zeros(12)
+    V.apply_boundary_conditions(f, {1: (1.0, 2.0, 3.0), 3: (0.0, 0.0, -1.0)})
+    assert list(f) == [0, 0, 0, 1, 2, 3, 0, 0, 0, 0, 0, -1]
+    with pytest.raises(V.VfemError):
+        V.apply_boundary_conditions(np.zeros(6), {5: (1.0, 0.0, 0.0)})

@@ -1,0 +1,168 @@
+"""The CPU oracle (oracle/vfem_oracle.c) pinned against the reference's own data
+files, closed forms and brute-force explicit-matrix restatements (CPU only)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import bruteforce as BF
+from vulkanfem_amd import problems
+
+
+def test_ke0_closed_form_matches_reference_data(ke0):
+    # data/Ke0.dat is the E=1, nu=0.3 trilinear hex8 matrix in the rotated DOF frame
+    assert np.abs(O.ke0_closed_form(1.0, 0.3) - ke0).max() < 1e-15
+    # closed-form invariants: symmetric, rigid translations in the null space
+    K = O.ke0_closed_form(2.5, 0.2)
+    assert np.allclose(K, K.T, atol=1e-15)
+    for c in range(3):
+        t = np.zeros(24)
+        t[c::3] = 1
+        assert np.abs(K @ t).max() < 1e-14
+    assert np.sum(np.abs(np.linalg.eigvalsh(K)) < 1e-12) == 6
+
+
+def test_transfer_tables_match_reference(tables):
+    assert np.array_equal(O.transfer_weights(2), tables["r27"])
+    assert np.array_equal(O.transfer_weights(4), tables["r125"])
+    assert np.array_equal(O.child_lattice_map(2), tables["map8"])
+    assert np.array_equal(O.child_lattice_map(4), tables["map64"])
+    assert np.array_equal(np.kron(O.transfer_weights(2), np.eye(3)), tables["rdof"])
+    assert np.array_equal(np.kron(O.transfer_weights(4), np.eye(3)), tables["rdof4"])
+
+
+@pytest.mark.parametrize("k", [0, 1, 2])
+def test_matvec_matches_scipy_assembly(goldens, ke0, k):
+    s = O.OracleSystem(goldens[f"mv{k}_voxels"], ke0, goldens[f"mv{k}_fixed"], 2)
+    y = s.matvec(goldens[f"mv{k}_x"])
+    ref = goldens[f"mv{k}_y"]
+    assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_matvec_rigid_modes_and_symmetry(ke0):
+    vox, _ = problems.ellipsoid_union(12, 6, 2.0, 6.0, 22)
+    ids = problems.compact_ids(vox)
+    # rigid translations are in the null space of K, so also of K with fixed rows zeroed
+    s = O.OracleSystem(vox, ke0, np.sort(ids[0][ids[0] >= 0]), 2)
+    for c in range(3):
+        t = np.zeros(s.ndof)
+        t[c::3] = 1
+        assert np.abs(s.matvec(t)).max() < 1e-13
+    # symmetric on vectors that vanish at fixed DOFs (the CG subspace)
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    mask = np.repeat(s.fixed(0), 3)
+    a[mask] = 0
+    b[mask] = 0
+    assert abs(a @ s.matvec(b) - b @ s.matvec(a)) < 1e-12 * abs(a @ s.matvec(b))
+
+
+@pytest.fixture(scope="module")
+def small_case(ke0):
+    vox, _ = problems.ellipsoid_union(14, 8, 2.0, 6.0, 32)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    return vox, fixed
+
+
+def test_hierarchy_matches_brute_force(ke0, small_case):
+    vox, fixed = small_case
+    s = O.OracleSystem(vox, ke0, fixed, 3)
+    bf = BF.Hierarchy(vox, ke0, fixed, 3)
+    assert s.num_levels == 3
+    for l in range(3):
+        assert s.nodes(l) == bf.A[l].shape[0] // 3
+        # invDiag: level 0 = diag(K); level 1 = diag(P^T K P) (Galerkin, exact);
+        # level 2 = Galerkin of the level-1 elements taken full (the reference's unique-K rule)
+        np.testing.assert_allclose(s.invdiag(l), bf.invdiag(l), rtol=1e-13)
+        fx = np.zeros(s.nodes(l), bool)
+        fx[bf.fixed[l]] = True
+        assert np.array_equal(s.fixed(l), fx)
+    np.testing.assert_allclose(s.kc(0), bf.kc(), rtol=1e-12, atol=1e-12 * np.abs(bf.kc()).max())
+
+
+def test_transfers_adjoint_and_reproduce_linear_fields(ke0, small_case):
+    vox, fixed = small_case
+    s = O.OracleSystem(vox, ke0, fixed, 3)
+    rng = np.random.default_rng(1)
+    for l in range(2):
+        r = rng.standard_normal(3 * s.nodes(l))
+        xc = rng.standard_normal(3 * s.nodes(l + 1))
+        assert abs(s.restrict(l, r) @ xc - r @ s.interpolate(l, xc)) < 1e-12 * np.abs(r).sum()
+    # interpolation of a coarse linear field is the fine linear field (pins mu = 1/multiplicity)
+    bf = BF.Hierarchy(vox, ke0, fixed, 2)
+    fc, _ = BF.node_coords(bf.occ[0])
+    cc, _ = BF.node_coords(bf.occ[1])
+    a = np.array([0.3, -1.2, 0.7])
+    xc = np.repeat(cc @ a + 2.0, 3)
+    xf = s.interpolate(0, xc)
+    np.testing.assert_allclose(xf, np.repeat(fc @ a / 2 + 2.0, 3), rtol=1e-13, atol=1e-13)
+
+
+def test_additive_vcycle_matches_explicit_formula(ke0, small_case):
+    vox, fixed = small_case
+    s = O.OracleSystem(vox, ke0, fixed, 3)
+    bf = BF.Hierarchy(vox, ke0, fixed, 3)
+    r = np.random.default_rng(2).standard_normal(s.ndof)
+    z = s.vcycle(r, O.REF_ADDITIVE)
+    zb = bf.vcycle_additive(r)
+    np.testing.assert_allclose(z, zb, rtol=1e-10, atol=1e-12 * np.abs(zb).max())
+
+
+def test_damped_vcycle_is_symmetric_positive(ke0, small_case):
+    vox, fixed = small_case
+    s = O.OracleSystem(vox, ke0, fixed, 3)
+    rng = np.random.default_rng(3)
+    mask = np.repeat(s.fixed(0), 3)
+    a, b = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    a[mask] = 0
+    b[mask] = 0
+    Ma, Mb = s.vcycle(a, O.DAMPED, 0.6), s.vcycle(b, O.DAMPED, 0.6)
+    assert abs(b @ Ma - a @ Mb) < 1e-10 * abs(a @ Mb)
+    assert a @ Ma > 0 and b @ Mb > 0
+
+
+@pytest.mark.parametrize("mode", [O.REF_ADDITIVE, O.REF_ADDITIVE_FP32, O.DAMPED, O.IDENTITY])
+def test_pcg_box_matches_direct_solve(goldens, ke0, mode):
+    s = O.OracleSystem(np.ones((16, 16, 32), np.int32), ke0, goldens["box_fixed"], 3)
+    assert [(s.nodes(l), s.elems(l), s.nfixed(l)) for l in range(3)] == [(9537, 8192, 289), (1377, 1024, 81),
+                                                                          (225, 128, 25)]
+    assert s.free_dofs().size == 600
+    x, it, rr = s.pcg(goldens["box_f"], tol=1e-12, precond=mode)
+    assert rr < 1e-12
+    err = np.linalg.norm(x - goldens["box_u"]) / np.linalg.norm(goldens["box_u"])
+    assert err < 1e-9, err
+
+
+def test_pcg_eigen_semantics(ke0, goldens):
+    box = np.ones((4, 4, 8), np.int32)
+    ids = problems.compact_ids(box)
+    fixed = np.sort(ids[:, :, 0].reshape(-1))
+    b = np.zeros(3 * int(ids.max() + 1))
+    b[3 * ids[:, :, 8].reshape(-1) + 2] = -1.0
+    # one level: the cycle is the exact inverse -> converged in the first step, 0 iterations counted
+    s1 = O.OracleSystem(box, ke0, fixed, 1)
+    x, it, rr = s1.pcg(b, tol=1e-10)
+    assert it == 0 and rr < 1e-10
+    # zero right-hand side: x = 0 and no iteration
+    x, it, rr = s1.pcg(np.zeros_like(b))
+    assert it == 0 and not x.any()
+    # iteration cap
+    s3 = O.OracleSystem(box, ke0, fixed, 2)
+    x, it, rr = s3.pcg(b, tol=1e-14, max_iter=2, precond=O.IDENTITY)
+    assert it == 2 and rr > 1e-14
+
+
+def test_skip_level_hierarchy(ke0):
+    vox, _ = problems.ellipsoid_union(16, 8, 2.0, 7.0, 41)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    s = O.OracleSystem(vox, ke0, fixed, 3, skip=1)
+    # level 1 is a x4 coarsening with the 125-node lattice (FEM.cpp:266-272)
+    assert s.rmap(1).shape[1] == 125
+    assert s.extent(1) == (4, 4, 4)
+    fine = BF.Hierarchy(vox, ke0, fixed, 1)
+    c = BF.coarsen(vox, 4)
+    P = BF.prolongation(vox, c, 4)
+    np.testing.assert_allclose(s.invdiag(1), 1.0 / (P.T @ fine.A[0] @ P).diagonal(), rtol=1e-12)
+    r = np.random.default_rng(5).standard_normal(s.ndof)
+    np.testing.assert_allclose(s.restrict(0, r), P.T @ r, rtol=1e-12, atol=1e-12)

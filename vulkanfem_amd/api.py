@@ -1,0 +1,313 @@
+"""Python host interface over the C-ABI of include/vfem.h (libvfem.so).
+
+Mirrors the reference's solver interface for the GPU path:
+
+    reference (hirademirkol/VulkanFEM)                       here
+    assembleSystemMatrix<double>(...)  FEM.hpp:33-34         assemble_system_matrix(...)
+    MatrixFreeSparse<double>           MatrixFreeSparse.hpp  MatrixFreeSparse
+    applyBoundaryConditions(...)       FEM.hpp:63-64         apply_boundary_conditions(...)
+    solveWithKompute<double>(A, f, u)  FEMGPU.hpp:11-12      solve_with_hip(A, f, u)
+    MultigridPreconditioner            MultigridPreconditioner.hpp  MultigridPreconditioner
+
+The library has no CPU fallback: importing works anywhere, but every call that
+needs the device raises VfemError when the HIP library is missing or fails.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvfem.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "vfem.h")
+
+FP64, FP32, BF16 = 0, 1, 2
+CYCLE_REFERENCE, CYCLE_DAMPED = 0, 1
+
+OK, ERR_ARGUMENT, ERR_EMPTY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_UNSUPPORTED = range(8)
+
+
+class VfemError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"vfem status {status}: {message}")
+        self.status = status
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("cycle", ctypes.c_int32), ("omega", ctypes.c_double), ("true_res_every", ctypes.c_int32),
+                ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_residual", ctypes.c_double),
+                ("rec_residual", ctypes.c_double), ("solve_ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("true_res_count", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+def header_functions():
+    """Names of every function declared in include/vfem.h."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vfem_\w+)\s*\(", text)))
+
+
+def lib():
+    """Load libvfem.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VfemError(ERR_DEVICE, f"{LIB_PATH} is not built; run `python -m vulkanfem_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.vfem_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.vfem_default_options.restype = None
+    L.vfem_assemble.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f64p, _i64p, ctypes.c_int64,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]
+    L.vfem_destroy.argtypes = [vp]
+    L.vfem_destroy.restype = None
+    L.vfem_num_dofs.argtypes = [vp]
+    L.vfem_num_dofs.restype = ctypes.c_int64
+    L.vfem_num_levels.argtypes = [vp]
+    L.vfem_num_levels.restype = ctypes.c_int32
+    L.vfem_level_info.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int64)] * 3
+    L.vfem_apply_boundary_conditions.argtypes = [_f64p, ctypes.c_int64, _i64p, _f64p, ctypes.c_int64]
+    L.vfem_solve.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options), ctypes.POINTER(Stats)]
+    L.vfem_solve_batched.argtypes = [vp, _f64p, _f64p, ctypes.c_int32, ctypes.POINTER(Options),
+                                     ctypes.POINTER(Stats)]
+    L.vfem_set_rhs.argtypes = [vp, _f64p]
+    L.vfem_solve_resident.argtypes = [vp, ctypes.POINTER(Options), ctypes.POINTER(Stats)]
+    L.vfem_get_solution.argtypes = [vp, _f64p]
+    L.vfem_matvec.argtypes = [vp, _f64p, _f64p, ctypes.c_int32]
+    L.vfem_precondition.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options)]
+    L.vfem_dot.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_double)]
+    L.vfem_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_double)]
+    L.vfem_inspect_hierarchy.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f64p, _i64p,
+                                         ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                                         ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_int64)]
+    L.vfem_status_string.argtypes = [ctypes.c_int]
+    L.vfem_status_string.restype = ctypes.c_char_p
+    L.vfem_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise VfemError(rc, lib().vfem_last_error().decode())
+    return rc
+
+
+def options(**kw):
+    o = Options()
+    lib().vfem_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def _prep(voxels, ke, fixed):
+    vox = np.ascontiguousarray(voxels, dtype=np.int32)
+    if vox.ndim != 3:
+        raise ValueError("voxels must be a (nz, ny, nx) array, x fastest")
+    ke = np.ascontiguousarray(ke, dtype=np.float64).reshape(576)
+    fixed = np.ascontiguousarray(np.asarray(sorted(set(int(v) for v in np.asarray(fixed).reshape(-1)))),
+                                 dtype=np.int64).reshape(-1)
+    return vox, ke, fixed
+
+
+class MatrixFreeSparse:
+    """The assembled system: MatrixFreeSparse<double> plus its multigrid data
+    (MatrixFreeSparse.hpp:31-94), resident on the GPU."""
+
+    def __init__(self, voxels, ke, fixed_nodes, num_levels=3, skip_levels=0):
+        vox, ke, fixed = _prep(voxels, ke, fixed_nodes)
+        nz, ny, nx = vox.shape
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.vfem_assemble(vox.reshape(-1), nx, ny, nz, ke, fixed, fixed.size, int(num_levels),
+                               int(skip_levels), ctypes.byref(h)))
+        self._h = h
+        self._lib = L
+        self.num_levels = int(L.vfem_num_levels(h))
+        self.skip_levels = int(skip_levels)
+        self.n = int(L.vfem_num_dofs(h))
+
+    def rows(self):
+        return self.n
+
+    def cols(self):
+        return self.n
+
+    @property
+    def numElements(self):  # the reference's name for the DOF count (FEM.cpp:186,221)
+        return self.n
+
+    def level_info(self, level):
+        ne, nn, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.vfem_level_info(self._h, int(level), ctypes.byref(ne), ctypes.byref(nn), ctypes.byref(nf)))
+        return {"elements": ne.value, "nodes": nn.value, "fixed": nf.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vfem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __matmul__(self, x):
+        return self.matvec(x)
+
+    def matvec(self, x, precision=FP64):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (self.n,):
+            raise ValueError("x has the wrong size")
+        y = np.zeros(self.n)
+        _check(self._lib.vfem_matvec(self._h, x, y, int(precision)))
+        return y
+
+    def dot(self, a, b):
+        out = ctypes.c_double()
+        _check(self._lib.vfem_dot(self._h, np.ascontiguousarray(a, np.float64), np.ascontiguousarray(b, np.float64),
+                                  ctypes.byref(out)))
+        return out.value
+
+    def solve(self, f, u=None, **opts):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if u is None:
+            u = np.zeros(self.n)
+        st = Stats()
+        o = options(**opts)
+        rc = self._lib.vfem_solve(self._h, f, u, ctypes.byref(o), ctypes.byref(st))
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _check(rc)
+        return u, st.as_dict()
+
+    def solve_batched(self, F, **opts):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        nrhs = F.shape[0]
+        U = np.zeros_like(F)
+        st = (Stats * nrhs)()
+        o = options(**opts)
+        rc = self._lib.vfem_solve_batched(self._h, F.reshape(-1), U.reshape(-1), nrhs, ctypes.byref(o), st)
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _check(rc)
+        return U, [s.as_dict() for s in st]
+
+    # resident split, for timing without host transfers
+    def set_rhs(self, f):
+        _check(self._lib.vfem_set_rhs(self._h, np.ascontiguousarray(f, np.float64)))
+
+    def solve_resident(self, **opts):
+        st = Stats()
+        o = options(**opts)
+        rc = self._lib.vfem_solve_resident(self._h, ctypes.byref(o), ctypes.byref(st))
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _check(rc)
+        return st.as_dict()
+
+    def get_solution(self):
+        u = np.zeros(self.n)
+        _check(self._lib.vfem_get_solution(self._h, u))
+        return u
+
+    def time_kernel(self, which, precision=FP64, reps=10):
+        out = ctypes.c_double()
+        _check(self._lib.vfem_time_kernel(self._h, int(which), int(precision), int(reps), ctypes.byref(out)))
+        return out.value
+
+
+class MultigridPreconditioner:
+    """Eigen preconditioner concept over the device V-cycle
+    (MultigridPreconditioner.hpp:24-58,195-203)."""
+
+    def __init__(self, mat=None, **opts):
+        self.matrix = None
+        self.opts = opts
+        if mat is not None:
+            self.compute(mat)
+
+    def analyzePattern(self, mat):
+        return self
+
+    def factorize(self, mat):
+        self.matrix = mat
+        return self
+
+    def compute(self, mat):
+        return self.factorize(mat)
+
+    def info(self):
+        return 0  # Eigen::Success
+
+    def solve(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        o = options(**self.opts)
+        _check(self.matrix._lib.vfem_precondition(self.matrix._h, r, z, ctypes.byref(o)))
+        return z
+
+
+def assemble_system_matrix(voxels, ke, fixed_nodes, num_levels=3, skip_levels=0):
+    """assembleSystemMatrix<double>(voxelModel, dims, Ke, fixedNodes, numLevels, skipLevels)."""
+    return MatrixFreeSparse(voxels, ke, fixed_nodes, num_levels, skip_levels)
+
+
+def apply_boundary_conditions(f, loaded_nodes, fixed_nodes=None):
+    """applyBoundaryConditions(f, loadedNodes, fixedNodes) (FEM.cpp:568-584):
+    loaded_nodes maps compact node id -> (x, y, z) load; f is modified in place."""
+    items = sorted(loaded_nodes.items())
+    ids = np.ascontiguousarray([k for k, _ in items], dtype=np.int64)
+    loads = np.ascontiguousarray([v for _, v in items], dtype=np.float64).reshape(-1)
+    if not f.flags["C_CONTIGUOUS"] or f.dtype != np.float64:
+        raise ValueError("f must be a contiguous float64 array")
+    _check(lib().vfem_apply_boundary_conditions(f, f.size, ids, loads, ids.size))
+    return f
+
+
+def solve_with_hip(A, f, u, **opts):
+    """solveWithKompute<double>(systemMatrix, f, u): u is overwritten with the
+    solution (FEMGPU.cpp:551).  Returns the solver statistics."""
+    sol, st = A.solve(f, **opts)
+    u[:] = sol
+    return st
+
+
+def inspect_hierarchy(voxels, ke, fixed_nodes, num_levels, skip_levels, level):
+    """Host-only view of the hierarchy the device uses (no GPU call)."""
+    vox, ke, fixed = _prep(voxels, ke, fixed_nodes)
+    nz, ny, nx = vox.shape
+    L = lib()
+    nl, nn, nf, nfree = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(L.vfem_inspect_hierarchy(vox.reshape(-1), nx, ny, nz, ke, fixed, fixed.size, num_levels, skip_levels,
+                                    level, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(nf), None, None,
+                                    ctypes.byref(nfree)))
+    if level >= nl.value:
+        return {"levels": nl.value}
+    inv = np.zeros(3 * nn.value)
+    mask = np.zeros(nn.value, np.uint8)
+    _check(L.vfem_inspect_hierarchy(vox.reshape(-1), nx, ny, nz, ke, fixed, fixed.size, num_levels, skip_levels,
+                                    level, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(nf),
+                                    inv.ctypes.data_as(ctypes.c_void_p), mask.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.byref(nfree)))
+    return {"levels": nl.value, "nodes": nn.value, "fixed_count": nf.value, "invdiag": inv,
+            "fixed": mask.astype(bool), "free_dofs": nfree.value}

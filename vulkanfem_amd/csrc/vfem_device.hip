@@ -1,0 +1,1400 @@
+// Device side of the MGPCG solver: kernels, the device-resident CG loop and the
+// C-ABI of include/vfem.h.
+//
+// Reference kernels replaced (shaders/*.comp of hirademirkol/VulkanFEM):
+//   MatxVec.comp + FixedNodes.comp      -> k_matvec (node-centric, atomic-free, fixed rows fused)
+//   VecDotVec.comp                      -> block_reduce_finalize (single pass, deterministic)
+//   ConjugateGradient_1.comp            -> k_cg_update (x, r update fused with r.r)
+//   ConjugateGradient_2.comp            -> k_cg_pupdate
+//   CWiseMult.comp + Restrict(Double)   -> k_restrict (coarse-node gather; mu cancels, see below)
+//   Interpolate(Double) + PreSmoothAndAXPY + Smooth -> k_prolong (fine-node gather fused with
+//                                          the smoother, the fixed mask and r.z)
+//
+// Transfer operators.  The reference restricts with r_c[N] = sum_E sum_f W mu[f] r[f]
+// over coarse elements E (MultigridPreconditioner.hpp:67-108) and interpolates with
+// x[f] = mu[f] sum_E sum_N W x_c[N] (:110-155), mu = 1/multiplicity (FEM.cpp:400).
+// Every coarse element whose lattice holds a fine node f contains, as corners,
+// all coarse nodes N with |f - div N|_inf < div (the nodes with W > 0), and the
+// weight W depends only on f - div N.  The multiplicity therefore cancels:
+//   r_c[N] = sum_{used f, |f - div N| < div} w(f - div N) r[f],
+//   x[f]   = sum_{N, |f - div N| < div} w(f - div N) x_c[N],
+// with w(d) = prod_k (1 - |d_k| / div): plain trilinear full weighting and
+// interpolation, and R = P^T exactly.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "vfem.h"
+#include "vfem_internal.h"
+#include "vfem_kernels.h"
+
+namespace vfem {
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+
+// Blocks are dispatched round-robin over the 8 XCDs; give XCD k the contiguous
+// block range [k G/8, (k+1) G/8) so each XCD's L2 sees one slab of the grid.
+__device__ __forceinline__ int remap_block()
+{
+    int G = gridDim.x, b = blockIdx.x;
+    if (G & 7) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+struct Range {
+    int64_t begin, end;
+};
+
+__device__ __forceinline__ Range block_range(int64_t W)
+{
+    int64_t G = gridDim.x;
+    int64_t b = remap_block();
+    int64_t per = (W + G - 1) / G;
+    int64_t beg = b * per;
+    int64_t end = beg + per < W ? beg + per : W;
+    return {beg, end};
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ void finalize(CGState *st, int fin, double v)
+{
+    switch (fin) {
+    case FIN_STORE:
+        st->result = v;
+        break;
+    case FIN_PAP:
+        st->pAp = v;
+        if (!(v > 0) || !isfinite(v)) st->done = 3;
+        else st->alpha = st->absNew / v;
+        break;
+    case FIN_RR:
+        st->rr = v;
+        st->last_true = 0;
+        if (!isfinite(v)) st->done = 3;
+        else if (v < st->threshold) st->done = 1;
+        break;
+    case FIN_RZ_INIT:
+        st->absNew = v;
+        st->beta = 0;
+        if (!isfinite(v)) st->done = 3;
+        break;
+    case FIN_RZ:
+        st->absOld = st->absNew;
+        st->absNew = v;
+        st->beta = v / st->absOld;
+        st->iter += 1;
+        if (!isfinite(st->beta)) st->done = 3;
+        else if (st->iter >= st->maxit) st->done = 2;
+        break;
+    case FIN_RHS: {
+        st->rhs2 = v;
+        double thr = st->tol * st->tol * v;
+        st->threshold = thr > DBL_MIN ? thr : DBL_MIN;
+        st->rr = v;
+        if (v == 0.0) st->done = 1;
+        break;
+    }
+    case FIN_RR_TRUE:
+        st->rr = v;
+        st->last_true = 1;
+        if (st->done == 2 || st->done == 3) break;
+        if (!isfinite(v)) st->done = 3;
+        else st->done = (v < st->threshold) ? 1 : 0;
+        break;
+    }
+}
+
+// Single-pass grid reduction: block partials in fixed order, the last block to
+// arrive sums them in fixed order (deterministic) and runs the finalizer.
+__device__ void block_reduce_finalize(double v, Reduction red, CGState *st, int fin)
+{
+    __shared__ double sh[kWaves];
+    __shared__ int am_last;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+        for (int w = 0; w < kWaves; w++) s += sh[w];
+        __hip_atomic_store(&red.partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned t = atomicAdd(red.counter, 1u);
+        am_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double s = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+        s += __hip_atomic_load(&red.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = wave_sum(s);
+    __syncthreads();
+    if (lane == 0) sh[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0;
+        for (int w = 0; w < kWaves; w++) tot += sh[w];
+        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        finalize(st, fin, tot);
+    }
+}
+
+// compact id of node (x, y, z) of level L, or -1 when unused / outside
+__device__ __forceinline__ int lookup(const DevLevel &L, int x, int y, int z)
+{
+    if ((unsigned)x >= (unsigned)L.nx || (unsigned)y >= (unsigned)L.ny || (unsigned)z >= (unsigned)L.nz) return -1;
+    int64_t seg = ((int64_t)z * L.ny + y) * L.S + (x >> 6);
+    uint64_t w = L.nbits[seg];
+    int b = x & 63;
+    if (!((w >> b) & 1)) return -1;
+    return L.nbase[seg] + __popcll(w & ((1ull << b) - 1));
+}
+
+template <typename T> __device__ __forceinline__ T ld(const T *p) { return *p; }
+
+// ---------------------------------------------------------------------------
+// matvec: y = A x with fixed rows zeroed (MatrixFreeSparse.hpp:198-214)
+// ---------------------------------------------------------------------------
+
+// Gathers x at the 27 neighbours of the wave's 64 nodes.  xv[k][c], k = (dz*3+dy)*3+dx.
+template <typename T>
+__device__ __forceinline__ void gather27(const DevLevel &L, const T *__restrict__ x, int s, int yy, int zz, int lane,
+                                         T (&xv)[27][3])
+{
+    const uint64_t lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int dz = 0; dz < 3; dz++) {
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++) {
+            const int ry = yy + dy - 1, rz = zz + dz - 1;
+            uint64_t w = 0, wp = 0, wn = 0;
+            int base = 0;
+            if (ry >= 0 && ry < L.ny && rz >= 0 && rz < L.nz) {
+                const int64_t seg = ((int64_t)rz * L.ny + ry) * L.S + s;
+                w = L.nbits[seg];
+                base = L.nbase[seg];
+                if (s > 0) wp = L.nbits[seg - 1];
+                if (s + 1 < L.S) wn = L.nbits[seg + 1];
+            }
+            const int idc = base + __popcll(w & lt);
+            const bool vc = (w >> lane) & 1;
+            const bool vm = lane ? ((w >> (lane - 1)) & 1) : (wp >> 63);
+            const bool vp = (lane < 63) ? ((w >> (lane + 1)) & 1) : (wn & 1);
+            const int k = (dz * 3 + dy) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                xv[k + 0][c] = vm ? x[3 * (idc - 1) + c] : T(0);
+                xv[k + 1][c] = vc ? x[3 * idc + c] : T(0);
+                xv[k + 2][c] = vp ? x[3 * (idc + (vc ? 1 : 0)) + c] : T(0);
+            }
+        }
+    }
+}
+
+// presence of the 8 elements around each lane's node; bit e = a + 2b + 4c for
+// element (x-1+a, y-1+b, z-1+c)
+__device__ __forceinline__ unsigned element_ring(const DevLevel &L, int s, int yy, int zz, int lane)
+{
+    unsigned ring = 0;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int ey = yy - 1 + b, ez = zz - 1 + c;
+            if (ey < 0 || ey >= L.ey || ez < 0 || ez >= L.ez) continue;
+            const int64_t eseg = ((int64_t)ez * L.ey + ey) * L.SE;
+            const uint64_t ew = (s < L.SE) ? L.ebits[eseg + s] : 0;
+            const uint64_t ewp = (s > 0 && s - 1 < L.SE) ? L.ebits[eseg + s - 1] : 0;
+            const unsigned e1 = (ew >> lane) & 1;
+            const unsigned e0 = lane ? ((ew >> (lane - 1)) & 1) : (unsigned)(ewp >> 63);
+            ring |= (e0 << (2 * b + 4 * c)) | (e1 << (1 + 2 * b + 4 * c));
+        }
+    }
+    return ring;
+}
+
+// Node-centric element form: y[n] = sum_{present e around n} Ke[rows of n in e] x_e.
+// Level 0 (uniform Ke, coefficients wave-uniform -> scalar loads).  Optional
+// fused dot x.y (p.Ap) with finalizer `fin`.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__restrict__ ke, const T *__restrict__ x,
+                                                         T *__restrict__ y, CGState *st, Reduction red, int fin,
+                                                         int honor_done)
+{
+    if (honor_done && st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t W = (int64_t)L.nz * L.ny * L.S;
+    const Range R = block_range(W);
+    double dacc = 0;
+    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+        const uint64_t wself = L.nbits[it];
+        if (wself == 0) continue;
+        const int s = (int)(it % L.S);
+        const int64_t row = it / L.S;
+        const int yy = (int)(row % L.ny), zz = (int)(row / L.ny);
+        T xv[27][3];
+        gather27(L, x, s, yy, zz, lane, xv);
+        const unsigned ring = element_ring(L, s, yy, zz, lane);
+        T acc0 = 0, acc1 = 0, acc2 = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;
+            const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+            T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const T v = xv[k][j];
+                    s0 += ke[(3 * Ln + 0) * 24 + 3 * m + j] * v;
+                    s1 += ke[(3 * Ln + 1) * 24 + 3 * m + j] * v;
+                    s2 += ke[(3 * Ln + 2) * 24 + 3 * m + j] * v;
+                }
+            }
+            if ((ring >> e) & 1) {
+                acc0 += s0;
+                acc1 += s1;
+                acc2 += s2;
+            }
+        }
+        if ((wself >> lane) & 1) {
+            const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+            if ((L.fbits[it] >> lane) & 1) acc0 = acc1 = acc2 = 0;
+            y[3 * id + 0] = acc0;
+            y[3 * id + 1] = acc1;
+            y[3 * id + 2] = acc2;
+            dacc += (double)xv[13][0] * (double)acc0 + (double)xv[13][1] * (double)acc1 +
+                    (double)xv[13][2] * (double)acc2;
+        }
+    }
+    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+}
+
+// Coarse-level operator of the damped cycle: element matrices from a table
+// indexed by the element's child pattern (tab[p*576 ...], pattern 0 = absent).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *__restrict__ tab,
+                                                           const T *__restrict__ x, T *__restrict__ y,
+                                                           const CGState *st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t W = (int64_t)L.nz * L.ny * L.S;
+    const Range R = block_range(W);
+    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+        const uint64_t wself = L.nbits[it];
+        if (wself == 0) continue;
+        const int s = (int)(it % L.S);
+        const int64_t row = it / L.S;
+        const int yy = (int)(row % L.ny), zz = (int)(row / L.ny);
+        T xv[27][3];
+        gather27(L, x, s, yy, zz, lane, xv);
+        const int xx = 64 * s + lane;
+        T acc0 = 0, acc1 = 0, acc2 = 0;
+        if ((wself >> lane) & 1) {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;
+                const int ex = xx - 1 + a, ey = yy - 1 + b, ez = zz - 1 + c;
+                if (ex < 0 || ex >= L.ex || ey < 0 || ey >= L.ey || ez < 0 || ez >= L.ez) continue;
+                const int p = L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex];
+                if (p == 0) continue;
+                const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+                const T *K = tab + (int64_t)p * 576 + 3 * Ln * 24;
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const T v = xv[k][j];
+                        acc0 += K[0 * 24 + 3 * m + j] * v;
+                        acc1 += K[1 * 24 + 3 * m + j] * v;
+                        acc2 += K[2 * 24 + 3 * m + j] * v;
+                    }
+                }
+            }
+            const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+            if ((L.fbits[it] >> lane) & 1) acc0 = acc1 = acc2 = 0;
+            y[3 * id + 0] = acc0;
+            y[3 * id + 1] = acc1;
+            y[3 * id + 2] = acc2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// transfers
+// ---------------------------------------------------------------------------
+
+// r_c[N] = sum_{|f - div N| < div} w(f - div N) (r_f[f] - t_f[f]); t_f may be null.
+// Coarse fixed rows zeroed when maskfixed.
+template <typename Tf, typename Tt, typename Tc>
+__global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
+                                                      const Tt *__restrict__ tf, Tc *__restrict__ rc, int maskfixed,
+                                                      const CGState *st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t W = (int64_t)C.nz * C.ny * C.S;
+    const Range R = block_range(W);
+    const int div = C.div;
+    const float inv = 1.0f / div;
+    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+        const uint64_t wself = C.nbits[it];
+        if (wself == 0) continue;
+        if (!((wself >> lane) & 1)) continue;
+        const int s = (int)(it % C.S);
+        const int64_t row = it / C.S;
+        const int Y = (int)(row % C.ny), Z = (int)(row / C.ny), X = 64 * s + lane;
+        Tc a0 = 0, a1 = 0, a2 = 0;
+        for (int dz = 1 - div; dz < div; dz++) {
+            const int fz = div * Z + dz;
+            if (fz < 0 || fz >= F.nz) continue;
+            const Tc wz = Tc(1) - Tc(abs(dz) * inv);
+            for (int dy = 1 - div; dy < div; dy++) {
+                const int fy = div * Y + dy;
+                if (fy < 0 || fy >= F.ny) continue;
+                const Tc wzy = wz * (Tc(1) - Tc(abs(dy) * inv));
+                for (int dx = 1 - div; dx < div; dx++) {
+                    const int id = lookup(F, div * X + dx, fy, fz);
+                    if (id < 0) continue;
+                    const Tc w = wzy * (Tc(1) - Tc(abs(dx) * inv));
+                    Tc v0 = Tc(rf[3 * id + 0]), v1 = Tc(rf[3 * id + 1]), v2 = Tc(rf[3 * id + 2]);
+                    if (tf) {
+                        v0 -= Tc(tf[3 * id + 0]);
+                        v1 -= Tc(tf[3 * id + 1]);
+                        v2 -= Tc(tf[3 * id + 2]);
+                    }
+                    a0 += w * v0;
+                    a1 += w * v1;
+                    a2 += w * v2;
+                }
+            }
+        }
+        const int id = C.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+        if (maskfixed && ((C.fbits[it] >> lane) & 1)) a0 = a1 = a2 = 0;
+        rc[3 * id + 0] = a0;
+        rc[3 * id + 1] = a1;
+        rc[3 * id + 2] = a2;
+    }
+}
+
+// Fine-node kernel of the upward sweep.
+//   mode PROLONG_SMOOTH (reference additive cycle, Interpolate + PreSmoothAndAXPY + Smooth):
+//        z_f = P z_c + 2 D^-1 r_f
+//   mode PROLONG_ADD (damped cycle):  z_f += P z_c
+// maskfixed zeroes fixed rows; fin >= 0 reduces r.z.
+enum { PROLONG_SMOOTH = 0, PROLONG_ADD = 1 };
+
+template <typename Tc, typename Tr, typename Td, typename Tz>
+__global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, const Tc *__restrict__ zc,
+                                                     const Tr *__restrict__ rf, const Td *__restrict__ dinv,
+                                                     Tz *__restrict__ zf, int mode, int maskfixed, CGState *st,
+                                                     Reduction red, int fin)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t W = (int64_t)F.nz * F.ny * F.S;
+    const Range R = block_range(W);
+    const int div = C.div;
+    double dacc = 0;
+    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+        const uint64_t wself = F.nbits[it];
+        if (!((wself >> lane) & 1)) continue;
+        const int s = (int)(it % F.S);
+        const int64_t row = it / F.S;
+        const int y = (int)(row % F.ny), z = (int)(row / F.ny), x = 64 * s + lane;
+        const int X0 = x / div, Y0 = y / div, Z0 = z / div;
+        const int rx = x - div * X0, ry = y - div * Y0, rz = z - div * Z0;
+        Tz a0 = 0, a1 = 0, a2 = 0;
+        for (int cz = 0; cz < 2; cz++) {
+            if (cz && !rz) break;
+            const Tz wz = cz ? Tz(rz) / Tz(div) : Tz(1) - Tz(rz) / Tz(div);
+            for (int cy = 0; cy < 2; cy++) {
+                if (cy && !ry) break;
+                const Tz wy = cy ? Tz(ry) / Tz(div) : Tz(1) - Tz(ry) / Tz(div);
+                for (int cx = 0; cx < 2; cx++) {
+                    if (cx && !rx) break;
+                    const Tz wx = cx ? Tz(rx) / Tz(div) : Tz(1) - Tz(rx) / Tz(div);
+                    const int id = lookup(C, X0 + cx, Y0 + cy, Z0 + cz);
+                    if (id < 0) continue;
+                    const Tz w = wz * wy * wx;
+                    a0 += w * Tz(zc[3 * id + 0]);
+                    a1 += w * Tz(zc[3 * id + 1]);
+                    a2 += w * Tz(zc[3 * id + 2]);
+                }
+            }
+        }
+        const int id = F.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+        const bool fx = maskfixed && ((F.fbits[it] >> lane) & 1);
+        Tz o[3] = {a0, a1, a2};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            Tz v;
+            if (mode == PROLONG_SMOOTH) {
+                const Tz dr = Tz(dinv[3 * id + c]) * Tz(rf[3 * id + c]);
+                v = (o[c] + dr) + dr;
+            } else {
+                v = zf[3 * id + c] + o[c];
+            }
+            if (fx) v = 0;
+            zf[3 * id + c] = v;
+            if (fin >= 0) dacc += (double)rf[3 * id + c] * (double)v;
+        }
+    }
+    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+}
+
+// Damped Jacobi on one level.
+//   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
+// fixed rows zeroed; fin >= 0 reduces r.z.
+enum { SMOOTH_INIT = 0, SMOOTH_POST = 1 };
+
+template <typename Tr, typename Td, typename Tz>
+__global__ __launch_bounds__(kBlock) void k_smooth(DevLevel L, const Tr *__restrict__ r, const Tz *__restrict__ t,
+                                                    const Td *__restrict__ dinv, Tz *__restrict__ z, double omega,
+                                                    int mode, CGState *st, Reduction red, int fin)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t W = (int64_t)L.nz * L.ny * L.S;
+    const Range R = block_range(W);
+    const Tz om = Tz(omega);
+    double dacc = 0;
+    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+        const uint64_t wself = L.nbits[it];
+        if (!((wself >> lane) & 1)) continue;
+        const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+        const bool fx = (L.fbits[it] >> lane) & 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int64_t k = 3 * (int64_t)id + c;
+            Tz v;
+            if (mode == SMOOTH_INIT) v = om * Tz(dinv[k]) * Tz(r[k]);
+            else v = z[k] + om * Tz(dinv[k]) * (Tz(r[k]) - t[k]);
+            if (fx) v = 0;
+            z[k] = v;
+            if (fin >= 0) dacc += (double)r[k] * (double)v;
+        }
+    }
+    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+}
+
+// Coarsest level: z[freedof[i]] = sum_k Kc^-1[i][k] r[freedof[k]] (replaces the
+// host SimplicialLDLT solve of FEMGPU.cpp:509-521).  One wavefront per row.
+template <typename Tk, typename Tv>
+__global__ __launch_bounds__(kBlock) void k_coarse_solve(const Tk *__restrict__ kinv, int n,
+                                                          const int32_t *__restrict__ freedof,
+                                                          const Tv *__restrict__ rc, Tv *__restrict__ zc,
+                                                          const CGState *st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rowi = blockIdx.x * kWaves + wave;
+    if (rowi >= n) return;
+    const Tk *Krow = kinv + (int64_t)rowi * n;
+    double acc = 0;
+    for (int k = lane; k < n; k += 64) acc += (double)Krow[k] * (double)rc[freedof[k]];
+    acc = wave_sum(acc);
+    if (lane == 0) zc[freedof[rowi]] = Tv(acc);
+}
+
+// ---------------------------------------------------------------------------
+// CG vector kernels (flat over 3*nn DOFs)
+// ---------------------------------------------------------------------------
+
+// ConjugateGradient_1.comp fused with the r.r of VecDotVec.comp
+template <typename Tp, typename Ta>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                       const Tp *__restrict__ p, const Ta *__restrict__ ap,
+                                                       CGState *st, Reduction red)
+{
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const Range R = block_range(n);
+    double acc = 0;
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
+        x[i] += alpha * (double)p[i];
+        const double rv = r[i] - alpha * (double)ap[i];
+        r[i] = rv;
+        acc += rv * rv;
+    }
+    block_reduce_finalize(acc, red, st, FIN_RR);
+}
+
+// ConjugateGradient_2.comp: p = z + beta p
+template <typename Tz, typename Tp>
+__global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
+                                                        const CGState *st)
+{
+    if (st->done) return;
+    const double beta = st->beta;
+    const Range R = block_range(n);
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) p[i] = Tp((double)z[i] + beta * (double)p[i]);
+}
+
+// x = 0, r = f, |f|^2 (x0 = 0)   or   r = f - t, |r|^2 (true residual / warm start)
+__global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__restrict__ f,
+                                                     const double *__restrict__ t, double *__restrict__ r,
+                                                     double *__restrict__ x, CGState *st, Reduction red, int fin)
+{
+    const Range R = block_range(n);
+    double acc = 0;
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
+        double v = f[i];
+        if (t) v -= t[i];
+        r[i] = v;
+        if (x) x[i] = 0;
+        acc += v * v;
+    }
+    block_reduce_finalize(acc, red, st, fin);
+}
+
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict__ a, const Tb *__restrict__ b,
+                                                CGState *st, Reduction red, int fin)
+{
+    const Range R = block_range(n);
+    double acc = 0;
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) acc += (double)a[i] * (double)b[i];
+    block_reduce_finalize(acc, red, st, fin);
+}
+
+template <typename Ts, typename Td>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts *__restrict__ s, Td *__restrict__ d)
+{
+    const Range R = block_range(n);
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) d[i] = Td(s[i]);
+}
+
+__global__ void k_set_state(CGState *st, int maxit, double tol)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->iter = 0;
+        st->done = 0;
+        st->last_true = 0;
+        st->maxit = maxit;
+        st->tol = tol;
+        st->absNew = st->absOld = st->pAp = st->alpha = st->beta = st->rr = 0;
+    }
+}
+
+}  // namespace vfem
+
+// ===========================================================================
+// host side
+// ===========================================================================
+
+using namespace vfem;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define HIP_TRY(expr)                                                                                          \
+    do {                                                                                                       \
+        hipError_t _e = (expr);                                                                                \
+        if (_e != hipSuccess) {                                                                                \
+            g_last_error = std::string(#expr) + ": " + hipGetErrorString(_e);                                  \
+            return VFEM_ERR_DEVICE;                                                                            \
+        }                                                                                                      \
+    } while (0)
+
+template <typename T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+};
+
+int grid_for_items(int64_t W)
+{
+    int64_t g = (W + 4 * kWaves - 1) / (4 * kWaves);
+    g = std::max<int64_t>(8, std::min<int64_t>(4096, g));
+    return (int)((g + 7) / 8 * 8);
+}
+
+int grid_for_flat(int64_t n)
+{
+    int64_t g = (n + 8 * kBlock - 1) / (8 * kBlock);
+    g = std::max<int64_t>(8, std::min<int64_t>(2048, g));
+    return (int)((g + 7) / 8 * 8);
+}
+
+}  // namespace
+
+struct vfem_system {
+    Hierarchy H;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    DevLevel dl[kMaxLevels];
+    double *ke64 = nullptr;
+    float *ke32 = nullptr;
+    double *dinv64[kMaxLevels] = {};
+    float *dinv32[kMaxLevels] = {};
+    double *ktab64[kMaxLevels] = {};
+    float *ktab32[kMaxLevels] = {};
+    double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
+    float *kinv_ref32 = nullptr, *kinv_gal32 = nullptr;
+    int32_t *freedof = nullptr;
+    int64_t n3 = 0;
+    // level-0 vectors
+    double *x = nullptr, *r = nullptr, *f = nullptr, *t64 = nullptr, *p64 = nullptr, *ap64 = nullptr, *z64 = nullptr;
+    float *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr;
+    // coarse-level vectors (index = level), t = operator scratch of the damped cycle
+    double *rl64[kMaxLevels] = {}, *zl64[kMaxLevels] = {}, *tl64[kMaxLevels] = {};
+    float *rl32[kMaxLevels] = {}, *zl32[kMaxLevels] = {}, *tl32[kMaxLevels] = {};
+    float *t0_32 = nullptr;
+    CGState *st = nullptr;
+    Reduction red{};
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::map<int, hipGraphExec_t> graphs;
+    double setup_ms = 0;
+
+    template <typename T> int alloc(T **p, size_t n, bool zero = true)
+    {
+        void *q = nullptr;
+        HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+        allocs.push_back(q);
+        if (zero) HIP_TRY(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+        *p = (T *)q;
+        return 0;
+    }
+    template <typename T> int upload(T **p, const std::vector<T> &v)
+    {
+        int rc = alloc(p, v.size(), false);
+        if (rc) return rc;
+        if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+    ~vfem_system()
+    {
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
+        for (void *q : allocs) (void)hipFree(q);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T> std::vector<T> convert(const std::vector<double> &v)
+{
+    std::vector<T> o(v.size());
+    for (size_t i = 0; i < v.size(); i++) o[i] = (T)v[i];
+    return o;
+}
+
+int upload_system(vfem_system *S)
+{
+    Hierarchy &H = S->H;
+    HIP_TRY(hipGetDevice(&S->device));
+    HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&S->ev0));
+    HIP_TRY(hipEventCreate(&S->ev1));
+    for (int l = 0; l < H.nlev; l++) {
+        HostLevel &L = H.L[l];
+        DevLevel &D = S->dl[l];
+        D.nx = L.nx; D.ny = L.ny; D.nz = L.nz; D.S = L.S;
+        D.ex = L.ex; D.ey = L.ey; D.ez = L.ez; D.SE = L.SE;
+        D.div = L.div;
+        D.nn = L.nn;
+        uint64_t *nb, *fb, *eb;
+        int32_t *base;
+        uint8_t *ep = nullptr;
+        if (int rc = S->upload(&nb, L.nbits)) return rc;
+        if (int rc = S->upload(&fb, L.fbits)) return rc;
+        if (int rc = S->upload(&eb, L.ebits)) return rc;
+        if (int rc = S->upload(&base, L.nbase)) return rc;
+        if (!L.epat.empty())
+            if (int rc = S->upload(&ep, L.epat)) return rc;
+        D.nbits = nb; D.fbits = fb; D.ebits = eb; D.nbase = base; D.epat = ep;
+        if (int rc = S->upload(&S->dinv64[l], L.invdiag)) return rc;
+        if (int rc = S->upload(&S->dinv32[l], convert<float>(L.invdiag))) return rc;
+        if (l >= 1 && l < (int)H.ktab.size() && !H.ktab[l].empty()) {
+            if (int rc = S->upload(&S->ktab64[l], H.ktab[l])) return rc;
+            if (int rc = S->upload(&S->ktab32[l], convert<float>(H.ktab[l]))) return rc;
+        }
+    }
+    std::vector<double> ke(H.ke, H.ke + 576);
+    if (int rc = S->upload(&S->ke64, ke)) return rc;
+    if (int rc = S->upload(&S->ke32, convert<float>(ke))) return rc;
+    if (int rc = S->upload(&S->kinv_ref64, H.kcinv_ref)) return rc;
+    if (int rc = S->upload(&S->kinv_gal64, H.kcinv_gal)) return rc;
+    if (int rc = S->upload(&S->kinv_ref32, convert<float>(H.kcinv_ref))) return rc;
+    if (int rc = S->upload(&S->kinv_gal32, convert<float>(H.kcinv_gal))) return rc;
+    if (int rc = S->upload(&S->freedof, H.freedof)) return rc;
+    S->n3 = 3 * H.L[0].nn;
+    int64_t n3 = S->n3;
+    if (int rc = S->alloc(&S->x, n3)) return rc;
+    if (int rc = S->alloc(&S->r, n3)) return rc;
+    if (int rc = S->alloc(&S->f, n3)) return rc;
+    if (int rc = S->alloc(&S->t64, n3)) return rc;
+    if (int rc = S->alloc(&S->p64, n3)) return rc;
+    if (int rc = S->alloc(&S->ap64, n3)) return rc;
+    if (int rc = S->alloc(&S->z64, n3)) return rc;
+    if (int rc = S->alloc(&S->p32, n3)) return rc;
+    if (int rc = S->alloc(&S->ap32, n3)) return rc;
+    if (int rc = S->alloc(&S->z32, n3)) return rc;
+    if (int rc = S->alloc(&S->t0_32, n3)) return rc;
+    for (int l = 1; l < H.nlev; l++) {
+        int64_t m = 3 * H.L[l].nn;
+        if (int rc = S->alloc(&S->rl64[l], m)) return rc;
+        if (int rc = S->alloc(&S->zl64[l], m)) return rc;
+        if (int rc = S->alloc(&S->tl64[l], m)) return rc;
+        if (int rc = S->alloc(&S->rl32[l], m)) return rc;
+        if (int rc = S->alloc(&S->zl32[l], m)) return rc;
+        if (int rc = S->alloc(&S->tl32[l], m)) return rc;
+    }
+    if (int rc = S->alloc(&S->st, 1)) return rc;
+    if (int rc = S->alloc(&S->red.partials, 4096)) return rc;
+    if (int rc = S->alloc(&S->red.counter, 1)) return rc;
+    return 0;
+}
+
+// ---- launch helpers -------------------------------------------------------
+
+int items_grid(const DevLevel &L) { return grid_for_items((int64_t)L.nz * L.ny * L.S); }
+
+template <typename T> T *level_r(vfem_system *S, int l);
+template <> double *level_r<double>(vfem_system *S, int l) { return S->rl64[l]; }
+template <> float *level_r<float>(vfem_system *S, int l) { return S->rl32[l]; }
+template <typename T> T *level_z(vfem_system *S, int l);
+template <> double *level_z<double>(vfem_system *S, int l) { return S->zl64[l]; }
+template <> float *level_z<float>(vfem_system *S, int l) { return S->zl32[l]; }
+template <typename T> T *level_t(vfem_system *S, int l);
+template <> double *level_t<double>(vfem_system *S, int l) { return S->tl64[l]; }
+template <> float *level_t<float>(vfem_system *S, int l) { return S->tl32[l]; }
+template <typename T> const T *dinv(vfem_system *S, int l);
+template <> const double *dinv<double>(vfem_system *S, int l) { return S->dinv64[l]; }
+template <> const float *dinv<float>(vfem_system *S, int l) { return S->dinv32[l]; }
+template <typename T> const T *kefine(vfem_system *S);
+template <> const double *kefine<double>(vfem_system *S) { return S->ke64; }
+template <> const float *kefine<float>(vfem_system *S) { return S->ke32; }
+template <typename T> const T *ktab(vfem_system *S, int l);
+template <> const double *ktab<double>(vfem_system *S, int l) { return S->ktab64[l]; }
+template <> const float *ktab<float>(vfem_system *S, int l) { return S->ktab32[l]; }
+template <typename T> const T *kinv(vfem_system *S, bool gal);
+template <> const double *kinv<double>(vfem_system *S, bool gal) { return gal ? S->kinv_gal64 : S->kinv_ref64; }
+template <> const float *kinv<float>(vfem_system *S, bool gal) { return gal ? S->kinv_gal32 : S->kinv_ref32; }
+
+template <typename T>
+void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done)
+{
+    const DevLevel &L = S->dl[0];
+    hipLaunchKernelGGL(k_matvec_fine<T>, dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, kefine<T>(S), x, y,
+                       S->st, S->red, fin, honor_done);
+}
+
+template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x, T *y)
+{
+    if (l == 0) {
+        launch_matvec_fine<T>(S, x, y, -1, 1);
+        return;
+    }
+    const DevLevel &L = S->dl[l];
+    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
+                       S->st);
+}
+
+template <typename T> void launch_coarse_solve(vfem_system *S, const T *rc, T *zc, bool gal)
+{
+    int n = (int)S->H.nfree;
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_coarse_solve<T, T>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, S->stream,
+                       kinv<T>(S, gal), n, S->freedof, rc, zc, S->st);
+}
+
+// Reference additive V-cycle (MultigridPreconditioner.hpp:158-193) on the
+// device: restrict down, coarse solve, prolong + 2 D^-1 r up.  r is the FP64
+// level-0 residual; z (type T) gets the masked preconditioned residual and
+// r.z is reduced with finalizer fin.
+template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T *z, int fin)
+{
+    const int nl = S->H.nlev;
+    for (int l = 0; l < nl - 1; l++) {
+        const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
+        if (l == 0)
+            hipLaunchKernelGGL((k_restrict<double, double, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, F,
+                               C, r, (const double *)nullptr, level_r<T>(S, 1), 0, S->st);
+        else
+            hipLaunchKernelGGL((k_restrict<T, T, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, F, C,
+                               (const T *)level_r<T>(S, l), (const T *)nullptr, level_r<T>(S, l + 1), 0, S->st);
+    }
+    launch_coarse_solve<T>(S, level_r<T>(S, nl - 1), level_z<T>(S, nl - 1), false);
+    for (int l = nl - 2; l >= 1; l--) {
+        const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
+        hipLaunchKernelGGL((k_prolong<T, T, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
+                           (const T *)level_z<T>(S, l + 1), (const T *)level_r<T>(S, l), dinv<T>(S, l),
+                           level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1);
+    }
+    const DevLevel &F = S->dl[0], &C = S->dl[1];
+    hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
+                       (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin);
+}
+
+// Damped multiplicative V(1,1) cycle, level l >= 0.  rl: level residual
+// (level 0: FP64 CG residual), zl: output, tl: operator scratch.
+template <typename T, typename TR>
+void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin)
+{
+    const int nl = S->H.nlev;
+    if (l == nl - 1) {
+        launch_coarse_solve<T>(S, (const T *)rl, zl, true);
+        return;
+    }
+    const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, rl, (const T *)tl,
+                       dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1);
+    launch_level_matvec<T>(S, l, zl, tl);
+    hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, L, C, rl,
+                       (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
+    enqueue_vcycle_damped_level<T, T>(S, l + 1, level_r<T>(S, l + 1), level_z<T>(S, l + 1), level_t<T>(S, l + 1),
+                                      omega, -1);
+    hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, C,
+                       (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
+                       -1);
+    launch_level_matvec<T>(S, l, zl, tl);
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, rl, (const T *)tl,
+                       dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin);
+}
+
+struct Plan {
+    int precision;  // VFEM_FP64 / VFEM_FP32
+    int cycle;
+    double omega;
+};
+
+// z = M r on the level-0 FP64 residual r, reducing r.z with `fin`
+void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
+{
+    if (S->H.nlev == 1) {
+        // single level: the cycle is the exact coarse solve, z = Kc^-1 r (FP64), then r.z
+        launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(S->n3)), dim3(kBlock), 0, S->stream, S->n3, r,
+                           S->z64, S->st, S->red, fin);
+        return;
+    }
+    if (P.precision == VFEM_FP64) {
+        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<double>(S, r, S->z64, fin);
+        else enqueue_vcycle_damped_level<double, double>(S, 0, r, S->z64, S->t64, P.omega, fin);
+    } else {
+        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin);
+        else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin);
+    }
+}
+
+// FP64 true residual r = f - A x and |r|^2 (FIN_RR_TRUE)
+void enqueue_true_residual(vfem_system *S)
+{
+    launch_matvec_fine<double>(S, S->x, S->t64, -1, 0);
+    hipLaunchKernelGGL(k_residual, dim3(grid_for_flat(S->n3)), dim3(kBlock), 0, S->stream, S->n3, S->f, S->t64,
+                       S->r, (double *)nullptr, S->st, S->red, (int)FIN_RR_TRUE);
+}
+
+// p = z + beta p after the preconditioner
+void enqueue_pupdate(vfem_system *S, const Plan &P)
+{
+    const int64_t n3 = S->n3;
+    const int gf = grid_for_flat(n3);
+    if (P.precision == VFEM_FP64)
+        hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
+                           S->st);
+    else if (S->H.nlev == 1)
+        hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
+                           S->st);
+    else
+        hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
+                           S->st);
+}
+
+// one CG iteration (FEMGPU.cpp:471-541 / Eigen ConjugateGradient loop body).
+// with_true: residual replacement -- after the x/r update, r = f - A x in FP64,
+// and that true residual decides convergence (mixed precision).
+void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
+{
+    const int64_t n3 = S->n3;
+    const int gf = grid_for_flat(n3);
+    if (P.precision == VFEM_FP64) {
+        launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1);
+        hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
+                           S->p64, S->ap64, S->st, S->red);
+    } else {
+        launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1);
+        hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
+                           S->p32, S->ap32, S->st, S->red);
+    }
+    if (with_true) enqueue_true_residual(S);
+    enqueue_precond(S, P, S->r, FIN_RZ);
+    enqueue_pupdate(S, P);
+}
+
+int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
+{
+    if (!(o.tol >= 0) || o.max_iter < 0 || o.precision < 0 || o.precision > 2 || o.cycle < 0 || o.cycle > 1) {
+        g_last_error = "invalid solver options";
+        return VFEM_ERR_ARGUMENT;
+    }
+    P.precision = o.precision == VFEM_FP64 ? VFEM_FP64 : VFEM_FP32;
+    P.cycle = o.cycle;
+    P.omega = o.omega;
+    if (o.cycle == VFEM_CYCLE_DAMPED) {
+        for (int l = 1; l < S->H.nlev; l++)
+            if (S->H.L[l].div != 2) {
+                g_last_error = "the damped cycle needs skip_levels = 0";
+                return VFEM_ERR_UNSUPPORTED;
+            }
+    }
+    return 0;
+}
+
+int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphExec_t *out)
+{
+    int key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0);
+    // omega is baked into the graph's kernel arguments
+    key += (int)std::lround(P.omega * 1000) * 64;
+    auto it = S->graphs.find(key);
+    if (it != S->graphs.end()) {
+        *out = it->second;
+        return 0;
+    }
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(S, P, with_true);
+    HIP_TRY(hipStreamEndCapture(S->stream, &g));
+    hipGraphExec_t ge;
+    HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphDestroy(g));
+    S->graphs[key] = ge;
+    *out = ge;
+    return 0;
+}
+
+int read_state(vfem_system *S, CGState &h)
+{
+    HIP_TRY(hipMemcpyAsync(&h, S->st, sizeof(CGState), hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, bool warm)
+{
+    Plan P;
+    if (int rc = check_plan(S, o, P)) return rc;
+    const int64_t n3 = S->n3;
+    const int gf = grid_for_flat(n3);
+    const bool mixed = P.precision != VFEM_FP64;
+    hipGraphExec_t g_norm = nullptr, g_true = nullptr;
+    if (o.use_graph) {
+        if (int rc = get_iteration_graph(S, P, false, &g_norm)) return rc;
+        if (mixed)
+            if (int rc = get_iteration_graph(S, P, true, &g_true)) return rc;
+    }
+    HIP_TRY(hipEventRecord(S->ev0, S->stream));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, S->stream, S->st, o.max_iter, o.tol);
+    // |f|^2 and the threshold; r = f with x = 0, or r = f - A x0
+    hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr, S->r,
+                       warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
+    if (warm)
+        enqueue_true_residual(S);
+    else  // initial r.r test (Eigen: residualNorm2 < threshold -> 0 iterations)
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->r, S->r, S->st,
+                           S->red, (int)FIN_RR);
+    HIP_TRY(hipMemsetAsync(S->p64, 0, n3 * sizeof(double), S->stream));
+    HIP_TRY(hipMemsetAsync(S->p32, 0, n3 * sizeof(float), S->stream));
+    // p = M r, absNew = r.p
+    enqueue_precond(S, P, S->r, FIN_RZ_INIT);
+    enqueue_pupdate(S, P);
+    CGState h{};
+    if (int rc = read_state(S, h)) return rc;
+    int true_count = warm ? 1 : 0;
+    const int chunk = mixed ? std::max(1, o.true_res_every) : std::max(1, o.check_every);
+    int64_t launched = 0;
+    const int64_t cap = (int64_t)o.max_iter + 2 * chunk + 8;
+    if (o.max_iter == 0 && !h.done) h.done = 2;
+    while (true) {
+        if (h.done) {
+            // mixed precision stops on the FP64 true residual only
+            if (h.done == 1 && mixed && !h.last_true) {
+                enqueue_true_residual(S);
+                true_count++;
+                if (int rc = read_state(S, h)) return rc;
+                if (!h.done) {
+                    // finish the interrupted iteration: z = M r, beta, p
+                    enqueue_precond(S, P, S->r, FIN_RZ);
+                    enqueue_pupdate(S, P);
+                    if (int rc = read_state(S, h)) return rc;
+                    continue;
+                }
+            }
+            break;
+        }
+        if (launched >= cap) {
+            h.done = 2;
+            break;
+        }
+        for (int k = 0; k < chunk; k++) {
+            const bool wt = mixed && k == chunk - 1;
+            hipGraphExec_t g = wt ? g_true : g_norm;
+            if (g) HIP_TRY(hipGraphLaunch(g, S->stream));
+            else enqueue_iteration(S, P, wt);
+            if (wt) true_count++;
+        }
+        launched += chunk;
+        if (int rc = read_state(S, h)) return rc;
+    }
+    HIP_TRY(hipEventRecord(S->ev1, S->stream));
+    HIP_TRY(hipEventSynchronize(S->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+    double rec = h.rhs2 > 0 ? std::sqrt(h.rr / h.rhs2) : 0.0;
+    // true residual at exit (FP64), outside the timed region
+    launch_matvec_fine<double>(S, S->x, S->t64, -1, 0);
+    hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, S->stream, n3, S->f, S->t64, S->t64,
+                       (double *)nullptr, S->st, S->red, (int)FIN_STORE);
+    CGState h2{};
+    if (int rc = read_state(S, h2)) return rc;
+    int status = VFEM_OK;
+    if (h.done == 2) status = VFEM_ERR_NOT_CONVERGED;
+    if (h.done == 3) status = VFEM_ERR_BREAKDOWN;
+    if (stats) {
+        stats->iterations = h.iter;
+        stats->converged = h.done == 1;
+        stats->rec_residual = rec;
+        stats->rel_residual = h.rhs2 > 0 ? std::sqrt(h2.result / h.rhs2) : 0.0;
+        stats->solve_ms = ms;
+        stats->setup_ms = S->setup_ms;
+        stats->true_res_count = true_count;
+        stats->status = status;
+    }
+    if (status == VFEM_ERR_NOT_CONVERGED) g_last_error = "max_iter reached before the tolerance";
+    if (status == VFEM_ERR_BREAKDOWN) g_last_error = "CG breakdown (non-positive p.Ap or non-finite scalar)";
+    return status;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+
+extern "C" {
+
+void vfem_default_options(vfem_options *o)
+{
+    if (!o) return;
+    o->tol = 1e-8;
+    o->max_iter = 30000;
+    o->precision = VFEM_FP64;
+    o->cycle = VFEM_CYCLE_REFERENCE;
+    o->omega = 0.6;
+    o->true_res_every = 20;
+    o->check_every = 8;
+    o->use_graph = 1;
+    o->warm_start = 0;
+}
+
+const char *vfem_status_string(int status)
+{
+    switch (status) {
+    case VFEM_OK: return "ok";
+    case VFEM_ERR_ARGUMENT: return "invalid argument";
+    case VFEM_ERR_EMPTY: return "empty voxel model";
+    case VFEM_ERR_DEVICE: return "device error";
+    case VFEM_ERR_SINGULAR: return "coarsest matrix not positive definite";
+    case VFEM_ERR_NOT_CONVERGED: return "not converged";
+    case VFEM_ERR_BREAKDOWN: return "CG breakdown";
+    case VFEM_ERR_UNSUPPORTED: return "unsupported option";
+    default: return "unknown status";
+    }
+}
+
+const char *vfem_last_error(void) { return g_last_error.c_str(); }
+
+int vfem_assemble(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, const double *ke,
+                  const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
+                  vfem_system **out)
+{
+    if (!out) {
+        g_last_error = "out is null";
+        return VFEM_ERR_ARGUMENT;
+    }
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    vfem_system *S = new vfem_system();
+    std::string err;
+    int rc = build_hierarchy(voxels, nx, ny, nz, ke, fixed_nodes, n_fixed, num_levels, skip_levels, S->H, err);
+    if (rc) {
+        g_last_error = err;
+        delete S;
+        return rc;
+    }
+    rc = upload_system(S);
+    if (rc) {
+        delete S;
+        return rc;
+    }
+    S->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = S;
+    return VFEM_OK;
+}
+
+void vfem_destroy(vfem_system *sys) { delete sys; }
+
+int vfem_inspect_hierarchy(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, const double *ke,
+                           const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
+                           int32_t level, int32_t *n_levels_out, int64_t *n_nodes, int64_t *n_fixed_out,
+                           double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out)
+{
+    Hierarchy H;
+    std::string err;
+    int rc = build_hierarchy(voxels, nx, ny, nz, ke, fixed_nodes, n_fixed, num_levels, skip_levels, H, err);
+    if (rc) {
+        g_last_error = err;
+        return rc;
+    }
+    if (n_levels_out) *n_levels_out = H.nlev;
+    if (level < 0 || level >= H.nlev) return VFEM_OK;
+    const HostLevel &L = H.L[level];
+    if (n_nodes) *n_nodes = L.nn;
+    if (n_fixed_out) *n_fixed_out = L.nfixed;
+    if (invdiag) std::memcpy(invdiag, L.invdiag.data(), sizeof(double) * 3 * L.nn);
+    if (fixed_mask) {
+        int64_t k = 0;
+        for (int64_t r = 0; r < (int64_t)L.ny * L.nz; r++)
+            for (int s = 0; s < L.S; s++) {
+                uint64_t w = L.nbits[r * L.S + s], fw = L.fbits[r * L.S + s];
+                for (int b = 0; b < 64; b++)
+                    if ((w >> b) & 1) fixed_mask[k++] = (fw >> b) & 1;
+            }
+    }
+    if (free_dofs_out) *free_dofs_out = H.nfree;
+    return VFEM_OK;
+}
+
+int64_t vfem_num_dofs(const vfem_system *sys) { return sys ? sys->n3 : 0; }
+
+int32_t vfem_num_levels(const vfem_system *sys) { return sys ? sys->H.nlev : 0; }
+
+int vfem_level_info(const vfem_system *sys, int32_t level, int64_t *n_elems, int64_t *n_nodes, int64_t *n_fixed)
+{
+    if (!sys || level < 0 || level >= sys->H.nlev) {
+        g_last_error = "invalid level";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const HostLevel &L = sys->H.L[level];
+    int64_t ne = 0;
+    for (uint64_t w : L.ebits) ne += __builtin_popcountll(w);
+    if (n_elems) *n_elems = ne;
+    if (n_nodes) *n_nodes = L.nn;
+    if (n_fixed) *n_fixed = L.nfixed;
+    return VFEM_OK;
+}
+
+int vfem_apply_boundary_conditions(double *f, int64_t n_dofs, const int64_t *loaded_nodes, const double *loads,
+                                   int64_t n_loaded)
+{
+    if (!f || (n_loaded > 0 && (!loaded_nodes || !loads))) {
+        g_last_error = "null pointer";
+        return VFEM_ERR_ARGUMENT;
+    }
+    for (int64_t k = 0; k < n_loaded; k++) {
+        int64_t i = loaded_nodes[k];
+        if (i < 0 || 3 * i + 2 >= n_dofs) {
+            g_last_error = "loaded node out of range";
+            return VFEM_ERR_ARGUMENT;
+        }
+        for (int c = 0; c < 3; c++) f[3 * i + c] = loads[3 * k + c];
+    }
+    return VFEM_OK;
+}
+
+int vfem_set_rhs(vfem_system *sys, const double *f)
+{
+    if (!sys || !f) {
+        g_last_error = "null pointer";
+        return VFEM_ERR_ARGUMENT;
+    }
+    HIP_TRY(hipMemcpy(sys->f, f, sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    return VFEM_OK;
+}
+
+int vfem_get_solution(vfem_system *sys, double *u)
+{
+    if (!sys || !u) {
+        g_last_error = "null pointer";
+        return VFEM_ERR_ARGUMENT;
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(u, sys->x, sys->n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return VFEM_OK;
+}
+
+int vfem_solve_resident(vfem_system *sys, const vfem_options *opts, vfem_stats *stats)
+{
+    if (!sys) {
+        g_last_error = "null system";
+        return VFEM_ERR_ARGUMENT;
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    if (opts) o = *opts;
+    return solve_resident(sys, o, stats, o.warm_start != 0);
+}
+
+int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options *opts, vfem_stats *stats)
+{
+    if (!sys || !f || !u) {
+        g_last_error = "null pointer";
+        return VFEM_ERR_ARGUMENT;
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    if (opts) o = *opts;
+    if (int rc = vfem_set_rhs(sys, f)) return rc;
+    if (o.warm_start) HIP_TRY(hipMemcpy(sys->x, u, sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    int rc = solve_resident(sys, o, stats, o.warm_start != 0);
+    if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED) return rc;
+    int rc2 = vfem_get_solution(sys, u);
+    return rc2 ? rc2 : rc;
+}
+
+int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrhs, const vfem_options *opts,
+                       vfem_stats *stats)
+{
+    if (!sys || !f || !u || nrhs < 1) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    int worst = VFEM_OK;
+    for (int k = 0; k < nrhs; k++) {
+        vfem_stats s{};
+        int rc = vfem_solve(sys, f + (int64_t)k * sys->n3, u + (int64_t)k * sys->n3, opts, &s);
+        if (stats) stats[k] = s;
+        if (rc != VFEM_OK && worst == VFEM_OK) worst = rc;
+        if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED) return rc;
+    }
+    return worst;
+}
+
+int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
+{
+    if (!sys || !x || !y || precision < 0 || precision > 2) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const int64_t n3 = sys->n3;
+    const int gf = grid_for_flat(n3);
+    HIP_TRY(hipMemcpy(sys->t64, x, n3 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0);
+    if (precision == VFEM_FP64) {
+        launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
+    } else {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->t64, sys->p32);
+        launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->ap32,
+                           sys->ap64);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(y, sys->ap64, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return VFEM_OK;
+}
+
+int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_options *opts)
+{
+    if (!sys || !r || !z) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    if (opts) o = *opts;
+    Plan P;
+    if (int rc = check_plan(sys, o, P)) return rc;
+    const int64_t n3 = sys->n3;
+    const int gf = grid_for_flat(n3);
+    HIP_TRY(hipMemcpy(sys->r, r, n3 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0);
+    enqueue_precond(sys, P, sys->r, FIN_STORE);
+    if (P.precision != VFEM_FP64 && sys->H.nlev > 1)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->z32,
+                           sys->z64);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(z, sys->z64, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return VFEM_OK;
+}
+
+int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
+{
+    if (!sys || !a || !b || !out) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const int64_t n3 = sys->n3;
+    HIP_TRY(hipMemcpy(sys->t64, a, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sys->ap64, b, n3 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, sys->stream, n3, sys->t64,
+                       sys->ap64, sys->st, sys->red, (int)FIN_STORE);
+    CGState h{};
+    if (int rc = read_state(sys, h)) return rc;
+    *out = h.result;
+    return VFEM_OK;
+}
+
+int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms)
+{
+    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 2) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    o.precision = precision;
+    Plan P;
+    if (int rc = check_plan(sys, o, P)) return rc;
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1 << 30, 0.0);
+    auto one = [&]() {
+        if (which == 0) {
+            if (P.precision == VFEM_FP64) launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0);
+            else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0);
+        } else if (which == 1) {
+            enqueue_precond(sys, P, sys->r, -1);
+        } else {
+            enqueue_iteration(sys, P, false);
+        }
+    };
+    one();
+    HIP_TRY(hipEventRecord(sys->ev0, sys->stream));
+    for (int i = 0; i < reps; i++) one();
+    HIP_TRY(hipEventRecord(sys->ev1, sys->stream));
+    HIP_TRY(hipEventSynchronize(sys->ev1));
+    HIP_TRY(hipGetLastError());
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+    *avg_ms = ms / reps;
+    return VFEM_OK;
+}
+
+}  // extern "C"

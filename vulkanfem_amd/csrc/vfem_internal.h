@@ -1,0 +1,57 @@
+// Internal data structures shared by the host-side hierarchy setup
+// (vfem_setup.cpp) and the device solver (vfem_device.hip).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vfem {
+
+constexpr int kMaxLevels = 16;
+
+// One multigrid level, stored as a grid: every node row (y, z) of the level's
+// node grid is split into 64-node segments.  A segment holds a 64-bit mask of
+// the used nodes and the compact id of its first used node, so the compact id
+// of node (x, y, z) is nbase[seg] + popcount(nbits[seg] & bits below x): the
+// reference's compact order (FEM.cpp:179-184) without any per-node index array.
+struct HostLevel {
+    int ex = 0, ey = 0, ez = 0;      // element extents
+    int nx = 0, ny = 0, nz = 0;      // node extents (element extents + 1)
+    int rdx = 0, rdy = 0, rdz = 0;   // reference level dims (FEM.cpp:256-259), stop rule only
+    int S = 0, SE = 0;               // 64-wide segments per node row / per element row
+    int div = 1;                     // coarsening factor from the previous level
+    int64_t ne = 0, nn = 0, nfixed = 0;
+    double fullscale = 1.0;          // Ke multiplier of full elements (FEM.cpp:463)
+    double partscale = 1.0;          // multiplier of the Galerkin sums of partial elements (FEM.cpp:381-383)
+    std::vector<uint64_t> nbits;     // [nz*ny*S] used nodes
+    std::vector<uint64_t> fbits;     // [nz*ny*S] fixed nodes
+    std::vector<int32_t> nbase;      // [nz*ny*S] compact id of the segment's first used node
+    std::vector<uint64_t> ebits;     // [ez*ey*SE] present elements
+    std::vector<uint8_t> epat;       // [ez*ey*ex] child pattern (div 2 levels), 0 = absent, 255 = full
+    std::vector<double> invdiag;     // [3*nn]
+};
+
+struct Hierarchy {
+    int nlev = 0, skip = 0;
+    double ke[576];                  // symmetrised from the upper triangle (FEM.cpp:200-210)
+    std::vector<HostLevel> L;
+    int64_t nfree = 0;               // free DOFs of the coarsest level
+    std::vector<int32_t> freedof;    // coarsest DOF index of each free DOF
+    std::vector<double> kcinv_ref;   // inverse of the reference Kc (FEM.cpp:498-513)
+    std::vector<double> kcinv_gal;   // inverse of Kc assembled from the level's element matrices
+    // element matrices per child pattern (256 x 576) for the operator of each
+    // div-2 coarse level; empty at level 0 (uniform Ke) and at a div-4 level
+    std::vector<std::vector<double>> ktab;
+};
+
+// Builds the hierarchy on the host.  Returns 0 or a vfem_status code; *err gets a message.
+int build_hierarchy(const int32_t *voxels, int nx, int ny, int nz, const double *ke, const int64_t *fixed,
+                    int64_t nfixed, int num_levels, int skip, Hierarchy &H, std::string &err);
+
+// Galerkin element matrix of a coarse element with the given child mask: sum over
+// present children c of P_c^T Ke P_c, P_c the trilinear map from the coarse
+// element's corners to child c's corners (div 2: 8 children, div 4: 64).
+void galerkin_element(const double *ke, int div, uint64_t mask, double *K);
+
+}  // namespace vfem
