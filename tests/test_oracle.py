@@ -166,3 +166,43 @@ def test_skip_level_hierarchy(ke0):
     np.testing.assert_allclose(s.invdiag(1), 1.0 / (P.T @ fine.A[0] @ P).diagonal(), rtol=1e-12)
     r = np.random.default_rng(5).standard_normal(s.ndof)
     np.testing.assert_allclose(s.restrict(0, r), P.T @ r, rtol=1e-12, atol=1e-12)
+
+
+def test_flexible_cg_tolerates_inexact_preconditioner(goldens, ke0):
+    """Why the reduced-precision solves use Polak-Ribiere beta: with FP32 storage
+    of p, A p and z and a cycle inexact at the 1e-6 level (as an FP32 cycle is),
+    Fletcher-Reeves PCG with FP64 residual replacement every 20 iterations
+    stalls, while flexible PCG converges (emulated with the FP64 oracle)."""
+    s = O.OracleSystem(np.ones((16, 16, 32), np.int32), ke0, goldens["box_fixed"], 3)
+    f = goldens["box_f"]
+    rhs = np.linalg.norm(f)
+
+    def f32(v):
+        return v.astype(np.float32).astype(np.float64)
+
+    def run(flex, seed):
+        rng = np.random.default_rng(seed)
+        M = lambda r: f32(s.vcycle(r, O.DAMPED, 0.6) * (1 + 1e-6 * rng.standard_normal(f.size)))  # noqa: E731
+        x, r = np.zeros_like(f), f.copy()
+        z = M(r)
+        p, zold, absnew = f32(z), z, r @ z
+        for it in range(150):
+            Ap = f32(s.matvec(f32(p)) * (1 + 1e-7 * rng.standard_normal(f.size)))
+            a = absnew / (p @ Ap)
+            x += a * p
+            r -= a * Ap
+            if it % 20 == 19:
+                r = f - s.matvec(x)
+            if np.linalg.norm(f - s.matvec(x)) < 1e-10 * rhs:
+                break
+            z = M(r)
+            absold, absnew = absnew, r @ z
+            beta = (absnew - (r @ zold if flex else 0.0)) / absold
+            zold = z
+            p = f32(z + beta * p)
+        return it, np.linalg.norm(f - s.matvec(x)) / rhs
+
+    it_flex, res_flex = run(True, 0)
+    it_fr, res_fr = run(False, 0)
+    assert res_flex < 1e-10 and it_flex < 40
+    assert res_fr > 1e-9  # stalls

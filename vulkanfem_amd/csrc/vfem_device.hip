@@ -74,8 +74,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__device__ void finalize(CGState *st, int fin, double v)
+__device__ void finalize(CGState *st, int fin, double v, double v2)
 {
+    if (st->no_stop) {
+        // timing runs: keep the recurrence finite and never stop
+        st->result = v;
+        st->alpha = 1e-30;
+        st->beta = 0.5;
+        return;
+    }
     switch (fin) {
     case FIN_STORE:
         st->result = v;
@@ -112,6 +119,16 @@ __device__ void finalize(CGState *st, int fin, double v)
         if (v == 0.0) st->done = 1;
         break;
     }
+    case FIN_RZ_FLEX:
+        // flexible CG (Notay 2000): beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k,
+        // robust to the rounding-level variability of a reduced-precision cycle
+        st->absOld = st->absNew;
+        st->absNew = v;
+        st->beta = (v - v2) / st->absOld;
+        st->iter += 1;
+        if (!isfinite(st->beta)) st->done = 3;
+        else if (st->iter >= st->maxit) st->done = 2;
+        break;
     case FIN_RR_TRUE:
         st->rr = v;
         st->last_true = 1;
@@ -122,20 +139,29 @@ __device__ void finalize(CGState *st, int fin, double v)
     }
 }
 
-// Single-pass grid reduction: block partials in fixed order, the last block to
-// arrive sums them in fixed order (deterministic) and runs the finalizer.
-__device__ void block_reduce_finalize(double v, Reduction red, CGState *st, int fin)
+// Single-pass grid reduction of up to two sums: block partials in fixed order,
+// the last block to arrive sums them in fixed order (deterministic) and runs
+// the finalizer.
+__device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGState *st, int fin)
 {
-    __shared__ double sh[kWaves];
+    __shared__ double sh[2][kWaves];
     __shared__ int am_last;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     v = wave_sum(v);
-    if (lane == 0) sh[wave] = v;
+    v2 = wave_sum(v2);
+    if (lane == 0) {
+        sh[0][wave] = v;
+        sh[1][wave] = v2;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double s = 0;
-        for (int w = 0; w < kWaves; w++) s += sh[w];
+        double s = 0, s2 = 0;
+        for (int w = 0; w < kWaves; w++) {
+            s += sh[0][w];
+            s2 += sh[1][w];
+        }
         __hip_atomic_store(&red.partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&red.partials[kMaxPartials + blockIdx.x], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         unsigned t = atomicAdd(red.counter, 1u);
         am_last = (t == gridDim.x - 1);
@@ -143,19 +169,33 @@ __device__ void block_reduce_finalize(double v, Reduction red, CGState *st, int 
     __syncthreads();
     if (!am_last) return;
     __threadfence();
-    double s = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+    double s = 0, s2 = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
         s += __hip_atomic_load(&red.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s2 += __hip_atomic_load(&red.partials[kMaxPartials + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     s = wave_sum(s);
+    s2 = wave_sum(s2);
     __syncthreads();
-    if (lane == 0) sh[wave] = s;
+    if (lane == 0) {
+        sh[0][wave] = s;
+        sh[1][wave] = s2;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double tot = 0;
-        for (int w = 0; w < kWaves; w++) tot += sh[w];
+        double tot = 0, tot2 = 0;
+        for (int w = 0; w < kWaves; w++) {
+            tot += sh[0][w];
+            tot2 += sh[1][w];
+        }
         __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finalize(st, fin, tot);
+        finalize(st, fin, tot, tot2);
     }
+}
+
+__device__ __forceinline__ void block_reduce_finalize(double v, Reduction red, CGState *st, int fin)
+{
+    block_reduce_finalize2(v, 0.0, red, st, fin);
 }
 
 // compact id of node (x, y, z) of level L, or -1 when unused / outside
@@ -176,6 +216,10 @@ template <typename T> __device__ __forceinline__ T ld(const T *p) { return *p; }
 // ---------------------------------------------------------------------------
 
 // Gathers x at the 27 neighbours of the wave's 64 nodes.  xv[k][c], k = (dz*3+dy)*3+dx.
+// Row validity is wave-uniform (scalar branch); inside a row every lane issues
+// its three loads unconditionally -- an unused neighbour reads the always-valid
+// entry 0 and is zeroed by a select -- so the 81 loads of a node are all in
+// flight together instead of being serialised behind per-load branches.
 template <typename T>
 __device__ __forceinline__ void gather27(const DevLevel &L, const T *__restrict__ x, int s, int yy, int zz, int lane,
                                          T (&xv)[27][3])
@@ -186,27 +230,45 @@ __device__ __forceinline__ void gather27(const DevLevel &L, const T *__restrict_
 #pragma unroll
         for (int dy = 0; dy < 3; dy++) {
             const int ry = yy + dy - 1, rz = zz + dz - 1;
-            uint64_t w = 0, wp = 0, wn = 0;
-            int base = 0;
-            if (ry >= 0 && ry < L.ny && rz >= 0 && rz < L.nz) {
-                const int64_t seg = ((int64_t)rz * L.ny + ry) * L.S + s;
-                w = L.nbits[seg];
-                base = L.nbase[seg];
-                if (s > 0) wp = L.nbits[seg - 1];
-                if (s + 1 < L.S) wn = L.nbits[seg + 1];
-            }
+            const int k = (dz * 3 + dy) * 3;
+            const bool rowok = ry >= 0 && ry < L.ny && rz >= 0 && rz < L.nz;  // wave-uniform
+            const int64_t seg = rowok ? ((int64_t)rz * L.ny + ry) * L.S + s : 0;
+            const uint64_t w = rowok ? L.nbits[seg] : 0;
+            const int base = L.nbase[seg];
+            const uint64_t wp = (rowok && s > 0) ? L.nbits[seg - 1] : 0;
+            const uint64_t wn = (rowok && s + 1 < L.S) ? L.nbits[seg + 1] : 0;
             const int idc = base + __popcll(w & lt);
             const bool vc = (w >> lane) & 1;
             const bool vm = lane ? ((w >> (lane - 1)) & 1) : (wp >> 63);
             const bool vp = (lane < 63) ? ((w >> (lane + 1)) & 1) : (wn & 1);
-            const int k = (dz * 3 + dy) * 3;
+            const int im = vm ? idc - 1 : 0, ic = vc ? idc : 0, ip = vp ? idc + (vc ? 1 : 0) : 0;
+            const T *pm = x + 3 * im, *pc = x + 3 * ic, *pp = x + 3 * ip;
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                xv[k + 0][c] = vm ? x[3 * (idc - 1) + c] : T(0);
-                xv[k + 1][c] = vc ? x[3 * idc + c] : T(0);
-                xv[k + 2][c] = vp ? x[3 * (idc + (vc ? 1 : 0)) + c] : T(0);
+                const T am = pm[c], ac = pc[c], ap = pp[c];
+                xv[k + 0][c] = vm ? am : T(0);
+                xv[k + 1][c] = vc ? ac : T(0);
+                xv[k + 2][c] = vp ? ap : T(0);
             }
         }
+    }
+}
+
+// Every element matrix (Ke and the Galerkin sums of the coarse levels) has the
+// rigid translations in its null space, so a per-component constant can be
+// subtracted from an element's nodal values without changing its product.
+// Subtracting the centre node's value from all 27 gathered neighbours makes the
+// contraction work on differences: the same exact result, but the FP32/BF16
+// rounding error scales with |x_m - x_n| instead of |x|, which is what keeps a
+// reduced-precision matvec accurate on the smooth vectors CG produces.  Absent
+// neighbours belong only to absent elements, whose terms are dropped.
+template <typename T> __device__ __forceinline__ void center_neighbours(T (&xv)[27][3])
+{
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const T ctr = xv[13][c];
+#pragma unroll
+        for (int k = 0; k < 27; k++) xv[k][c] -= ctr;
     }
 }
 
@@ -241,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
                                                          int honor_done)
 {
     if (honor_done && st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)L.nz * L.ny * L.S;
     const Range R = block_range(W);
     double dacc = 0;
@@ -253,7 +315,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
         const int yy = (int)(row % L.ny), zz = (int)(row / L.ny);
         T xv[27][3];
         gather27(L, x, s, yy, zz, lane, xv);
+        const T ctr[3] = {xv[13][0], xv[13][1], xv[13][2]};
+        center_neighbours(xv);
         const unsigned ring = element_ring(L, s, yy, zz, lane);
+        const T *kc = ke;
         T acc0 = 0, acc1 = 0, acc2 = 0;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
@@ -266,9 +331,9 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     const T v = xv[k][j];
-                    s0 += ke[(3 * Ln + 0) * 24 + 3 * m + j] * v;
-                    s1 += ke[(3 * Ln + 1) * 24 + 3 * m + j] * v;
-                    s2 += ke[(3 * Ln + 2) * 24 + 3 * m + j] * v;
+                    s0 += kc[(3 * Ln + 0) * 24 + 3 * m + j] * v;
+                    s1 += kc[(3 * Ln + 1) * 24 + 3 * m + j] * v;
+                    s2 += kc[(3 * Ln + 2) * 24 + 3 * m + j] * v;
                 }
             }
             if ((ring >> e) & 1) {
@@ -283,8 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
             y[3 * id + 0] = acc0;
             y[3 * id + 1] = acc1;
             y[3 * id + 2] = acc2;
-            dacc += (double)xv[13][0] * (double)acc0 + (double)xv[13][1] * (double)acc1 +
-                    (double)xv[13][2] * (double)acc2;
+            dacc += (double)ctr[0] * (double)acc0 + (double)ctr[1] * (double)acc1 + (double)ctr[2] * (double)acc2;
         }
     }
     if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
@@ -298,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
                                                            const CGState *st)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)L.nz * L.ny * L.S;
     const Range R = block_range(W);
     for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
@@ -309,6 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
         const int yy = (int)(row % L.ny), zz = (int)(row / L.ny);
         T xv[27][3];
         gather27(L, x, s, yy, zz, lane, xv);
+        center_neighbours(xv);
         const int xx = 64 * s + lane;
         T acc0 = 0, acc1 = 0, acc2 = 0;
         if ((wself >> lane) & 1) {
@@ -316,20 +381,21 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
             for (int e = 0; e < 8; e++) {
                 const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;
                 const int ex = xx - 1 + a, ey = yy - 1 + b, ez = zz - 1 + c;
-                if (ex < 0 || ex >= L.ex || ey < 0 || ey >= L.ey || ez < 0 || ez >= L.ez) continue;
-                const int p = L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex];
-                if (p == 0) continue;
-                const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
-                const T *K = tab + (int64_t)p * 576 + 3 * Ln * 24;
+                const bool inside = ex >= 0 && ex < L.ex && ey >= 0 && ey < L.ey && ez >= 0 && ez < L.ez;
+                const int p = inside ? L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex] : 0;
+                if (p != 0) {
+                    const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+                    const T *K = tab + (int64_t)p * 576 + 3 * Ln * 24;
 #pragma unroll
-                for (int m = 0; m < 8; m++) {
-                    const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
+                    for (int m = 0; m < 8; m++) {
+                        const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
 #pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        const T v = xv[k][j];
-                        acc0 += K[0 * 24 + 3 * m + j] * v;
-                        acc1 += K[1 * 24 + 3 * m + j] * v;
-                        acc2 += K[2 * 24 + 3 * m + j] * v;
+                        for (int j = 0; j < 3; j++) {
+                            const T v = xv[k][j];
+                            acc0 += K[0 * 24 + 3 * m + j] * v;
+                            acc1 += K[1 * 24 + 3 * m + j] * v;
+                            acc2 += K[2 * 24 + 3 * m + j] * v;
+                        }
                     }
                 }
             }
@@ -354,7 +420,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
                                                       const CGState *st)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)C.nz * C.ny * C.S;
     const Range R = block_range(W);
     const int div = C.div;
@@ -362,35 +428,79 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
     for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
         const uint64_t wself = C.nbits[it];
         if (wself == 0) continue;
-        if (!((wself >> lane) & 1)) continue;
         const int s = (int)(it % C.S);
         const int64_t row = it / C.S;
         const int Y = (int)(row % C.ny), Z = (int)(row / C.ny), X = 64 * s + lane;
         Tc a0 = 0, a1 = 0, a2 = 0;
-        for (int dz = 1 - div; dz < div; dz++) {
-            const int fz = div * Z + dz;
-            if (fz < 0 || fz >= F.nz) continue;
-            const Tc wz = Tc(1) - Tc(abs(dz) * inv);
-            for (int dy = 1 - div; dy < div; dy++) {
-                const int fy = div * Y + dy;
-                if (fy < 0 || fy >= F.ny) continue;
-                const Tc wzy = wz * (Tc(1) - Tc(abs(dy) * inv));
-                for (int dx = 1 - div; dx < div; dx++) {
-                    const int id = lookup(F, div * X + dx, fy, fz);
-                    if (id < 0) continue;
-                    const Tc w = wzy * (Tc(1) - Tc(abs(dx) * inv));
-                    Tc v0 = Tc(rf[3 * id + 0]), v1 = Tc(rf[3 * id + 1]), v2 = Tc(rf[3 * id + 2]);
-                    if (tf) {
-                        v0 -= Tc(tf[3 * id + 0]);
-                        v1 -= Tc(tf[3 * id + 1]);
-                        v2 -= Tc(tf[3 * id + 2]);
+        if (div == 2) {
+            // fine x = 2X + {-1, 0, +1}; 2X lies in fine word 2s (lanes < 32) or 2s+1
+            const int b = (2 * lane) & 63;
+            const uint64_t below = (1ull << b) - 1;
+            for (int dz = -1; dz <= 1; dz++) {
+                const int fz = 2 * Z + dz;
+                if (fz < 0 || fz >= F.nz) continue;
+                const Tc wz = dz ? Tc(0.5) : Tc(1);
+                for (int dy = -1; dy <= 1; dy++) {
+                    const int fy = 2 * Y + dy;
+                    if (fy < 0 || fy >= F.ny) continue;
+                    const Tc wzy = wz * (dy ? Tc(0.5) : Tc(1));
+                    const int64_t sb = ((int64_t)fz * F.ny + fy) * F.S;
+                    const int fs0 = 2 * s, fs1 = 2 * s + 1;
+                    const uint64_t wA = fs0 < F.S ? F.nbits[sb + fs0] : 0;
+                    const uint64_t wB = fs1 < F.S ? F.nbits[sb + fs1] : 0;
+                    const uint64_t wP = fs0 > 0 ? F.nbits[sb + fs0 - 1] : 0;
+                    const int bA = fs0 < F.S ? F.nbase[sb + fs0] : 0;
+                    const int bB = fs1 < F.S ? F.nbase[sb + fs1] : 0;
+                    const uint64_t w = lane < 32 ? wA : wB;
+                    const int id0 = (lane < 32 ? bA : bB) + __popcll(w & below);
+                    const bool v0 = (w >> b) & 1, vp = (w >> (b + 1)) & 1;
+                    const bool vm = b ? ((w >> (b - 1)) & 1) : (lane < 32 ? (wP >> 63) : (wA >> 63));
+                    const int i0 = v0 ? id0 : 0, ip = vp ? id0 + (v0 ? 1 : 0) : 0, im = vm ? id0 - 1 : 0;
+                    Tc q[3][3];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        q[0][c] = Tc(rf[3 * im + c]);
+                        q[1][c] = Tc(rf[3 * i0 + c]);
+                        q[2][c] = Tc(rf[3 * ip + c]);
+                        if (tf) {
+                            q[0][c] -= Tc(tf[3 * im + c]);
+                            q[1][c] -= Tc(tf[3 * i0 + c]);
+                            q[2][c] -= Tc(tf[3 * ip + c]);
+                        }
                     }
-                    a0 += w * v0;
-                    a1 += w * v1;
-                    a2 += w * v2;
+                    const Tc wm = vm ? wzy * Tc(0.5) : Tc(0), w0 = v0 ? wzy : Tc(0), wp = vp ? wzy * Tc(0.5) : Tc(0);
+                    a0 += wm * q[0][0] + w0 * q[1][0] + wp * q[2][0];
+                    a1 += wm * q[0][1] + w0 * q[1][1] + wp * q[2][1];
+                    a2 += wm * q[0][2] + w0 * q[1][2] + wp * q[2][2];
+                }
+            }
+        } else {
+            for (int dz = 1 - div; dz < div; dz++) {
+                const int fz = div * Z + dz;
+                if (fz < 0 || fz >= F.nz) continue;
+                const Tc wz = Tc(1) - Tc(abs(dz) * inv);
+                for (int dy = 1 - div; dy < div; dy++) {
+                    const int fy = div * Y + dy;
+                    if (fy < 0 || fy >= F.ny) continue;
+                    const Tc wzy = wz * (Tc(1) - Tc(abs(dy) * inv));
+                    for (int dx = 1 - div; dx < div; dx++) {
+                        const int id = lookup(F, div * X + dx, fy, fz);
+                        if (id < 0) continue;
+                        const Tc w = wzy * (Tc(1) - Tc(abs(dx) * inv));
+                        Tc v0 = Tc(rf[3 * id + 0]), v1 = Tc(rf[3 * id + 1]), v2 = Tc(rf[3 * id + 2]);
+                        if (tf) {
+                            v0 -= Tc(tf[3 * id + 0]);
+                            v1 -= Tc(tf[3 * id + 1]);
+                            v2 -= Tc(tf[3 * id + 2]);
+                        }
+                        a0 += w * v0;
+                        a1 += w * v1;
+                        a2 += w * v2;
+                    }
                 }
             }
         }
+        if (!((wself >> lane) & 1)) continue;
         const int id = C.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
         if (maskfixed && ((C.fbits[it] >> lane) & 1)) a0 = a1 = a2 = 0;
         rc[3 * id + 0] = a0;
@@ -410,14 +520,14 @@ template <typename Tc, typename Tr, typename Td, typename Tz>
 __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, const Tc *__restrict__ zc,
                                                      const Tr *__restrict__ rf, const Td *__restrict__ dinv,
                                                      Tz *__restrict__ zf, int mode, int maskfixed, CGState *st,
-                                                     Reduction red, int fin)
+                                                     Reduction red, int fin, const Tz *__restrict__ zold)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)F.nz * F.ny * F.S;
     const Range R = block_range(W);
     const int div = C.div;
-    double dacc = 0;
+    double dacc = 0, dacc2 = 0;
     for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
         const uint64_t wself = F.nbits[it];
         if (!((wself >> lane) & 1)) continue;
@@ -427,21 +537,44 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
         const int X0 = x / div, Y0 = y / div, Z0 = z / div;
         const int rx = x - div * X0, ry = y - div * Y0, rz = z - div * Z0;
         Tz a0 = 0, a1 = 0, a2 = 0;
-        for (int cz = 0; cz < 2; cz++) {
-            if (cz && !rz) break;
-            const Tz wz = cz ? Tz(rz) / Tz(div) : Tz(1) - Tz(rz) / Tz(div);
-            for (int cy = 0; cy < 2; cy++) {
-                if (cy && !ry) break;
-                const Tz wy = cy ? Tz(ry) / Tz(div) : Tz(1) - Tz(ry) / Tz(div);
-                for (int cx = 0; cx < 2; cx++) {
-                    if (cx && !rx) break;
-                    const Tz wx = cx ? Tz(rx) / Tz(div) : Tz(1) - Tz(rx) / Tz(div);
-                    const int id = lookup(C, X0 + cx, Y0 + cy, Z0 + cz);
-                    if (id < 0) continue;
-                    const Tz w = wz * wy * wx;
-                    a0 += w * Tz(zc[3 * id + 0]);
-                    a1 += w * Tz(zc[3 * id + 1]);
-                    a2 += w * Tz(zc[3 * id + 2]);
+        if (div == 2) {
+            // coarse x = X0 (and X0+1 when x is odd); X0 lies in coarse word s/2
+            const int cs = s >> 1, b = X0 & 63;
+            const uint64_t below = (1ull << b) - 1;
+            for (int cz = 0; cz <= rz; cz++) {
+                const Tz wz = rz ? Tz(0.5) : Tz(1);
+                for (int cy = 0; cy <= ry; cy++) {
+                    const Tz wzy = wz * (ry ? Tz(0.5) : Tz(1));
+                    const int64_t seg = ((int64_t)(Z0 + cz) * C.ny + (Y0 + cy)) * C.S + cs;
+                    const uint64_t w = C.nbits[seg];
+                    const uint64_t wn = cs + 1 < C.S ? C.nbits[seg + 1] : 0;
+                    const int id0 = C.nbase[seg] + __popcll(w & below);
+                    const bool v0 = (w >> b) & 1;
+                    const bool v1 = rx && (b < 63 ? ((w >> (b + 1)) & 1) : (wn & 1));
+                    const int i0 = v0 ? id0 : 0, i1 = v1 ? id0 + (v0 ? 1 : 0) : 0;
+                    const Tz w0 = v0 ? wzy * (rx ? Tz(0.5) : Tz(1)) : Tz(0), w1 = v1 ? wzy * Tz(0.5) : Tz(0);
+                    a0 += w0 * Tz(zc[3 * i0 + 0]) + w1 * Tz(zc[3 * i1 + 0]);
+                    a1 += w0 * Tz(zc[3 * i0 + 1]) + w1 * Tz(zc[3 * i1 + 1]);
+                    a2 += w0 * Tz(zc[3 * i0 + 2]) + w1 * Tz(zc[3 * i1 + 2]);
+                }
+            }
+        } else {
+            for (int cz = 0; cz < 2; cz++) {
+                if (cz && !rz) break;
+                const Tz wz = cz ? Tz(rz) / Tz(div) : Tz(1) - Tz(rz) / Tz(div);
+                for (int cy = 0; cy < 2; cy++) {
+                    if (cy && !ry) break;
+                    const Tz wy = cy ? Tz(ry) / Tz(div) : Tz(1) - Tz(ry) / Tz(div);
+                    for (int cx = 0; cx < 2; cx++) {
+                        if (cx && !rx) break;
+                        const Tz wx = cx ? Tz(rx) / Tz(div) : Tz(1) - Tz(rx) / Tz(div);
+                        const int id = lookup(C, X0 + cx, Y0 + cy, Z0 + cz);
+                        if (id < 0) continue;
+                        const Tz w = wz * wy * wx;
+                        a0 += w * Tz(zc[3 * id + 0]);
+                        a1 += w * Tz(zc[3 * id + 1]);
+                        a2 += w * Tz(zc[3 * id + 2]);
+                    }
                 }
             }
         }
@@ -459,45 +592,47 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
             }
             if (fx) v = 0;
             zf[3 * id + c] = v;
-            if (fin >= 0) dacc += (double)rf[3 * id + c] * (double)v;
+            if (fin >= 0) {
+                dacc += (double)rf[3 * id + c] * (double)v;
+                if (zold) dacc2 += (double)rf[3 * id + c] * (double)zold[3 * id + c];
+            }
         }
     }
-    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+    if (fin >= 0) block_reduce_finalize2(dacc, dacc2, red, st, fin);
 }
 
-// Damped Jacobi on one level.
+// Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
 //   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
-// fixed rows zeroed; fin >= 0 reduces r.z.
+// fixed rows zeroed (compact-order fixed bitmask); fin >= 0 reduces r.z (and
+// r.z_old for flexible CG).
 enum { SMOOTH_INIT = 0, SMOOTH_POST = 1 };
 
 template <typename Tr, typename Td, typename Tz>
-__global__ __launch_bounds__(kBlock) void k_smooth(DevLevel L, const Tr *__restrict__ r, const Tz *__restrict__ t,
+__global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *__restrict__ fixc,
+                                                    const Tr *__restrict__ r, const Tz *__restrict__ t,
                                                     const Td *__restrict__ dinv, Tz *__restrict__ z, double omega,
-                                                    int mode, CGState *st, Reduction red, int fin)
+                                                    int mode, CGState *st, Reduction red, int fin,
+                                                    const Tz *__restrict__ zold)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t W = (int64_t)L.nz * L.ny * L.S;
-    const Range R = block_range(W);
+    const Range R = block_range(n3);
     const Tz om = Tz(omega);
-    double dacc = 0;
-    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
-        const uint64_t wself = L.nbits[it];
-        if (!((wself >> lane) & 1)) continue;
-        const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
-        const bool fx = (L.fbits[it] >> lane) & 1;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const int64_t k = 3 * (int64_t)id + c;
-            Tz v;
-            if (mode == SMOOTH_INIT) v = om * Tz(dinv[k]) * Tz(r[k]);
-            else v = z[k] + om * Tz(dinv[k]) * (Tz(r[k]) - t[k]);
-            if (fx) v = 0;
-            z[k] = v;
-            if (fin >= 0) dacc += (double)r[k] * (double)v;
+    double dacc = 0, dacc2 = 0;
+    for (int64_t k = R.begin + threadIdx.x; k < R.end; k += kBlock) {
+        const int64_t node = k / 3;
+        const bool fx = (fixc[node >> 5] >> (node & 31)) & 1;
+        const Tr rv = r[k];
+        Tz v;
+        if (mode == SMOOTH_INIT) v = om * Tz(dinv[k]) * Tz(rv);
+        else v = z[k] + om * Tz(dinv[k]) * (Tz(rv) - t[k]);
+        if (fx) v = 0;
+        z[k] = v;
+        if (fin >= 0) {
+            dacc += (double)rv * (double)v;
+            if (zold) dacc2 += (double)rv * (double)zold[k];
         }
     }
-    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+    if (fin >= 0) block_reduce_finalize2(dacc, dacc2, red, st, fin);
 }
 
 // Coarsest level: z[freedof[i]] = sum_k Kc^-1[i][k] r[freedof[k]] (replaces the
@@ -509,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_solve(const Tk *__restrict__ 
                                                           const CGState *st)
 {
     if (st->done) return;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rowi = blockIdx.x * kWaves + wave;
     if (rowi >= n) return;
     const Tk *Krow = kinv + (int64_t)rowi * n;
@@ -543,14 +678,19 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 }
 
 // ConjugateGradient_2.comp: p = z + beta p
+// (flexible CG: also keeps z for the next Polak-Ribiere beta)
 template <typename Tz, typename Tp>
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
-                                                        const CGState *st)
+                                                        Tz *__restrict__ zold, const CGState *st)
 {
     if (st->done) return;
     const double beta = st->beta;
     const Range R = block_range(n);
-    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) p[i] = Tp((double)z[i] + beta * (double)p[i]);
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
+        const Tz zv = z[i];
+        p[i] = Tp((double)zv + beta * (double)p[i]);
+        if (zold) zold[i] = zv;
+    }
 }
 
 // x = 0, r = f, |f|^2 (x0 = 0)   or   r = f - t, |r|^2 (true residual / warm start)
@@ -572,8 +712,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__
 
 template <typename Ta, typename Tb>
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict__ a, const Tb *__restrict__ b,
-                                                CGState *st, Reduction red, int fin)
+                                                CGState *st, Reduction red, int fin, int honor_done)
 {
+    if (honor_done && st->done) return;
     const Range R = block_range(n);
     double acc = 0;
     for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) acc += (double)a[i] * (double)b[i];
@@ -587,9 +728,10 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts *__restr
     for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) d[i] = Td(s[i]);
 }
 
-__global__ void k_set_state(CGState *st, int maxit, double tol)
+__global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->no_stop = no_stop;
         st->iter = 0;
         st->done = 0;
         st->last_true = 0;
@@ -649,6 +791,7 @@ struct vfem_system {
     DevLevel dl[kMaxLevels];
     double *ke64 = nullptr;
     float *ke32 = nullptr;
+    uint32_t *fixc[kMaxLevels] = {};   // compact-order fixed-node bitmask per level
     double *dinv64[kMaxLevels] = {};
     float *dinv32[kMaxLevels] = {};
     double *ktab64[kMaxLevels] = {};
@@ -659,7 +802,7 @@ struct vfem_system {
     int64_t n3 = 0;
     // level-0 vectors
     double *x = nullptr, *r = nullptr, *f = nullptr, *t64 = nullptr, *p64 = nullptr, *ap64 = nullptr, *z64 = nullptr;
-    float *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr;
+    float *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr, *zold32 = nullptr;
     // coarse-level vectors (index = level), t = operator scratch of the damped cycle
     double *rl64[kMaxLevels] = {}, *zl64[kMaxLevels] = {}, *tl64[kMaxLevels] = {};
     float *rl32[kMaxLevels] = {}, *zl32[kMaxLevels] = {}, *tl32[kMaxLevels] = {};
@@ -729,6 +872,17 @@ int upload_system(vfem_system *S)
         if (!L.epat.empty())
             if (int rc = S->upload(&ep, L.epat)) return rc;
         D.nbits = nb; D.fbits = fb; D.ebits = eb; D.nbase = base; D.epat = ep;
+        {
+            std::vector<uint32_t> fc((L.nn + 31) / 32 + 1, 0);
+            int64_t k = 0;
+            for (size_t q = 0; q < L.nbits.size(); q++)
+                for (int b = 0; b < 64; b++)
+                    if ((L.nbits[q] >> b) & 1) {
+                        if ((L.fbits[q] >> b) & 1) fc[k >> 5] |= 1u << (k & 31);
+                        k++;
+                    }
+            if (int rc = S->upload(&S->fixc[l], fc)) return rc;
+        }
         if (int rc = S->upload(&S->dinv64[l], L.invdiag)) return rc;
         if (int rc = S->upload(&S->dinv32[l], convert<float>(L.invdiag))) return rc;
         if (l >= 1 && l < (int)H.ktab.size() && !H.ktab[l].empty()) {
@@ -756,6 +910,7 @@ int upload_system(vfem_system *S)
     if (int rc = S->alloc(&S->p32, n3)) return rc;
     if (int rc = S->alloc(&S->ap32, n3)) return rc;
     if (int rc = S->alloc(&S->z32, n3)) return rc;
+    if (int rc = S->alloc(&S->zold32, n3)) return rc;
     if (int rc = S->alloc(&S->t0_32, n3)) return rc;
     for (int l = 1; l < H.nlev; l++) {
         int64_t m = 3 * H.L[l].nn;
@@ -767,7 +922,7 @@ int upload_system(vfem_system *S)
         if (int rc = S->alloc(&S->tl32[l], m)) return rc;
     }
     if (int rc = S->alloc(&S->st, 1)) return rc;
-    if (int rc = S->alloc(&S->red.partials, 4096)) return rc;
+    if (int rc = S->alloc(&S->red.partials, 2 * kMaxPartials)) return rc;
     if (int rc = S->alloc(&S->red.counter, 1)) return rc;
     return 0;
 }
@@ -829,7 +984,7 @@ template <typename T> void launch_coarse_solve(vfem_system *S, const T *rc, T *z
 // device: restrict down, coarse solve, prolong + 2 D^-1 r up.  r is the FP64
 // level-0 residual; z (type T) gets the masked preconditioned residual and
 // r.z is reduced with finalizer fin.
-template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T *z, int fin)
+template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T *z, int fin, const T *zold)
 {
     const int nl = S->H.nlev;
     for (int l = 0; l < nl - 1; l++) {
@@ -846,17 +1001,19 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
         const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
         hipLaunchKernelGGL((k_prolong<T, T, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
                            (const T *)level_z<T>(S, l + 1), (const T *)level_r<T>(S, l), dinv<T>(S, l),
-                           level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1);
+                           level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1, (const T *)nullptr);
     }
     const DevLevel &F = S->dl[0], &C = S->dl[1];
     hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
-                       (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin);
+                       (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin,
+                       zold);
 }
 
 // Damped multiplicative V(1,1) cycle, level l >= 0.  rl: level residual
 // (level 0: FP64 CG residual), zl: output, tl: operator scratch.
 template <typename T, typename TR>
-void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin)
+void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin,
+                                 const T *zold)
 {
     const int nl = S->H.nlev;
     if (l == nl - 1) {
@@ -864,19 +1021,21 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         return;
     }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
-    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, rl, (const T *)tl,
-                       dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1);
+    const int64_t n3l = 3 * L.nn;
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l)), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
+                       (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
+                       (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, L, C, rl,
                        (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
     enqueue_vcycle_damped_level<T, T>(S, l + 1, level_r<T>(S, l + 1), level_z<T>(S, l + 1), level_t<T>(S, l + 1),
-                                      omega, -1);
+                                      omega, -1, (const T *)nullptr);
     hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, C,
                        (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
-                       -1);
+                       -1, (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
-    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, rl, (const T *)tl,
-                       dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin);
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l)), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
+                       (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
 }
 
 struct Plan {
@@ -885,22 +1044,31 @@ struct Plan {
     double omega;
 };
 
-// z = M r on the level-0 FP64 residual r, reducing r.z with `fin`
+// z = M r on the level-0 FP64 residual r, reducing r.z with `fin`.  The
+// reduced-precision cycles run flexible CG: FIN_RZ becomes FIN_RZ_FLEX, which
+// also reduces r.z_old.
+bool flexible(const vfem_system *S, const Plan &P) { return P.precision != VFEM_FP64 && S->H.nlev > 1; }
+
 void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 {
     if (S->H.nlev == 1) {
         // single level: the cycle is the exact coarse solve, z = Kc^-1 r (FP64), then r.z
         launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
         hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(S->n3)), dim3(kBlock), 0, S->stream, S->n3, r,
-                           S->z64, S->st, S->red, fin);
+                           S->z64, S->st, S->red, fin, 1);
         return;
     }
     if (P.precision == VFEM_FP64) {
-        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<double>(S, r, S->z64, fin);
-        else enqueue_vcycle_damped_level<double, double>(S, 0, r, S->z64, S->t64, P.omega, fin);
+        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<double>(S, r, S->z64, fin, nullptr);
+        else enqueue_vcycle_damped_level<double, double>(S, 0, r, S->z64, S->t64, P.omega, fin, nullptr);
     } else {
-        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin);
-        else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin);
+        const float *zold = nullptr;
+        if (fin == FIN_RZ && flexible(S, P)) {
+            fin = FIN_RZ_FLEX;
+            zold = S->zold32;
+        }
+        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin, zold);
+        else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin, zold);
     }
 }
 
@@ -919,13 +1087,13 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
     const int gf = grid_for_flat(n3);
     if (P.precision == VFEM_FP64)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           S->st);
+                           (double *)nullptr, S->st);
     else if (S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
-                           S->st);
+                           (double *)nullptr, S->st);
     else
         hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
-                           S->st);
+                           S->zold32, S->st);
 }
 
 // one CG iteration (FEMGPU.cpp:471-541 / Eigen ConjugateGradient loop body).
@@ -1011,7 +1179,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             if (int rc = get_iteration_graph(S, P, true, &g_true)) return rc;
     }
     HIP_TRY(hipEventRecord(S->ev0, S->stream));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, S->stream, S->st, o.max_iter, o.tol);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, S->stream, S->st, o.max_iter, o.tol, 0);
     // |f|^2 and the threshold; r = f with x = 0, or r = f - A x0
     hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr, S->r,
                        warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
@@ -1019,7 +1187,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
         enqueue_true_residual(S);
     else  // initial r.r test (Eigen: residualNorm2 < threshold -> 0 iterations)
         hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->r, S->r, S->st,
-                           S->red, (int)FIN_RR);
+                           S->red, (int)FIN_RR, 0);
     HIP_TRY(hipMemsetAsync(S->p64, 0, n3 * sizeof(double), S->stream));
     HIP_TRY(hipMemsetAsync(S->p32, 0, n3 * sizeof(float), S->stream));
     // p = M r, absNew = r.p
@@ -1306,7 +1474,7 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(hipMemcpy(sys->t64, x, n3 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0, 0);
     if (precision == VFEM_FP64) {
         launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
     } else {
@@ -1335,7 +1503,7 @@ int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_o
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(hipMemcpy(sys->r, r, n3 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0, 0);
     enqueue_precond(sys, P, sys->r, FIN_STORE);
     if (P.precision != VFEM_FP64 && sys->H.nlev > 1)
         hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->z32,
@@ -1356,7 +1524,7 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
     HIP_TRY(hipMemcpy(sys->t64, a, n3 * sizeof(double), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(sys->ap64, b, n3 * sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, sys->stream, n3, sys->t64,
-                       sys->ap64, sys->st, sys->red, (int)FIN_STORE);
+                       sys->ap64, sys->st, sys->red, (int)FIN_STORE, 0);
     CGState h{};
     if (int rc = read_state(sys, h)) return rc;
     *out = h.result;
@@ -1374,7 +1542,7 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     o.precision = precision;
     Plan P;
     if (int rc = check_plan(sys, o, P)) return rc;
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1 << 30, 0.0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1 << 30, 0.0, 1);
     auto one = [&]() {
         if (which == 0) {
             if (P.precision == VFEM_FP64) launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0);

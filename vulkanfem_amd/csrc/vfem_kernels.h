@@ -43,6 +43,8 @@ struct CGState {
     int maxit;
     int done;          // 0 running, 1 converged, 2 iteration cap, 3 breakdown
     int last_true;     // 1 if the last convergence test used the FP64 true residual
+    int no_stop;       // kernel timing: finalizers never end the iteration
+    int pad;
 };
 
 enum Finalize {
@@ -52,11 +54,14 @@ enum Finalize {
     FIN_RZ_INIT = 3,  // absNew = v (first direction)
     FIN_RZ = 4,       // absOld = absNew; absNew = v; beta; iter++
     FIN_RHS = 5,      // rhs2 = v; threshold; zero right-hand side -> done
-    FIN_RR_TRUE = 6   // rr = v (true residual); converged test, else continue
+    FIN_RR_TRUE = 6,  // rr = v (true residual); converged test, else continue
+    FIN_RZ_FLEX = 7   // v = r.z, v2 = r.z_old: Polak-Ribiere beta (flexible CG); iter++
 };
 
+constexpr int kMaxPartials = 4096;
+
 struct Reduction {
-    double *partials;   // [gridDim.x]
+    double *partials;   // [2 * kMaxPartials]: block partials of up to two sums
     unsigned *counter;  // zero between launches (the last block resets it)
 };
 
