@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-kernels", type=int, default=0,
+                    help="only launch the level-0 matvec variants and the copy reference this many times each "
+                         "(for rocprofv3 counter runs), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -83,6 +86,13 @@ def main():
     n_vox = int(np.prod(p.voxels.shape))
     log(f"setup {t_setup:.2f} s, levels {A.num_levels}")
     A.set_rhs(p.loads[0])
+    if args.profile_kernels:
+        for which, prec_k, name in ((0, V.FP32, "matvec fp32 6-pass"), (3, V.FP32, "matvec fp32 3-pass"),
+                                    (0, V.BF16, "matvec bf16 1-pass"), (0, V.FP64, "matvec fp64 valu"),
+                                    (4, V.FP32, "copy fp32 vector")):
+            ms = A.time_kernel(which, prec_k, reps=args.profile_kernels)
+            log(f"{name}: {ms * 1e3:.1f} us")
+        return
     prec = V.FP32 if args.precision == "fp32" else V.BF16
     mixed = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=0.6, true_res_every=20)
 
@@ -118,11 +128,14 @@ def main():
     # level-0 matvec: HIP-event timing on the library's stream
     t_mv32 = A.time_kernel(0, V.FP32, reps=20) / 1e3
     t_mv64 = A.time_kernel(0, V.FP64, reps=20) / 1e3
+    t_mv16 = A.time_kernel(0, V.BF16, reps=20) / 1e3
+    t_mv32c = A.time_kernel(3, V.FP32, reps=20) / 1e3
     t_it = A.time_kernel(2, prec, reps=5) / 1e3
     b32 = matvec_bytes(n_nodes, n_vox, 4)
     achieved = b32 / t_mv32 / 1e9
     out_extra.update({"matvec_gdofs_fp32": round(A.n / t_mv32 / 1e9, 2), "matvec_gdofs_fp64": round(A.n / t_mv64 / 1e9, 2),
                       "matvec_ms_fp32": round(t_mv32 * 1e3, 4), "matvec_ms_fp64": round(t_mv64 * 1e3, 4),
+                      "matvec_ms_bf16_1pass": round(t_mv16 * 1e3, 4), "matvec_ms_fp32_3pass": round(t_mv32c * 1e3, 4),
                       "iteration_ms": round(t_it * 1e3, 4), "iterations": stats[-1]["iterations"],
                       "rel_residual": stats[-1]["rel_residual"], "gdof_iter_per_s":
                           round(A.n * stats[-1]["iterations"] / t_mixed / 1e9, 2),

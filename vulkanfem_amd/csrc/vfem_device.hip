@@ -161,7 +161,7 @@ __device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGSta
             s2 += sh[1][w];
         }
         __hip_atomic_store(&red.partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&red.partials[kMaxPartials + blockIdx.x], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&red.partials[red.cap + blockIdx.x], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         unsigned t = atomicAdd(red.counter, 1u);
         am_last = (t == gridDim.x - 1);
@@ -172,7 +172,7 @@ __device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGSta
     double s = 0, s2 = 0;
     for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
         s += __hip_atomic_load(&red.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s2 += __hip_atomic_load(&red.partials[kMaxPartials + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s2 += __hip_atomic_load(&red.partials[red.cap + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     s = wave_sum(s);
     s2 = wave_sum(s2);
@@ -350,6 +350,359 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
             y[3 * id + 2] = acc2;
             dacc += (double)ctr[0] * (double)acc0 + (double)ctr[1] * (double)acc1 + (double)ctr[2] * (double)acc2;
         }
+    }
+    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+}
+
+// ---------------------------------------------------------------------------
+// Level-0 matvec on the matrix cores: element form Y_e = Ke X_e on
+// v_mfma_f32_16x16x32_bf16, then an atomic-free node gather.
+//
+// A block owns a tile of MT_X x MT_Y output nodes in x-y and marches through a
+// chunk of MZ node planes.  For every element layer it stages the two node
+// planes it touches in LDS (64 nodes per row = one wavefront, so the row's
+// bitmask words are wave-uniform scalar loads), computes the 24-DOF products of
+// MT_X+1 x MT_Y+1 elements in groups of 16 (N = 16 elements, K = 24 DOFs padded
+// to 32, M = 24 rows in two 16-row tiles), parks them in LDS, and each thread
+// sums the 8 element contributions of the nodes it owns in a fixed order: the
+// lower four come from the current layer, the upper four were kept in registers
+// from the previous layer.  x planes are prefetched two planes ahead.
+//
+// Precision: each element vector is centred on its first node (Ke annihilates
+// rigid translations, so the product is unchanged) and split x = hi + lo in
+// BF16, Ke likewise; PASSES = 3 sums hi*hi + hi*lo + lo*hi (about 2^-16
+// relative to |Ke||x_e - x_e0|), PASSES = 1 keeps hi*hi only (BF16 arithmetic).
+// Accumulation is FP32.
+// ---------------------------------------------------------------------------
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int MT_X = 62;               // output nodes per tile row
+constexpr int MT_Y = 7;                // output node rows per tile
+constexpr int MN_Y = MT_Y + 2;         // staged node rows (with halo)
+constexpr int ME_Y = MT_Y + 1;         // element rows
+constexpr int ME_X = 64;               // element slots per row (63 real + 1 padding)
+constexpr int MZ = 16;                 // node planes per block
+constexpr int YS_STRIDE = 25;          // padded row of an element's 24 results
+constexpr int MROWS_PER_WAVE = (MN_Y + kWaves - 1) / kWaves;  // 3
+constexpr int MOWN = (MT_X * MT_Y + kBlock - 1) / kBlock;      // 2 output nodes per thread
+
+struct MfmaTiles {
+    int tx, ty, tz;  // tile counts in x, y, z
+};
+
+inline MfmaTiles mfma_tiles(const DevLevel &L)
+{
+    return {(L.nx + MT_X - 1) / MT_X, (L.ny + MT_Y - 1) / MT_Y, (L.nz + MZ - 1) / MZ};
+}
+
+// compact id of node (gx, gy, gz) where gx = x_first + lane, from the row's bit
+// mask (wave-uniform words).  All loads are unconditional with clamped, in-bounds
+// indices (no branch around a load); `valid` says whether the node is used.
+__device__ __forceinline__ int row_node_id(const DevLevel &L, int gx, int gy, int gz, int lane, bool &valid)
+{
+    const bool rowok = gy >= 0 && gy < L.ny && gz >= 0 && gz < L.nz;
+    const int64_t rowseg = rowok ? ((int64_t)gz * L.ny + gy) * L.S : 0;
+    const int xb = gx - lane;  // x of lane 0, wave-uniform (>= -1)
+    const int s0 = xb >> 6;    // -1 when xb = -1
+    const int sa = max(s0, 0), sb = min(s0 + 1, L.S - 1);
+    const uint64_t w0 = L.nbits[rowseg + sa], w1 = L.nbits[rowseg + sb];
+    const int b0 = L.nbase[rowseg + sa], b1 = L.nbase[rowseg + sb];
+    const bool inx = gx >= 0 && gx < L.nx;
+    const int s = inx ? gx >> 6 : sa, b = inx ? gx & 63 : 0;
+    const uint64_t w = (s == s0) ? w0 : w1;
+    const int base = (s == s0) ? b0 : b1;
+    valid = rowok && inx && ((w >> b) & 1);
+    return base + __popcll(w & ((1ull << b) - 1));
+}
+
+__device__ __forceinline__ bool element_present(const DevLevel &L, int ex, int ey, int ez)
+{
+    if (ex < 0 || ex >= L.ex || ey < 0 || ey >= L.ey || ez < 0 || ez >= L.ez) return false;
+    return (L.ebits[((int64_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1;
+}
+
+// Workgroup barrier that only orders LDS: waits for this wave's LDS (and scalar)
+// accesses, not for its outstanding global loads, so prefetches stay in flight
+// across the barrier (__syncthreads() would drain them).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__device__ __forceinline__ void split_bf16(float v, __bf16 &h, __bf16 &m, __bf16 &l, int parts)
+{
+    h = (__bf16)v;
+    if (parts == 1) return;
+    const float r1 = v - (float)h;
+    m = (__bf16)r1;
+    if (parts == 2) return;
+    l = (__bf16)(r1 - (float)m);
+}
+
+// Bit masks of a block's whole chunk, loaded once into LDS by the prologue:
+// for every staged node plane the two row words covering x in [x0-1, x0+63),
+// for every element layer the element words, for every output plane the
+// used/fixed words and bases covering [x0, x0+62).
+constexpr int MPL = MZ + 2;  // staged planes zb-1 .. zb+MZ
+struct MfmaMasks {
+    uint64_t rw[MPL][MN_Y][2];
+    int rb[MPL][MN_Y][2];
+    uint64_t ew[MZ + 1][ME_Y][2];
+    uint64_t nw[MZ][MT_Y][2], fw[MZ][MT_Y][2];
+    int nb[MZ][MT_Y][2];
+};
+
+template <int PASSES, typename Tx, typename Ty>
+__global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float *__restrict__ ke,
+                                                         const Tx *__restrict__ x, Ty *__restrict__ y, CGState *st,
+                                                         Reduction red, int fin, int honor_done, MfmaTiles T)
+{
+    if (honor_done && st->done) return;
+    // PASSES 1: hi*hi.  3: two-way split, hi*hi + hi*lo + lo*hi.  6: three-way split,
+    // all products of combined order < 3 (about FP32 accuracy).
+    constexpr int PARTS = PASSES == 1 ? 1 : (PASSES == 3 ? 2 : 3);
+    __shared__ float Xs[2][MN_Y][64][3];
+    __shared__ float Ys[ME_Y * ME_X * YS_STRIDE];
+    __shared__ MfmaMasks K;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int b = remap_block();
+    const int tx = b % T.tx;
+    b /= T.tx;
+    const int ty = b % T.ty;
+    const int tz = b / T.ty;
+    const int x0 = tx * MT_X, y0 = ty * MT_Y, zb = tz * MZ;
+    const int ze = min(zb + MZ, L.nz);
+    const int ew0 = (x0 - 1) >> 6;  // element word of the tile's first element column (-1 when x0 = 0)
+    const int nw0 = x0 >> 6;
+    const int rs0 = (x0 - 1) >> 6;  // node word of the staged rows' first node (-1 when x0 = 0)
+
+    // ---- prologue: all masks of the chunk into LDS --------------------------
+    for (int q = threadIdx.x; q < MPL * MN_Y * 2; q += kBlock) {
+        const int wsel = q & 1, r = (q >> 1) % MN_Y, pl = (q >> 1) / MN_Y;
+        const int gz = zb - 1 + pl, gy = y0 - 1 + r;
+        const int w = rs0 + wsel;
+        const bool ok = gz >= 0 && gz < L.nz && gy >= 0 && gy < L.ny && w >= 0 && w < L.S;
+        const int64_t idx = ok ? ((int64_t)gz * L.ny + gy) * L.S + w : 0;
+        const uint64_t v = L.nbits[idx];
+        const int bs = L.nbase[idx];
+        K.rw[pl][r][wsel] = ok ? v : 0;
+        K.rb[pl][r][wsel] = ok ? bs : 0;
+    }
+    for (int q = threadIdx.x; q < (MZ + 1) * ME_Y * 2; q += kBlock) {
+        const int wsel = q & 1, r = (q >> 1) % ME_Y, ly = (q >> 1) / ME_Y;
+        const int lz = zb - 1 + ly, ey = y0 - 1 + r, w = ew0 + wsel;
+        const bool ok = lz >= 0 && lz < L.ez && ey >= 0 && ey < L.ey && w >= 0 && w < L.SE;
+        const uint64_t v = L.ebits[ok ? ((int64_t)lz * L.ey + ey) * L.SE + w : 0];
+        K.ew[ly][r][wsel] = ok ? v : 0;
+    }
+    for (int q = threadIdx.x; q < MZ * MT_Y * 2; q += kBlock) {
+        const int wsel = q & 1, r = (q >> 1) % MT_Y, pl = (q >> 1) / MT_Y;
+        const int gz = zb + pl, gy = y0 + r, w = nw0 + wsel;
+        const bool ok = gz < L.nz && gy < L.ny && w < L.S;
+        const int64_t idx = ok ? ((int64_t)gz * L.ny + gy) * L.S + w : 0;
+        const uint64_t nv = L.nbits[idx], fv = L.fbits[idx];
+        const int bv = L.nbase[idx];
+        K.nw[pl][r][wsel] = ok ? nv : 0;
+        K.fw[pl][r][wsel] = ok ? fv : 0;
+        K.nb[pl][r][wsel] = ok ? bv : 0;
+    }
+
+    // A operand: Ke rows 16*t + (lane % 16), columns 8*(lane/16) + j, split in three
+    bf16x8 a1[2], a2[2], a3[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int row = 16 * t + (lane & 15), col = 8 * (lane >> 4) + j;
+            const float v = (row < 24 && col < 24) ? ke[row * 24 + col] : 0.0f;
+            __bf16 h, m, l;
+            split_bf16(v, h, m, l, 3);
+            a1[t][j] = h;
+            a2[t][j] = m;
+            a3[t][j] = l;
+        }
+    }
+    __syncthreads();
+
+    // staged-plane loads: wave w handles node rows r = w + 4 i; plane gz is local
+    // plane gz - (zb - 1).  The loaded values are not touched until they are
+    // written to LDS (a select on them would make the compiler wait for the load);
+    // validity goes into a bit mask.
+    constexpr int PRE = 3;
+    auto fetch = [&](int gz, float (&dst)[MROWS_PER_WAVE][3], unsigned &valid) {
+        valid = 0;
+        const int pl = min(max(gz - (zb - 1), 0), MPL - 1);
+        const bool plok = gz >= zb - 1 && gz <= zb + MZ;
+        const int gx = x0 - 1 + lane;
+        const bool inx = gx >= 0 && gx < L.nx;
+        const int s = inx ? (gx >> 6) - rs0 : 0, bit = inx ? gx & 63 : 0;
+#pragma unroll
+        for (int i = 0; i < MROWS_PER_WAVE; i++) {
+            const int r = min(wave + kWaves * i, MN_Y - 1);
+            const uint64_t w = K.rw[pl][r][s];
+            const bool v = plok && inx && ((w >> bit) & 1);
+            const int id = K.rb[pl][r][s] + __popcll(w & ((1ull << bit) - 1));
+            const int idc = v ? id : 0;
+            dst[i][0] = (float)x[3 * idc + 0];
+            dst[i][1] = (float)x[3 * idc + 1];
+            dst[i][2] = (float)x[3 * idc + 2];
+            valid |= (v ? 1u : 0u) << i;
+        }
+    };
+    auto store = [&](int buf, const float (&src)[MROWS_PER_WAVE][3], unsigned valid) {
+#pragma unroll
+        for (int i = 0; i < MROWS_PER_WAVE; i++) {
+            const int r = wave + kWaves * i;
+            const bool v = (valid >> i) & 1;
+            if (r < MN_Y) {
+                Xs[buf][r][lane][0] = v ? src[i][0] : 0.0f;
+                Xs[buf][r][lane][1] = v ? src[i][1] : 0.0f;
+                Xs[buf][r][lane][2] = v ? src[i][2] : 0.0f;
+            }
+        }
+    };
+
+    float acc[MOWN][3];
+    float xc[MOWN][3];
+#pragma unroll
+    for (int k = 0; k < MOWN; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+    double dacc = 0;
+
+    // prefetch ring of PRE planes in three fixed register sets (no register moves:
+    // a copy of a register with a load in flight would wait for the load).
+    float s0[MROWS_PER_WAVE][3], s1[MROWS_PER_WAVE][3], s2[MROWS_PER_WAVE][3];
+    unsigned v0 = 0, v1 = 0, v2 = 0;
+    {
+        float tmp[MROWS_PER_WAVE][3];
+        unsigned tv;
+        fetch(zb - 1, tmp, tv);
+        store((zb - 1) & 1, tmp, tv);
+        fetch(zb, tmp, tv);
+        store(zb & 1, tmp, tv);
+        fetch(zb + 1, s1, v1);
+        fetch(zb + 2, s2, v2);
+        fetch(zb + 3, s0, v0);
+    }
+    __syncthreads();
+
+    // one element layer; `slot` holds plane ez+2 and is refilled with plane ez+2+PRE
+    auto step = [&](int ez, float (&slot)[MROWS_PER_WAVE][3], unsigned &sv) {
+        const int bl = ez & 1, bu = (ez + 1) & 1;
+        const int ly = ez - (zb - 1);  // local element layer / staged plane of ez
+        const bool live = ez < ze;
+        bool layer = false;
+#pragma unroll
+        for (int r = 0; r < ME_Y; r++) layer |= (K.ew[min(ly, MZ)][r][0] | K.ew[min(ly, MZ)][r][1]) != 0;
+        layer = layer && live;
+        // centre values of the owned nodes on plane ez (for the fused x.Ax)
+#pragma unroll
+        for (int k = 0; k < MOWN; k++) {
+            const int n = threadIdx.x + kBlock * k;
+            const int a = n % MT_X, bb = n / MT_X;
+            if (n < MT_X * MT_Y) {
+                xc[k][0] = Xs[bl][bb + 1][a + 1][0];
+                xc[k][1] = Xs[bl][bb + 1][a + 1][1];
+                xc[k][2] = Xs[bl][bb + 1][a + 1][2];
+            }
+        }
+        if (layer) {
+            // wave w: element rows 0..ME_Y-1, element columns 16 w .. 16 w + 15
+            const int j = lane & 15, g = lane >> 4;
+            const int exl = 16 * wave + j;
+            const int ex = x0 - 1 + exl;
+            const int ewl = (ex >> 6) - ew0, ebit = ex & 63;
+            for (int eyl = 0; eyl < ME_Y; eyl++) {
+                const bool pres = exl < ME_X - 1 && ex >= 0 && ((K.ew[ly][eyl][ewl] >> ebit) & 1);
+                const int cx = min(exl, ME_X - 2);
+                const float c0 = Xs[bl][eyl][cx][0], c1 = Xs[bl][eyl][cx][1], c2 = Xs[bl][eyl][cx][2];
+                bf16x8 b1, b2, b3;
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    const int k = 8 * g + t;  // element DOF
+                    float v = 0.0f;
+                    if (k < 24) {
+                        const int Ln = k / 3, c = k - 3 * Ln;
+                        const float xv = Xs[(Ln >> 2) ? bu : bl][eyl + ((Ln >> 1) & 1)][cx + (Ln & 1)][c];
+                        v = xv - (c == 0 ? c0 : (c == 1 ? c1 : c2));
+                    }
+                    __bf16 h, m, l;
+                    split_bf16(v, h, m, l, PARTS);
+                    b1[t] = h;
+                    if (PARTS > 1) b2[t] = m;
+                    if (PARTS > 2) b3[t] = l;
+                }
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+                    // small terms first
+                    if (PASSES == 6) {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b3, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t], b2, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[t], b1, d, 0, 0, 0);
+                    }
+                    if (PASSES >= 3) {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b2, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t], b1, d, 0, 0, 0);
+                    }
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b1, d, 0, 0, 0);
+                    // lane (j, g) holds rows 16 t + 4 g + q of element j
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int row = 16 * t + 4 * g + q;
+                        if (row < 24) Ys[(eyl * ME_X + exl) * YS_STRIDE + row] = pres ? d[q] : 0.0f;
+                    }
+                }
+            }
+        }
+        lds_barrier();
+        // node gather: plane ez gets acc (upper halves of layer ez-1) + lower halves of layer ez
+        const int pz = min(max(ez - zb, 0), MZ - 1);  // local output plane
+#pragma unroll
+        for (int k = 0; k < MOWN; k++) {
+            const int n = threadIdx.x + kBlock * k;
+            const bool own = n < MT_X * MT_Y;
+            const int a = own ? n % MT_X : 0, bb = own ? n / MT_X : 0;
+            float lo[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
+            if (layer && own) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int ex = e & 1, ey = e >> 1;
+                    const float *Yp = &Ys[((bb + ey) * ME_X + (a + ex)) * YS_STRIDE];
+                    const int Ll = (1 - ex) + 2 * (1 - ey);
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        lo[c] += Yp[3 * Ll + c];
+                        up[c] += Yp[3 * (Ll + 4) + c];
+                    }
+                }
+            }
+            if (own && live && ez >= zb) {
+                const int gx = x0 + a;
+                const int wl = (gx >> 6) - nw0, bit = gx & 63;
+                const uint64_t w = K.nw[pz][bb][wl];
+                if ((w >> bit) & 1) {
+                    const int id = K.nb[pz][bb][wl] + __popcll(w & ((1ull << bit) - 1));
+                    const bool fx = (K.fw[pz][bb][wl] >> bit) & 1;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const float o = fx ? 0.0f : acc[k][c] + lo[c];
+                        y[3 * id + c] = (Ty)o;
+                        dacc += (double)xc[k][c] * (double)o;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[k][c] = up[c];
+        }
+        // plane ez+2 (prefetched) replaces plane ez in LDS; refill the set with plane ez+2+PRE
+        store(bl, slot, sv);
+        fetch(ez + 2 + PRE, slot, sv);
+        lds_barrier();
+    };
+
+    // steps past the end of the chunk write nothing (their node planes are >= ze)
+    for (int ez = zb - 1; ez < ze; ez += 3) {
+        step(ez, s1, v1);
+        step(ez + 1, s2, v2);
+        step(ez + 2, s0, v0);
     }
     if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
 }
@@ -812,6 +1165,8 @@ struct vfem_system {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::map<int, hipGraphExec_t> graphs;
     double setup_ms = 0;
+    bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
+    bool vcycle_bf16 = false;   // set while enqueueing a BF16-precision cycle
 
     template <typename T> int alloc(T **p, size_t n, bool zero = true)
     {
@@ -922,7 +1277,11 @@ int upload_system(vfem_system *S)
         if (int rc = S->alloc(&S->tl32[l], m)) return rc;
     }
     if (int rc = S->alloc(&S->st, 1)) return rc;
-    if (int rc = S->alloc(&S->red.partials, 2 * kMaxPartials)) return rc;
+    {
+        const MfmaTiles t = mfma_tiles(S->dl[0]);
+        S->red.cap = std::max<int>(kMaxPartials, t.tx * t.ty * t.tz);
+    }
+    if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap)) return rc;
     if (int rc = S->alloc(&S->red.counter, 1)) return rc;
     return 0;
 }
@@ -953,10 +1312,29 @@ template <typename T> const T *kinv(vfem_system *S, bool gal);
 template <> const double *kinv<double>(vfem_system *S, bool gal) { return gal ? S->kinv_gal64 : S->kinv_ref64; }
 template <> const float *kinv<float>(vfem_system *S, bool gal) { return gal ? S->kinv_gal32 : S->kinv_ref32; }
 
+// level-0 matvec.  FP64: node-centric VALU kernel.  FP32: matrix cores with
+// `passes` BF16 products -- 6 (three-way split, about FP32 accuracy) for the CG
+// operator, 3 inside the FP32 cycle, 1 inside the BF16 cycle.
 template <typename T>
-void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done)
+void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done, int passes = 6)
 {
     const DevLevel &L = S->dl[0];
+    if constexpr (sizeof(T) == 4) {
+        if (S->use_mfma) {
+            const MfmaTiles t = mfma_tiles(L);
+            const int grid = t.tx * t.ty * t.tz;
+            if (passes == 1)
+                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                                   y, S->st, S->red, fin, honor_done, t);
+            else if (passes == 3)
+                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                                   y, S->st, S->red, fin, honor_done, t);
+            else
+                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                                   y, S->st, S->red, fin, honor_done, t);
+            return;
+        }
+    }
     hipLaunchKernelGGL(k_matvec_fine<T>, dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, kefine<T>(S), x, y,
                        S->st, S->red, fin, honor_done);
 }
@@ -964,7 +1342,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
 template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x, T *y)
 {
     if (l == 0) {
-        launch_matvec_fine<T>(S, x, y, -1, 1);
+        launch_matvec_fine<T>(S, x, y, -1, 1, S->vcycle_bf16 ? 1 : 3);
         return;
     }
     const DevLevel &L = S->dl[l];
@@ -1039,7 +1417,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
 }
 
 struct Plan {
-    int precision;  // VFEM_FP64 / VFEM_FP32
+    int precision;  // VFEM_FP64 / VFEM_FP32 (storage and arithmetic of the cycle)
+    bool bf16;      // VFEM_BF16: the cycle's level-0 matvecs use one BF16 MFMA pass
     int cycle;
     double omega;
 };
@@ -1067,8 +1446,10 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
             fin = FIN_RZ_FLEX;
             zold = S->zold32;
         }
+        S->vcycle_bf16 = P.bf16;
         if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin, zold);
         else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin, zold);
+        S->vcycle_bf16 = false;
     }
 }
 
@@ -1124,6 +1505,7 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
         return VFEM_ERR_ARGUMENT;
     }
     P.precision = o.precision == VFEM_FP64 ? VFEM_FP64 : VFEM_FP32;
+    P.bf16 = o.precision == VFEM_BF16;
     P.cycle = o.cycle;
     P.omega = o.omega;
     if (o.cycle == VFEM_CYCLE_DAMPED) {
@@ -1138,7 +1520,7 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
 
 int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphExec_t *out)
 {
-    int key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0);
+    int key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0) + (P.bf16 ? 2 : 0);
     // omega is baked into the graph's kernel arguments
     key += (int)std::lround(P.omega * 1000) * 64;
     auto it = S->graphs.find(key);
@@ -1479,7 +1861,7 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
         launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
     } else {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->t64, sys->p32);
-        launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0);
+        launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, precision == VFEM_BF16 ? 1 : 6);
         hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->ap32,
                            sys->ap64);
     }
@@ -1533,7 +1915,7 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
 
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms)
 {
-    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 2) {
+    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 4) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
@@ -1546,7 +1928,12 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     auto one = [&]() {
         if (which == 0) {
             if (P.precision == VFEM_FP64) launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0);
-            else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0);
+            else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, P.bf16 ? 1 : 6);
+        } else if (which == 3) {
+            launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, 3);
+        } else if (which == 4) {
+            hipLaunchKernelGGL((k_convert<float, float>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0, sys->stream,
+                               sys->n3, sys->p32, sys->ap32);
         } else if (which == 1) {
             enqueue_precond(sys, P, sys->r, -1);
         } else {

@@ -61,8 +61,9 @@ enum Finalize {
 constexpr int kMaxPartials = 4096;
 
 struct Reduction {
-    double *partials;   // [2 * kMaxPartials]: block partials of up to two sums
+    double *partials;   // [2 * cap]: block partials of up to two sums
     unsigned *counter;  // zero between launches (the last block resets it)
+    int cap;            // >= the grid of any reducing launch
 };
 
 constexpr int kBlock = 256;
