@@ -505,13 +505,17 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         K.nb[pl][r][wsel] = ok ? bv : 0;
     }
 
-    // A operand: Ke rows 16*t + (lane % 16), columns 8*(lane/16) + j, split in three
+    // A operand: Ke rows 16*t + (lane % 16).  The K index is ordered by node pairs:
+    // k = 8 g + 3 m + c holds DOF c of local node 2 g + m (m = 0, 1), k % 8 = 6, 7
+    // are zero padding -- so a B lane group g reads two x-adjacent nodes of one
+    // plane row.  Split in three.
     bf16x8 a1[2], a2[2], a3[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            const int row = 16 * t + (lane & 15), col = 8 * (lane >> 4) + j;
+            const int row = 16 * t + (lane & 15), g = lane >> 4;
+            const int col = j < 6 ? 3 * (2 * g + j / 3) + j % 3 : 24;
             const float v = (row < 24 && col < 24) ? ke[row * 24 + col] : 0.0f;
             __bf16 h, m, l;
             split_bf16(v, h, m, l, 3);
@@ -604,25 +608,34 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             }
         }
         if (layer) {
-            // wave w: element rows 0..ME_Y-1, element columns 16 w .. 16 w + 15
+            // wave w: element rows 0..ME_Y-1, element columns 16 w .. 16 w + 15.  Lane
+            // (j, g) builds DOFs of local nodes 2g, 2g+1 of element j: plane g >> 1,
+            // row offset g & 1, the two x-adjacent nodes cx, cx + 1.
             const int j = lane & 15, g = lane >> 4;
             const int exl = 16 * wave + j;
             const int ex = x0 - 1 + exl;
             const int ewl = (ex >> 6) - ew0, ebit = ex & 63;
+            const int cx = min(exl, ME_X - 2);
+            const float *pb = &Xs[(g >> 1) ? bu : bl][g & 1][cx][0];  // node pair of row eyl = 0
+            const float *pc = &Xs[bl][0][cx][0];                      // centre (local node 0)
+            constexpr int RSTR = 64 * 3;                              // floats per staged row
             for (int eyl = 0; eyl < ME_Y; eyl++) {
                 const bool pres = exl < ME_X - 1 && ex >= 0 && ((K.ew[ly][eyl][ewl] >> ebit) & 1);
-                const int cx = min(exl, ME_X - 2);
-                const float c0 = Xs[bl][eyl][cx][0], c1 = Xs[bl][eyl][cx][1], c2 = Xs[bl][eyl][cx][2];
+                const float *q = pb + eyl * RSTR, *qc = pc + eyl * RSTR;
+                const float c0 = qc[0], c1 = qc[1], c2 = qc[2];
+                float vv[8];
+                vv[0] = q[0] - c0;
+                vv[1] = q[1] - c1;
+                vv[2] = q[2] - c2;
+                vv[3] = q[3] - c0;
+                vv[4] = q[4] - c1;
+                vv[5] = q[5] - c2;
+                vv[6] = 0.0f;
+                vv[7] = 0.0f;
                 bf16x8 b1, b2, b3;
 #pragma unroll
                 for (int t = 0; t < 8; t++) {
-                    const int k = 8 * g + t;  // element DOF
-                    float v = 0.0f;
-                    if (k < 24) {
-                        const int Ln = k / 3, c = k - 3 * Ln;
-                        const float xv = Xs[(Ln >> 2) ? bu : bl][eyl + ((Ln >> 1) & 1)][cx + (Ln & 1)][c];
-                        v = xv - (c == 0 ? c0 : (c == 1 ? c1 : c2));
-                    }
+                    const float v = pres ? vv[t] : 0.0f;
                     __bf16 h, m, l;
                     split_bf16(v, h, m, l, PARTS);
                     b1[t] = h;
@@ -647,7 +660,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
                         const int row = 16 * t + 4 * g + q;
-                        if (row < 24) Ys[(eyl * ME_X + exl) * YS_STRIDE + row] = pres ? d[q] : 0.0f;
+                        if (row < 24) Ys[(eyl * ME_X + exl) * YS_STRIDE + row] = d[q];
                     }
                 }
             }
