@@ -34,6 +34,19 @@ def matvec_bytes(n_nodes, n_vox, s):
     return 6 * s * n_nodes + math.ceil(n_vox / 8) + math.ceil(n_nodes / 8)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter
+    summary (rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each scaled by the
+    same counter's reading for a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh)
+        return rec[workload][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(p, nlev, iterations, sample_iters, threads):
     """The oracle (C restatement of the reference CPU path, additive cycle, FP64)
     on the host cores: setup plus `sample_iters` PCG iterations, scaled to the
@@ -130,16 +143,21 @@ def main():
     t_mv64 = A.time_kernel(0, V.FP64, reps=20) / 1e3
     t_mv16 = A.time_kernel(0, V.BF16, reps=20) / 1e3
     t_mv32c = A.time_kernel(3, V.FP32, reps=20) / 1e3
+    t_copy = A.time_kernel(4, V.FP32, reps=20) / 1e3
     t_it = A.time_kernel(2, prec, reps=5) / 1e3
     b32 = matvec_bytes(n_nodes, n_vox, 4)
     achieved = b32 / t_mv32 / 1e9
     out_extra.update({"matvec_gdofs_fp32": round(A.n / t_mv32 / 1e9, 2), "matvec_gdofs_fp64": round(A.n / t_mv64 / 1e9, 2),
-                      "matvec_ms_fp32": round(t_mv32 * 1e3, 4), "matvec_ms_fp64": round(t_mv64 * 1e3, 4),
+                      "matvec_gdofs_bf16_1pass": round(A.n / t_mv16 / 1e9, 2),
+                      "matvec_ms_fp32_6pass": round(t_mv32 * 1e3, 4), "matvec_ms_fp64": round(t_mv64 * 1e3, 4),
                       "matvec_ms_bf16_1pass": round(t_mv16 * 1e3, 4), "matvec_ms_fp32_3pass": round(t_mv32c * 1e3, 4),
+                      "copy_fp32_vector_gbs": round(2 * 4 * A.n / t_copy / 1e9, 1),
                       "iteration_ms": round(t_it * 1e3, 4), "iterations": stats[-1]["iterations"],
                       "rel_residual": stats[-1]["rel_residual"], "gdof_iter_per_s":
                           round(A.n * stats[-1]["iterations"] / t_mixed / 1e9, 2),
                       "setup_s": round(t_setup, 3), "ndof": A.n})
+    kernel = "k_matvec_mfma<6,float,float>"
+    traffic = measured_traffic(kernel, p.name)
     cpu = None
     if not args.no_cpu and "ref_cycle_fp64_iterations" in out_extra:
         threads = min(16, os.cpu_count() or 1)
@@ -160,8 +178,9 @@ def main():
         "config": {"workload": p.name, "voxels": list(p.voxels.shape[::-1]), "ndof": A.n, "levels": A.num_levels,
                    "cycle": "damped V(1,1), omega 0.6", "tol": 1e-8, "true_res_every": 20},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "k_matvec_fine<float> (level-0 matvec)", "bytes_per_launch": b32},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": kernel + " (level-0 element matvec: CG operator and cycle)",
+                     "algorithmic_bytes_per_launch": b32},
         "cpu_baseline": cpu,
     }
     rec.update(out_extra)

@@ -65,6 +65,9 @@ typedef struct {
     int32_t check_every;     /* iterations between host convergence polls, default 8 */
     int32_t use_graph;       /* capture one CG iteration in a hipGraph (default 1) */
     int32_t warm_start;      /* 1: start from the u passed in instead of zero */
+    int32_t op_passes;       /* reduced precision: BF16 products of the CG operator matvec, 3 or 6; 0 = 6 */
+    int32_t cycle_passes;    /* reduced precision: BF16 products of the level-0 cycle matvec, 1, 3 or 6;
+                                0 = 6 (VFEM_FP32) or 1 (VFEM_BF16) */
 } vfem_options;
 
 typedef struct {

@@ -37,7 +37,8 @@ class VfemError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("cycle", ctypes.c_int32), ("omega", ctypes.c_double), ("true_res_every", ctypes.c_int32),
-                ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
+                ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32),
+                ("op_passes", ctypes.c_int32), ("cycle_passes", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

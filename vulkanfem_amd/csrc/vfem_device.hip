@@ -67,6 +67,24 @@ __device__ __forceinline__ Range block_range(int64_t W)
     return {beg, end};
 }
 
+// Flat range of a block for streaming kernels: a multiple of kBlock items, so
+// every wavefront access starts on a 64-item boundary.
+__device__ __forceinline__ Range flat_range(int64_t n)
+{
+    int64_t G = gridDim.x;
+    int64_t b = remap_block();
+    int64_t per = ((n + G - 1) / G + kBlock - 1) / kBlock * kBlock;
+    int64_t beg = b * per;
+    if (beg > n) beg = n;
+    int64_t end = beg + per < n ? beg + per : n;
+    return {beg, end};
+}
+
+// Streaming loops issue FLAT_U independent loads per thread before using any of
+// them (item i + u kBlock, u < FLAT_U); tail items use a clamped index and are
+// not stored.
+constexpr int FLAT_U = 4;
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -981,21 +999,40 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
                                                     const Tz *__restrict__ zold)
 {
     if (st->done) return;
-    const Range R = block_range(n3);
+    const Range R = flat_range(n3);
     const Tz om = Tz(omega);
     double dacc = 0, dacc2 = 0;
-    for (int64_t k = R.begin + threadIdx.x; k < R.end; k += kBlock) {
-        const int64_t node = k / 3;
-        const bool fx = (fixc[node >> 5] >> (node & 31)) & 1;
-        const Tr rv = r[k];
-        Tz v;
-        if (mode == SMOOTH_INIT) v = om * Tz(dinv[k]) * Tz(rv);
-        else v = z[k] + om * Tz(dinv[k]) * (Tz(rv) - t[k]);
-        if (fx) v = 0;
-        z[k] = v;
-        if (fin >= 0) {
-            dacc += (double)rv * (double)v;
-            if (zold) dacc2 += (double)rv * (double)zold[k];
+    for (int64_t k0 = R.begin + threadIdx.x; k0 < R.end; k0 += kBlock * FLAT_U) {
+        Tr rv[FLAT_U];
+        Td dv[FLAT_U];
+        Tz zv[FLAT_U], tv[FLAT_U], ov[FLAT_U];
+        uint32_t fw[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t k = min(k0 + u * kBlock, R.end - 1);
+            rv[u] = r[k];
+            dv[u] = dinv[k];
+            fw[u] = fixc[(k / 3) >> 5];
+            if (mode != SMOOTH_INIT) {
+                zv[u] = z[k];
+                tv[u] = t[k];
+            }
+            if (fin >= 0 && zold) ov[u] = zold[k];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t k = k0 + u * kBlock;
+            if (k >= R.end) break;
+            const bool fx = (fw[u] >> ((k / 3) & 31)) & 1;
+            Tz v;
+            if (mode == SMOOTH_INIT) v = om * Tz(dv[u]) * Tz(rv[u]);
+            else v = zv[u] + om * Tz(dv[u]) * (Tz(rv[u]) - tv[u]);
+            if (fx) v = 0;
+            z[k] = v;
+            if (fin >= 0) {
+                dacc += (double)rv[u] * (double)v;
+                if (zold) dacc2 += (double)rv[u] * (double)ov[u];
+            }
         }
     }
     if (fin >= 0) block_reduce_finalize2(dacc, dacc2, red, st, fin);
@@ -1032,13 +1069,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 {
     if (st->done) return;
     const double alpha = st->alpha;
-    const Range R = block_range(n);
+    const Range R = flat_range(n);
     double acc = 0;
-    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
-        x[i] += alpha * (double)p[i];
-        const double rv = r[i] - alpha * (double)ap[i];
-        r[i] = rv;
-        acc += rv * rv;
+    for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
+        double xv[FLAT_U], rv[FLAT_U];
+        Tp pv[FLAT_U];
+        Ta av[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = min(i0 + u * kBlock, R.end - 1);
+            xv[u] = x[i];
+            rv[u] = r[i];
+            pv[u] = p[i];
+            av[u] = ap[i];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = i0 + u * kBlock;
+            if (i >= R.end) break;
+            x[i] = xv[u] + alpha * (double)pv[u];
+            const double rn = rv[u] - alpha * (double)av[u];
+            r[i] = rn;
+            acc += rn * rn;
+        }
     }
     block_reduce_finalize(acc, red, st, FIN_RR);
 }
@@ -1051,11 +1104,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__re
 {
     if (st->done) return;
     const double beta = st->beta;
-    const Range R = block_range(n);
-    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
-        const Tz zv = z[i];
-        p[i] = Tp((double)zv + beta * (double)p[i]);
-        if (zold) zold[i] = zv;
+    const Range R = flat_range(n);
+    for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
+        Tz zv[FLAT_U];
+        Tp pv[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = min(i0 + u * kBlock, R.end - 1);
+            zv[u] = z[i];
+            pv[u] = p[i];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = i0 + u * kBlock;
+            if (i >= R.end) break;
+            p[i] = Tp((double)zv[u] + beta * (double)pv[u]);
+            if (zold) zold[i] = zv[u];
+        }
     }
 }
 
@@ -1081,9 +1146,21 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict_
                                                 CGState *st, Reduction red, int fin, int honor_done)
 {
     if (honor_done && st->done) return;
-    const Range R = block_range(n);
+    const Range R = flat_range(n);
     double acc = 0;
-    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) acc += (double)a[i] * (double)b[i];
+    for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
+        Ta av[FLAT_U];
+        Tb bv[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = min(i0 + u * kBlock, R.end - 1);
+            av[u] = a[i];
+            bv[u] = b[i];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++)
+            if (i0 + u * kBlock < R.end) acc += (double)av[u] * (double)bv[u];
+    }
     block_reduce_finalize(acc, red, st, fin);
 }
 
@@ -1176,10 +1253,10 @@ struct vfem_system {
     CGState *st = nullptr;
     Reduction red{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::map<int, hipGraphExec_t> graphs;
+    std::map<int64_t, hipGraphExec_t> graphs;
     double setup_ms = 0;
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
-    bool vcycle_bf16 = false;   // set while enqueueing a BF16-precision cycle
+    int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
 
     template <typename T> int alloc(T **p, size_t n, bool zero = true)
     {
@@ -1355,7 +1432,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
 template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x, T *y)
 {
     if (l == 0) {
-        launch_matvec_fine<T>(S, x, y, -1, 1, S->vcycle_bf16 ? 1 : 3);
+        launch_matvec_fine<T>(S, x, y, -1, 1, S->cycle_passes);
         return;
     }
     const DevLevel &L = S->dl[l];
@@ -1431,9 +1508,11 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
 
 struct Plan {
     int precision;  // VFEM_FP64 / VFEM_FP32 (storage and arithmetic of the cycle)
-    bool bf16;      // VFEM_BF16: the cycle's level-0 matvecs use one BF16 MFMA pass
+    bool bf16;         // VFEM_BF16 requested
     int cycle;
     double omega;
+    int op_passes;     // BF16 MFMA products of the reduced-precision CG operator (3 or 6)
+    int cycle_passes;  // BF16 MFMA products of the cycle's level-0 matvecs (1, 3 or 6)
 };
 
 // z = M r on the level-0 FP64 residual r, reducing r.z with `fin`.  The
@@ -1459,10 +1538,10 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
             fin = FIN_RZ_FLEX;
             zold = S->zold32;
         }
-        S->vcycle_bf16 = P.bf16;
+        S->cycle_passes = P.cycle_passes;
         if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin, zold);
         else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin, zold);
-        S->vcycle_bf16 = false;
+        S->cycle_passes = 6;
     }
 }
 
@@ -1502,7 +1581,7 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p64, S->ap64, S->st, S->red);
     } else {
-        launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1);
+        launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p32, S->ap32, S->st, S->red);
     }
@@ -1521,6 +1600,13 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     P.bf16 = o.precision == VFEM_BF16;
     P.cycle = o.cycle;
     P.omega = o.omega;
+    P.op_passes = o.op_passes ? o.op_passes : 6;
+    P.cycle_passes = o.cycle_passes ? o.cycle_passes : (P.bf16 ? 1 : 6);
+    if ((P.op_passes != 3 && P.op_passes != 6) ||
+        (P.cycle_passes != 1 && P.cycle_passes != 3 && P.cycle_passes != 6)) {
+        g_last_error = "op_passes must be 3 or 6 and cycle_passes 1, 3 or 6";
+        return VFEM_ERR_ARGUMENT;
+    }
     if (o.cycle == VFEM_CYCLE_DAMPED) {
         for (int l = 1; l < S->H.nlev; l++)
             if (S->H.L[l].div != 2) {
@@ -1533,9 +1619,10 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
 
 int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphExec_t *out)
 {
-    int key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0) + (P.bf16 ? 2 : 0);
+    int64_t key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0) + (P.bf16 ? 2 : 0);
+    key += 1024 * (P.op_passes + 8 * P.cycle_passes);
     // omega is baked into the graph's kernel arguments
-    key += (int)std::lround(P.omega * 1000) * 64;
+    key += std::llround(P.omega * 1000) * 1024 * 64;
     auto it = S->graphs.find(key);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -1675,6 +1762,8 @@ void vfem_default_options(vfem_options *o)
     o->check_every = 8;
     o->use_graph = 1;
     o->warm_start = 0;
+    o->op_passes = 0;
+    o->cycle_passes = 0;
 }
 
 const char *vfem_status_string(int status)
@@ -1941,7 +2030,7 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     auto one = [&]() {
         if (which == 0) {
             if (P.precision == VFEM_FP64) launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0);
-            else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, P.bf16 ? 1 : 6);
+            else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, P.bf16 ? 1 : P.op_passes);
         } else if (which == 3) {
             launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, 3);
         } else if (which == 4) {
