@@ -109,3 +109,51 @@ def test_apply_boundary_conditions():
     assert list(f) == [0, 0, 0, 1, 2, 3, 0, 0, 0, 0, 0, -1]
     with pytest.raises(V.VfemError):
         V.apply_boundary_conditions(np.zeros(6), {5: (1.0, 0.0, 0.0)})
+
+
+CXX_SHIM_PROGRAM = r"""
+#include "vfem.hpp"
+#include <cstdio>
+int main(int argc, char **argv)
+{
+    if (argc > 1) {  // instantiates the whole interface; only run with a GPU
+        int vox[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+        double ke[24][24] = {};
+        std::set<uint64_t> fixed = {0, 1, 2, 3};
+        auto K = vfem::assembleSystemMatrix<double>(vox, vfem::Vec3i{2, 2, 2}, ke, fixed, 2, 0);
+        std::map<uint64_t, vfem::Vec3<double>> loads = {{20, {0.0, 0.0, -1.0}}};
+        std::vector<double> f(K.rows()), u;
+        vfem::applyBoundaryConditions(f, loads, fixed);
+        vfem::solveWithKompute(K, f, u);
+        vfem::solveWithHIP(K, f, u);
+        vfem::MultigridPreconditioner M;
+        M.compute(K);
+        std::vector<double> z = M.solve(f), y = K * u;
+        std::printf("%zu %zu\n", z.size(), y.size());
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    std::printf("%g %d\n", o.tol, o.max_iter);
+    return 0;
+}
+"""
+
+
+def test_cxx_shim_compiles_and_links(tmp_path):
+    """include/vfem.hpp (the solveWithKompute / MatrixFreeSparse /
+    MultigridPreconditioner layer) compiles with g++ and links against the
+    library; the program runs its host-only part."""
+    import os
+    import subprocess
+    import shutil
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "shim.cpp"
+    src.write_text(CXX_SHIM_PROGRAM)
+    exe = tmp_path / "shim"
+    libdir = os.path.dirname(V.api.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(root, "include"), str(src),
+                    "-L", libdir, "-lvfem", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    assert out.split() == ["1e-08", "30000"]
