@@ -46,7 +46,11 @@ enum vfem_status {
 enum vfem_precision {
     VFEM_FP64 = 0,  /* everything FP64 ("plain FP64 MGPCG") */
     VFEM_FP32 = 1,  /* V-cycle and CG matvec in FP32; CG dots, scalars, x/r updates FP64 */
-    VFEM_BF16 = 2   /* as VFEM_FP32 with the V-cycle's level-0 matvec on BF16 MFMA */
+    VFEM_BF16 = 2,  /* as VFEM_FP32 with the V-cycle's level-0 matvec on BF16 MFMA */
+    VFEM_MIXED = 3  /* FP64 vectors, CG and cycle; every level-0 matvec of the cycle and of the CG
+                       operator in FP32 arithmetic on the matrix cores (BF16 splits of Ke and of the
+                       FP64 element differences x - x_centre); FP64 true residual every
+                       true_res_every iterations; flexible CG */
 };
 
 /* multigrid cycle */
@@ -115,10 +119,21 @@ int vfem_apply_boundary_conditions(double *f, int64_t n_dofs, const int64_t *loa
  * vfem_num_dofs doubles.  opts may be NULL (defaults); stats may be NULL. */
 int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options *opts, vfem_stats *stats);
 
-/* Multi-RHS block: nrhs load cases on one geometry, f and u are nrhs x ndof row-major
- * (no reference counterpart; each column equals a single-RHS solve). */
+/* Multi-RHS block (no reference counterpart; the reference solves one load case,
+ * include/FEMGPU.hpp:11-12): nrhs load cases on one geometry, f and u are
+ * nrhs x ndof row-major.  Up to 256 columns run as one block -- every kernel of
+ * an iteration covers all columns, each column keeps its own FP64 CG scalars and
+ * stops on its own tolerance; stats receives nrhs entries.  Each column's
+ * solution equals the single-RHS solve of that column to the solver tolerance. */
 int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrhs, const vfem_options *opts,
                        vfem_stats *stats);
+
+/* Resident-data split of vfem_solve_batched for one block of 1..256 columns:
+ * upload nrhs right-hand sides (nrhs x ndof row-major), solve them on the device,
+ * download the solutions. */
+int vfem_set_rhs_block(vfem_system *sys, const double *f, int32_t nrhs);
+int vfem_solve_resident_block(vfem_system *sys, int32_t nrhs, const vfem_options *opts, vfem_stats *stats);
+int vfem_get_solution_block(vfem_system *sys, double *u, int32_t nrhs);
 
 /* Resident-data split of vfem_solve for timing without PCIe transfers:
  * vfem_set_rhs uploads f, vfem_solve_resident solves on the device and keeps u
@@ -129,8 +144,31 @@ int vfem_get_solution(vfem_system *sys, double *u);
 
 /* Replaces the matrix-free product y = A x with fixed rows zeroed
  * (include/MatrixFreeSparse.hpp:191-215; GPU: shaders/MatxVec.comp +
- * shaders/FixedNodes.comp).  precision selects the kernel (vfem_precision). */
+ * shaders/FixedNodes.comp).  precision selects the kernel (vfem_precision):
+ * FP64 VALU, FP32 (6 BF16 products), BF16 (1 product), MIXED (6 products on
+ * FP64 input differences). */
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision);
+
+/* Multi-RHS form of vfem_matvec (no reference counterpart): y_k = A x_k for nrhs
+ * columns (x, y nrhs x ndof row-major).  FP32/BF16 use the multi-RHS matrix-core
+ * kernel (16 columns per MFMA N tile) with the tile shape last chosen by
+ * vfem_autotune_block or vfem_set_block_config. */
+int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs, int32_t precision);
+
+/* Tile shapes of the multi-RHS kernel: elements per layer tex x tey, node planes
+ * per block mz. */
+int32_t vfem_block_config_count(void);
+int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz);
+int vfem_set_block_config(vfem_system *sys, int32_t index);
+
+/* Autotuner of the multi-RHS kernel (no reference counterpart): times every tile
+ * shape for nrhs columns and `passes` BF16 products (1, 3 or 6) over `reps`
+ * launches on a deterministic pseudo-random block, rejects any shape whose product
+ * differs from shape 0's, selects the fastest for this system and returns it in
+ * *best; ms_out (vfem_block_config_count() entries, may be null) receives the mean
+ * time of each shape (-1 for a rejected shape). */
+int vfem_autotune_block(vfem_system *sys, int32_t nrhs, int32_t passes, int32_t reps, double *ms_out,
+                        int32_t *best);
 
 /* Replaces MultigridPreconditioner::solve (include/MultigridPreconditioner.hpp:158-193;
  * GPU: FEMGPU.cpp:500-534): z = M r with the cycle/precision of opts. */

@@ -58,6 +58,9 @@ def test_matvec_parity(systems, name):
     say("matvec rel err fp64 %.2e fp32(3-pass bf16 MFMA) %.2e bf16(1 pass) %.2e" % (e64, e32, e16))
     assert e64 < 1e-13
     assert e32 < 1e-4
+    emx = rel(A.matvec(x, V.MIXED), ref)
+    say("matvec rel err mixed (6-pass MFMA on FP64 differences) %.2e" % emx)
+    assert emx < 1e-6
 
 
 def test_matvec_fp32_on_smooth_field(systems, goldens):
@@ -73,6 +76,11 @@ def test_matvec_fp32_on_smooth_field(systems, goldens):
     e32 = rel(A.matvec(u, V.FP32), ref32)
     say("smooth field fp32 matvec rel err vs exact product of the fp32 input %.2e" % e32)
     assert e32 < 1e-3
+    # FP64 input differences: the error scales with the local differences of u, so
+    # the product stays accurate where FP32 storage of u alone costs ~1e-4
+    emx = rel(A.matvec(u, V.MIXED), ref)
+    say("smooth field mixed matvec rel err %.2e" % emx)
+    assert emx < 1e-5
 
 
 def test_matvec_golden(goldens, ke0):
@@ -112,7 +120,7 @@ def test_box_solve_matches_direct_solution(systems, goldens):
     A, ora, _, _ = systems["box"]
     f, u_direct = goldens["box_f"], goldens["box_u"]
     for cycle in (V.CYCLE_REFERENCE, V.CYCLE_DAMPED):
-        for prec in (V.FP64, V.FP32):
+        for prec in (V.FP64, V.FP32, V.MIXED):
             say("box solve", cycle, prec)
             u, st = A.solve(f, tol=1e-10, precision=prec, cycle=cycle)
             assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
@@ -136,7 +144,7 @@ def test_solve_parity(systems, name):
     f[3 * top[top >= 0] + 2] = -1.0 / np.count_nonzero(top >= 0)
     u_ref, _, _ = ora.pcg(f, tol=1e-13, precond=O.DAMPED)
     for cycle in (V.CYCLE_REFERENCE, V.CYCLE_DAMPED):
-        for prec in (V.FP64, V.FP32):
+        for prec in (V.FP64, V.FP32, V.MIXED):
             say("solve", name, cycle, prec)
             u, st = A.solve(f, tol=1e-10, precision=prec, cycle=cycle)
             assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
@@ -202,3 +210,59 @@ def test_no_graph_path_matches_graph(systems):
     u2, s2 = A.solve(f, tol=1e-9, use_graph=0)
     assert s1["iterations"] == s2["iterations"]
     assert np.array_equal(u1, u2)  # deterministic reductions: bitwise identical
+
+
+@pytest.mark.parametrize("prec", [V.FP64, V.FP32, V.MIXED])
+def test_batched_solve_matches_single_solves(systems, prec):
+    """Multi-RHS block: each column converges on its own tolerance and matches the
+    single-RHS solve of that column -- bitwise in FP64 (same kernels per column,
+    same reduction order), to the solver tolerance in FP32 (the block uses the
+    multi-RHS matrix-core matvec).  Columns stop independently."""
+    A, ora, vox, fixed = systems["ell24"]
+    F = problems.patch_load_cases(vox, 5, seed=11, patch=4)
+    F[:, np.repeat(ora.fixed(0), 3)] = 0
+    F[3] *= 1e-3          # a column with a different scale converges on its own relative tolerance
+    F[4] = 0              # a zero column stops at once (Eigen: x = 0)
+    say("batched", prec)
+    U, stats = A.solve_batched(F, tol=1e-9, precision=prec, cycle=V.CYCLE_DAMPED)
+    assert stats[4]["iterations"] == 0 and not U[4].any()
+    for k in range(4):
+        u, st = A.solve(F[k], tol=1e-9, precision=prec, cycle=V.CYCLE_DAMPED)
+        say("column", k, "iterations", stats[k]["iterations"], st["iterations"], "rel diff", rel(U[k], u))
+        assert stats[k]["converged"] == 1 and stats[k]["rel_residual"] < 1e-9
+        if prec == V.FP64:
+            assert stats[k]["iterations"] == st["iterations"]
+            assert np.array_equal(U[k], u)
+        else:
+            assert rel(U[k], u) < 1e-6
+
+
+@pytest.mark.parametrize("cfg", range(len(V.api.block_configs()) if V.api.os.path.exists(V.api.LIB_PATH) else 0))
+def test_block_matvec_parity(systems, cfg):
+    """Every tile shape of the multi-RHS matrix-core matvec against the oracle,
+    column by column (20 columns: a full 16-column MFMA tile and a partial one)."""
+    A, ora, _, _ = systems["ell33"]
+    X = np.random.default_rng(5).standard_normal((20, A.n))
+    A.set_block_config(cfg)
+    say("block matvec config", cfg, V.api.block_configs()[cfg])
+    Y32 = A.matvec_block(X, V.FP32)
+    Y16 = A.matvec_block(X, V.BF16)
+    e32 = max(rel(Y32[k], ora.matvec(X[k])) for k in range(20))
+    e16 = max(rel(Y16[k], ora.matvec(X[k])) for k in range(20))
+    say("block matvec rel err fp32 %.2e bf16 %.2e" % (e32, e16))
+    assert e32 < 1e-6 and e16 < 2e-2
+    if cfg == 0:
+        Y64 = A.matvec_block(X[:3], V.FP64)
+        assert max(rel(Y64[k], ora.matvec(X[k])) for k in range(3)) < 1e-13
+    A.set_block_config(0)
+
+
+def test_block_autotuner(systems):
+    """The autotuner times every tile shape, rejects none (all reproduce shape 0
+    bitwise) and keeps the fastest."""
+    A, _, _, _ = systems["ell33"]
+    best, table = A.autotune_block(nrhs=16, passes=6, reps=2)
+    say("autotune", best, table)
+    assert all(v[3] > 0 for v in table.values())
+    assert table[best][3] == min(v[3] for v in table.values())
+    A.set_block_config(0)

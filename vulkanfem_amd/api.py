@@ -22,7 +22,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvfem.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "vfem.h")
 
-FP64, FP32, BF16 = 0, 1, 2
+FP64, FP32, BF16, MIXED = 0, 1, 2, 3
 CYCLE_REFERENCE, CYCLE_DAMPED = 0, 1
 
 OK, ERR_ARGUMENT, ERR_EMPTY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_UNSUPPORTED = range(8)
@@ -90,9 +90,18 @@ def lib():
     L.vfem_solve_batched.argtypes = [vp, _f64p, _f64p, ctypes.c_int32, ctypes.POINTER(Options),
                                      ctypes.POINTER(Stats)]
     L.vfem_set_rhs.argtypes = [vp, _f64p]
+    L.vfem_set_rhs_block.argtypes = [vp, _f64p, ctypes.c_int32]
+    L.vfem_solve_resident_block.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(Options), ctypes.POINTER(Stats)]
+    L.vfem_get_solution_block.argtypes = [vp, _f64p, ctypes.c_int32]
     L.vfem_solve_resident.argtypes = [vp, ctypes.POINTER(Options), ctypes.POINTER(Stats)]
     L.vfem_get_solution.argtypes = [vp, _f64p]
     L.vfem_matvec.argtypes = [vp, _f64p, _f64p, ctypes.c_int32]
+    L.vfem_matvec_block.argtypes = [vp, _f64p, _f64p, ctypes.c_int32, ctypes.c_int32]
+    L.vfem_block_config_count.restype = ctypes.c_int32
+    L.vfem_block_config.argtypes = [ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
+    L.vfem_set_block_config.argtypes = [vp, ctypes.c_int32]
+    L.vfem_autotune_block.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f64p,
+                                      ctypes.POINTER(ctypes.c_int32)]
     L.vfem_precondition.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options)]
     L.vfem_dot.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_double)]
     L.vfem_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -107,6 +116,17 @@ def lib():
     L.vfem_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
+
+
+def block_configs():
+    """Tile shapes (tex, tey, mz) of the multi-RHS kernel, by index."""
+    L = lib()
+    out = []
+    for i in range(L.vfem_block_config_count()):
+        t, u, m = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        L.vfem_block_config(i, ctypes.byref(t), ctypes.byref(u), ctypes.byref(m))
+        out.append((t.value, u.value, m.value))
+    return out
 
 
 def _check(rc):
@@ -230,6 +250,46 @@ class MatrixFreeSparse:
         u = np.zeros(self.n)
         _check(self._lib.vfem_get_solution(self._h, u))
         return u
+
+    # resident split of the multi-RHS block (1..256 columns)
+    def set_rhs_block(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        self._nrhs = F.shape[0]
+        _check(self._lib.vfem_set_rhs_block(self._h, F.reshape(-1), self._nrhs))
+
+    def solve_resident_block(self, nrhs=None, **opts):
+        nrhs = self._nrhs if nrhs is None else nrhs
+        st = (Stats * nrhs)()
+        o = options(**opts)
+        rc = self._lib.vfem_solve_resident_block(self._h, nrhs, ctypes.byref(o), st)
+        if rc not in (OK, ERR_NOT_CONVERGED):
+            _check(rc)
+        return [s.as_dict() for s in st]
+
+    def get_solution_block(self, nrhs=None):
+        nrhs = self._nrhs if nrhs is None else nrhs
+        U = np.zeros((nrhs, self.n))
+        _check(self._lib.vfem_get_solution_block(self._h, U.reshape(-1), nrhs))
+        return U
+
+    def matvec_block(self, X, precision=FP64):
+        """Y[k] = A X[k] for every row of X (multi-RHS kernel for FP32/BF16)."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Y = np.zeros_like(X)
+        _check(self._lib.vfem_matvec_block(self._h, X.reshape(-1), Y.reshape(-1), X.shape[0], int(precision)))
+        return Y
+
+    def set_block_config(self, index):
+        _check(self._lib.vfem_set_block_config(self._h, int(index)))
+
+    def autotune_block(self, nrhs=16, passes=6, reps=5):
+        """Time every tile shape of the multi-RHS kernel, keep the fastest; returns
+        (best index, {index: (tex, tey, mz, ms)})."""
+        n = self._lib.vfem_block_config_count()
+        ms = np.zeros(n)
+        best = ctypes.c_int32()
+        _check(self._lib.vfem_autotune_block(self._h, int(nrhs), int(passes), int(reps), ms, ctypes.byref(best)))
+        return best.value, {i: block_configs()[i] + (float(ms[i]),) for i in range(n)}
 
     def time_kernel(self, which, precision=FP64, reps=10):
         out = ctypes.c_double()

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -56,6 +57,19 @@ __device__ __forceinline__ int remap_block()
 struct Range {
     int64_t begin, end;
 };
+
+// Batched launches: blockIdx.y selects the right-hand side (column).  Column c
+// of a level vector starts at c * (3 * level nodes); every column has its own
+// CG state and reduction slots.
+__device__ __forceinline__ int64_t col_offset(int64_t n3) { return (int64_t)blockIdx.y * n3; }
+__device__ __forceinline__ CGState *col_state(CGState *st) { return st + blockIdx.y; }
+__device__ __forceinline__ const CGState *col_state(const CGState *st) { return st + blockIdx.y; }
+__device__ __forceinline__ Reduction col_red(Reduction red)
+{
+    red.partials += (size_t)blockIdx.y * 2 * red.cap;
+    red.counter += blockIdx.y;
+    return red;
+}
 
 __device__ __forceinline__ Range block_range(int64_t W)
 {
@@ -320,6 +334,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
                                                          T *__restrict__ y, CGState *st, Reduction red, int fin,
                                                          int honor_done)
 {
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (honor_done && st->done) return;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)L.nz * L.ny * L.S;
@@ -473,11 +494,22 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                                                          const Tx *__restrict__ x, Ty *__restrict__ y, CGState *st,
                                                          Reduction red, int fin, int honor_done, MfmaTiles T)
 {
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (honor_done && st->done) return;
     // PASSES 1: hi*hi.  3: two-way split, hi*hi + hi*lo + lo*hi.  6: three-way split,
     // all products of combined order < 3 (about FP32 accuracy).
     constexpr int PARTS = PASSES == 1 ? 1 : (PASSES == 3 ? 2 : 3);
-    __shared__ float Xs[2][MN_Y][64][3];
+    // staged node values keep the input precision: with FP64 input the element
+    // differences x - x_centre are formed in FP64 before the BF16 split, so the
+    // product's error scales with the local differences, not with |x|
+    using XT = typename std::conditional<sizeof(Tx) == 8, double, float>::type;
+    __shared__ XT Xs[2][MN_Y][64][3];
     __shared__ float Ys[ME_Y * ME_X * YS_STRIDE];
     __shared__ MfmaMasks K;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -549,7 +581,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     // written to LDS (a select on them would make the compiler wait for the load);
     // validity goes into a bit mask.
     constexpr int PRE = 3;
-    auto fetch = [&](int gz, float (&dst)[MROWS_PER_WAVE][3], unsigned &valid) {
+    auto fetch = [&](int gz, XT (&dst)[MROWS_PER_WAVE][3], unsigned &valid) {
         valid = 0;
         const int pl = min(max(gz - (zb - 1), 0), MPL - 1);
         const bool plok = gz >= zb - 1 && gz <= zb + MZ;
@@ -563,37 +595,37 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             const bool v = plok && inx && ((w >> bit) & 1);
             const int id = K.rb[pl][r][s] + __popcll(w & ((1ull << bit) - 1));
             const int idc = v ? id : 0;
-            dst[i][0] = (float)x[3 * idc + 0];
-            dst[i][1] = (float)x[3 * idc + 1];
-            dst[i][2] = (float)x[3 * idc + 2];
+            dst[i][0] = (XT)x[3 * idc + 0];
+            dst[i][1] = (XT)x[3 * idc + 1];
+            dst[i][2] = (XT)x[3 * idc + 2];
             valid |= (v ? 1u : 0u) << i;
         }
     };
-    auto store = [&](int buf, const float (&src)[MROWS_PER_WAVE][3], unsigned valid) {
+    auto store = [&](int buf, const XT (&src)[MROWS_PER_WAVE][3], unsigned valid) {
 #pragma unroll
         for (int i = 0; i < MROWS_PER_WAVE; i++) {
             const int r = wave + kWaves * i;
             const bool v = (valid >> i) & 1;
             if (r < MN_Y) {
-                Xs[buf][r][lane][0] = v ? src[i][0] : 0.0f;
-                Xs[buf][r][lane][1] = v ? src[i][1] : 0.0f;
-                Xs[buf][r][lane][2] = v ? src[i][2] : 0.0f;
+                Xs[buf][r][lane][0] = v ? src[i][0] : XT(0);
+                Xs[buf][r][lane][1] = v ? src[i][1] : XT(0);
+                Xs[buf][r][lane][2] = v ? src[i][2] : XT(0);
             }
         }
     };
 
     float acc[MOWN][3];
-    float xc[MOWN][3];
+    XT xc[MOWN][3];
 #pragma unroll
     for (int k = 0; k < MOWN; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
     double dacc = 0;
 
     // prefetch ring of PRE planes in three fixed register sets (no register moves:
     // a copy of a register with a load in flight would wait for the load).
-    float s0[MROWS_PER_WAVE][3], s1[MROWS_PER_WAVE][3], s2[MROWS_PER_WAVE][3];
+    XT s0[MROWS_PER_WAVE][3], s1[MROWS_PER_WAVE][3], s2[MROWS_PER_WAVE][3];
     unsigned v0 = 0, v1 = 0, v2 = 0;
     {
-        float tmp[MROWS_PER_WAVE][3];
+        XT tmp[MROWS_PER_WAVE][3];
         unsigned tv;
         fetch(zb - 1, tmp, tv);
         store((zb - 1) & 1, tmp, tv);
@@ -606,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     __syncthreads();
 
     // one element layer; `slot` holds plane ez+2 and is refilled with plane ez+2+PRE
-    auto step = [&](int ez, float (&slot)[MROWS_PER_WAVE][3], unsigned &sv) {
+    auto step = [&](int ez, XT (&slot)[MROWS_PER_WAVE][3], unsigned &sv) {
         const int bl = ez & 1, bu = (ez + 1) & 1;
         const int ly = ez - (zb - 1);  // local element layer / staged plane of ez
         const bool live = ez < ze;
@@ -634,20 +666,20 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             const int ex = x0 - 1 + exl;
             const int ewl = (ex >> 6) - ew0, ebit = ex & 63;
             const int cx = min(exl, ME_X - 2);
-            const float *pb = &Xs[(g >> 1) ? bu : bl][g & 1][cx][0];  // node pair of row eyl = 0
-            const float *pc = &Xs[bl][0][cx][0];                      // centre (local node 0)
+            const XT *pb = &Xs[(g >> 1) ? bu : bl][g & 1][cx][0];  // node pair of row eyl = 0
+            const XT *pc = &Xs[bl][0][cx][0];                      // centre (local node 0)
             constexpr int RSTR = 64 * 3;                              // floats per staged row
             for (int eyl = 0; eyl < ME_Y; eyl++) {
                 const bool pres = exl < ME_X - 1 && ex >= 0 && ((K.ew[ly][eyl][ewl] >> ebit) & 1);
-                const float *q = pb + eyl * RSTR, *qc = pc + eyl * RSTR;
-                const float c0 = qc[0], c1 = qc[1], c2 = qc[2];
+                const XT *q = pb + eyl * RSTR, *qc = pc + eyl * RSTR;
+                const XT c0 = qc[0], c1 = qc[1], c2 = qc[2];
                 float vv[8];
-                vv[0] = q[0] - c0;
-                vv[1] = q[1] - c1;
-                vv[2] = q[2] - c2;
-                vv[3] = q[3] - c0;
-                vv[4] = q[4] - c1;
-                vv[5] = q[5] - c2;
+                vv[0] = (float)(q[0] - c0);
+                vv[1] = (float)(q[1] - c1);
+                vv[2] = (float)(q[2] - c2);
+                vv[3] = (float)(q[3] - c0);
+                vv[4] = (float)(q[4] - c1);
+                vv[5] = (float)(q[5] - c2);
                 vv[6] = 0.0f;
                 vv[7] = 0.0f;
                 bf16x8 b1, b2, b3;
@@ -738,6 +770,310 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-RHS level-0 matvec on the matrix cores: the MFMA N dimension is the
+// right-hand side.  One wavefront contracts one element for 16 columns per
+// v_mfma_f32_16x16x32_bf16 pair (M = 24 Ke rows padded to 32, K = 24 element
+// DOFs in node-pair order padded to 32, N = 16 columns), so empty voxels cost
+// nothing (the wave skips them) and every node value staged in LDS serves 16
+// columns.  Tile: TEX x TEY elements per layer, owned nodes (TEX-1) x (TEY-1),
+// swept over `mz` node planes; element results go to LDS and an atomic-free
+// node gather sums the 8 element blocks of each owned node.
+// ---------------------------------------------------------------------------
+constexpr int BCOLS = 16;  // columns per MFMA N tile
+
+template <int TEX, int TEY> struct BlockTile {
+    static constexpr int NX = TEX + 1, NY = TEY + 1, NN = NX * NY;  // staged nodes per plane
+    static constexpr int NE = TEX * TEY;                              // elements per layer
+    static constexpr int OX = TEX - 1, OY = TEY - 1, NO = OX * OY;    // owned nodes per plane
+    static constexpr int PLANE_RAW = NN * 3 * BCOLS;
+    // plane stride = 32 (mod 64) floats: the four B lane groups hit distinct banks
+    static constexpr int PLANE = PLANE_RAW + ((32 - PLANE_RAW % 64) + 64) % 64;
+    static constexpr int RS = 24;                                     // floats per (element, column) row
+    static constexpr int ES = BCOLS * RS + 16;                        // floats per element
+    static constexpr int ITEMS = (NO * BCOLS + kBlock - 1) / kBlock;  // gather items per thread
+    static constexpr int SITEMS = (NN * 3 * BCOLS + kBlock - 1) / kBlock;  // staging loads per thread
+    static_assert(NE <= 64, "one wavefront builds the element list");
+    static_assert(NN <= kBlock, "one thread per staged node computes its id");
+};
+
+struct BlockGrid {
+    int tx, ty, tz, mz;  // tiles in x, y, z; node planes per tile
+};
+
+template <int PASSES, int TEX, int TEY>
+__global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float *__restrict__ ke,
+                                                          const float *__restrict__ x, float *__restrict__ y,
+                                                          CGState *st, Reduction red, int fin, int honor_done,
+                                                          int ncols, BlockGrid G)
+{
+    using TL = BlockTile<TEX, TEY>;
+    constexpr int PARTS = PASSES == 1 ? 1 : (PASSES == 3 ? 2 : 3);
+    __shared__ __align__(16) float Xs[2 * TL::PLANE];
+    __shared__ __align__(16) float Ys[TL::NE * TL::ES];
+    __shared__ int nid[3][TL::NN];
+    __shared__ unsigned char nfx[3][TL::NN];
+    __shared__ unsigned char elist[TL::NE];
+    __shared__ int ecount;
+    __shared__ unsigned long long epres;
+    __shared__ double dsh[TL::NO * BCOLS];
+    __shared__ int am_last;
+
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0 = blockIdx.y * BCOLS;  // first column of this group
+    const int nc = min(BCOLS, ncols - c0);
+    const int64_t n3 = 3 * L.nn;
+    // all columns of the group done: nothing to do (finalizers skip done columns)
+    if (honor_done) {
+        bool all = true;
+        for (int j = 0; j < nc; j++) all = all && st[c0 + j].done;
+        if (all) return;
+    }
+    int b = remap_block();
+    const int tx = b % G.tx;
+    b /= G.tx;
+    const int ty = b % G.ty;
+    const int tz = b / G.ty;
+    const int ex0 = tx * TL::OX - 1, ey0 = ty * TL::OY - 1;  // first element (and staged node) of the tile
+    const int zb = tz * G.mz, ze = min(zb + G.mz, L.nz);
+
+    // A operand: Ke rows 16 t + (lane % 16), K in node-pair order (k = 8 g + 3 m + c is
+    // DOF c of local node 2 g + m; k % 8 = 6, 7 zero), split in three BF16 parts
+    bf16x8 a1[2], a2[2], a3[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+#pragma unroll
+        for (int jj = 0; jj < 8; jj++) {
+            const int row = 16 * t + (lane & 15), g = lane >> 4;
+            const int col = jj < 6 ? 3 * (2 * g + jj / 3) + jj % 3 : 24;
+            const float v = (row < 24 && col < 24) ? ke[row * 24 + col] : 0.0f;
+            __bf16 h, m, l;
+            split_bf16(v, h, m, l, 3);
+            a1[t][jj] = h;
+            a2[t][jj] = m;
+            a3[t][jj] = l;
+        }
+    }
+
+    // ids of the staged nodes of plane gz into nid[slot] (-1: unused or outside)
+    auto plane_ids = [&](int gz, int slot) {
+        const int n = threadIdx.x;
+        if (n < TL::NN) {
+            const int gx = ex0 + n % TL::NX, gy = ey0 + n / TL::NX;
+            bool ok = gx >= 0 && gx < L.nx && gy >= 0 && gy < L.ny && gz >= 0 && gz < L.nz;
+            int id = -1;
+            unsigned char fx = 0;
+            if (ok) {
+                const int64_t seg = ((int64_t)gz * L.ny + gy) * L.S + (gx >> 6);
+                const uint64_t w = L.nbits[seg];
+                const int bit = gx & 63;
+                if ((w >> bit) & 1) {
+                    id = L.nbase[seg] + __popcll(w & ((1ull << bit) - 1));
+                    fx = (L.fbits[seg] >> bit) & 1;
+                }
+            }
+            nid[slot][n] = id;
+            nfx[slot][n] = fx;
+        }
+    };
+    // staged values of plane (ids in nid[slot]) -> registers; column clamped to the group
+    auto plane_load = [&](int slot, float (&v)[TL::SITEMS]) {
+#pragma unroll
+        for (int k = 0; k < TL::SITEMS; k++) {
+            const int q = threadIdx.x + kBlock * k;
+            const int qq = min(q, TL::NN * 3 * BCOLS - 1);
+            const int j = qq / (TL::NN * 3), rem = qq % (TL::NN * 3);
+            const int id = nid[slot][rem / 3];
+            const int jc = min(j, nc - 1);
+            const int64_t idx = id >= 0 ? (int64_t)(c0 + jc) * n3 + 3 * (int64_t)id + rem % 3 : 0;
+            const float val = x[idx];
+            v[k] = id >= 0 ? val : 0.0f;
+        }
+    };
+    auto plane_store = [&](int buf, const float (&v)[TL::SITEMS]) {
+#pragma unroll
+        for (int k = 0; k < TL::SITEMS; k++) {
+            const int q = threadIdx.x + kBlock * k;
+            if (q < TL::NN * 3 * BCOLS) {
+                const int j = q / (TL::NN * 3), rem = q % (TL::NN * 3);
+                Xs[buf * TL::PLANE + rem * BCOLS + j] = v[k];  // [node][c][column]
+            }
+        }
+    };
+
+    float acc[TL::ITEMS][3];
+    double dacc[TL::ITEMS];
+#pragma unroll
+    for (int k = 0; k < TL::ITEMS; k++) {
+        acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+        dacc[k] = 0.0;
+    }
+    float sv[TL::SITEMS];
+    // prologue: planes zb-1 and zb staged, ids of zb+1 ready
+    plane_ids(zb - 1, (zb + 2) % 3);
+    plane_ids(zb, (zb + 3) % 3);
+    __syncthreads();
+    plane_load((zb + 2) % 3, sv);
+    plane_store((zb - 1) & 1, sv);
+    plane_load((zb + 3) % 3, sv);
+    plane_store(zb & 1, sv);
+    __syncthreads();
+
+    for (int ez = zb - 1; ez < ze; ez++) {
+        const int bl = ez & 1, bu = (ez + 1) & 1;
+        const int sl = (ez + 3) % 3;  // id slot of plane ez (ez >= -1)
+        // ids of plane ez+2 (into the slot of plane ez-1, no longer read) and the element list
+        plane_ids(ez + 2, (ez + 5) % 3);
+        if (wave == 0) {
+            const int ex = ex0 + lane % TEX, ey = ey0 + lane / TEX;
+            const bool p = lane < TL::NE && element_present(L, ex, ey, ez);
+            const unsigned long long m = __ballot(p);
+            if (p) elist[__popcll(m & ((1ull << lane) - 1))] = (unsigned char)lane;
+            if (lane == 0) {
+                ecount = __popcll(m);
+                epres = m;
+            }
+        }
+        __syncthreads();
+        // element contractions: wave w takes elements w, w + 4, ...
+        {
+            const int j = lane & 15, g = lane >> 4;
+            const float *pb = &Xs[((g >> 1) ? bu : bl) * TL::PLANE];
+            const float *pc = &Xs[bl * TL::PLANE];
+            const int cnt = ecount;
+            for (int i = wave; i < cnt; i += kWaves) {
+                const int e = elist[i];
+                const int exl = e % TEX, eyl = e / TEX;
+                const int n0 = (eyl + (g & 1)) * TL::NX + exl;  // node 2g (m = 0); node 2g+1 is n0 + 1
+                const int nc0 = eyl * TL::NX + exl;
+                float vv[8];
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float cen = pc[(nc0 * 3 + c) * BCOLS + j];
+                    vv[c] = pb[(n0 * 3 + c) * BCOLS + j] - cen;
+                    vv[3 + c] = pb[((n0 + 1) * 3 + c) * BCOLS + j] - cen;
+                }
+                vv[6] = vv[7] = 0.0f;
+                bf16x8 b1, b2, b3;
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    __bf16 h, m, l;
+                    split_bf16(vv[t], h, m, l, PARTS);
+                    b1[t] = h;
+                    if (PARTS > 1) b2[t] = m;
+                    if (PARTS > 2) b3[t] = l;
+                }
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (PASSES == 6) {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b3, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t], b2, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[t], b1, d, 0, 0, 0);
+                    }
+                    if (PASSES >= 3) {
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b2, d, 0, 0, 0);
+                        d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[t], b1, d, 0, 0, 0);
+                    }
+                    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b1, d, 0, 0, 0);
+                    // lane (j, g) holds Ke rows 16 t + 4 g + q of column j
+                    if (16 * t + 4 * g < 24) {
+                        float4 o = {d[0], d[1], d[2], d[3]};
+                        *(float4 *)&Ys[e * TL::ES + j * TL::RS + 16 * t + 4 * g] = o;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // node gather for plane ez, prefetch of plane ez+2
+        plane_load((ez + 5) % 3, sv);
+        const unsigned long long em = epres;
+        const bool live = ez < ze;
+#pragma unroll
+        for (int k = 0; k < TL::ITEMS; k++) {
+            const int q = threadIdx.x + kBlock * k;
+            const bool own = q < TL::NO * BCOLS;
+            const int qq = own ? q : 0;
+            const int j = qq % BCOLS, o = qq / BCOLS;
+            const int a = o % TL::OX, bb = o / TL::OX;  // owned node (a+1, bb+1) of the staged grid
+            float lo[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int dx = s & 1, dy = s >> 1;  // element (a + dx, bb + dy)
+                const int e = (bb + dy) * TEX + (a + dx);
+                if ((em >> e) & 1) {
+                    const int Ll = (1 - dx) + 2 * (1 - dy);
+                    const float *Yp = &Ys[e * TL::ES + j * TL::RS];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        lo[c] += Yp[3 * Ll + c];
+                        up[c] += Yp[3 * (Ll + 4) + c];
+                    }
+                }
+            }
+            const int n = (bb + 1) * TL::NX + (a + 1);
+            const int id = nid[sl][n];
+            if (own && live && ez >= zb && id >= 0 && j < nc) {
+                const bool fx = nfx[sl][n];
+                float *yp = y + (int64_t)(c0 + j) * n3 + 3 * (int64_t)id;
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float out = fx ? 0.0f : acc[k][c] + lo[c];
+                    yp[c] = out;
+                    dacc[k] += (double)Xs[bl * TL::PLANE + (n * 3 + c) * BCOLS + j] * (double)out;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[k][c] = up[c];
+        }
+        __syncthreads();
+        plane_store(bl, sv);  // plane ez+2 replaces plane ez
+        __syncthreads();
+    }
+    if (fin < 0) return;
+
+    // per-column x.Ax: fixed-order sums over the block's items, then over blocks
+#pragma unroll
+    for (int k = 0; k < TL::ITEMS; k++) {
+        const int q = threadIdx.x + kBlock * k;
+        if (q < TL::NO * BCOLS) dsh[q] = dacc[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < BCOLS && (int)threadIdx.x < nc) {
+        const int j = threadIdx.x;
+        double s = 0;
+        for (int o = 0; o < TL::NO; o++) s += dsh[o * BCOLS + j];
+        __hip_atomic_store(&red.partials[(size_t)(c0 + j) * 2 * red.cap + blockIdx.x], s, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned t = atomicAdd(red.counter + c0, 1u);
+        am_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    for (int j = 0; j < nc; j++) {
+        CGState *sj = st + c0 + j;
+        double s = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+            s += __hip_atomic_load(&red.partials[(size_t)(c0 + j) * 2 * red.cap + i], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        s = wave_sum(s);
+        __syncthreads();
+        if (lane == 0) dsh[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = 0;
+            for (int w = 0; w < kWaves; w++) tot += dsh[w];
+            if (!sj->done) finalize(sj, fin, tot, 0.0);
+        }
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(red.counter + c0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Coarse-level operator of the damped cycle: element matrices from a table
 // indexed by the element's child pattern (tab[p*576 ...], pattern 0 = absent).
 template <typename T>
@@ -745,6 +1081,12 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
                                                            const T *__restrict__ x, T *__restrict__ y,
                                                            const CGState *st)
 {
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+    }
     if (st->done) return;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)L.nz * L.ny * L.S;
@@ -803,6 +1145,13 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
                                                       const Tt *__restrict__ tf, Tc *__restrict__ rc, int maskfixed,
                                                       const CGState *st)
 {
+    {
+        const int64_t cf = col_offset(3 * F.nn);
+        rf += cf;
+        if (tf) tf += cf;
+        rc += col_offset(3 * C.nn);
+        st = col_state(st);
+    }
     if (st->done) return;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)C.nz * C.ny * C.S;
@@ -906,6 +1255,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
                                                      Tz *__restrict__ zf, int mode, int maskfixed, CGState *st,
                                                      Reduction red, int fin, const Tz *__restrict__ zold)
 {
+    {
+        const int64_t cf = col_offset(3 * F.nn);
+        zc += col_offset(3 * C.nn);
+        rf += cf;
+        zf += cf;
+        if (zold) zold += cf;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (st->done) return;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)F.nz * F.ny * F.S;
@@ -998,6 +1356,15 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
                                                     int mode, CGState *st, Reduction red, int fin,
                                                     const Tz *__restrict__ zold)
 {
+    {
+        const int64_t co = col_offset(n3);
+        r += co;
+        if (t) t += co;
+        z += co;
+        if (zold) zold += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (st->done) return;
     const Range R = flat_range(n3);
     const Tz om = Tz(omega);
@@ -1044,8 +1411,11 @@ template <typename Tk, typename Tv>
 __global__ __launch_bounds__(kBlock) void k_coarse_solve(const Tk *__restrict__ kinv, int n,
                                                           const int32_t *__restrict__ freedof,
                                                           const Tv *__restrict__ rc, Tv *__restrict__ zc,
-                                                          const CGState *st)
+                                                          const CGState *st, int64_t n3c)
 {
+    rc += col_offset(n3c);
+    zc += col_offset(n3c);
+    st = col_state(st);
     if (st->done) return;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rowi = blockIdx.x * kWaves + wave;
@@ -1067,6 +1437,15 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
                                                        const Tp *__restrict__ p, const Ta *__restrict__ ap,
                                                        CGState *st, Reduction red)
 {
+    {
+        const int64_t co = col_offset(n);
+        x += co;
+        r += co;
+        p += co;
+        ap += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (st->done) return;
     const double alpha = st->alpha;
     const Range R = flat_range(n);
@@ -1102,6 +1481,13 @@ template <typename Tz, typename Tp>
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
                                                         Tz *__restrict__ zold, const CGState *st)
 {
+    {
+        const int64_t co = col_offset(n);
+        z += co;
+        p += co;
+        if (zold) zold += co;
+        st = col_state(st);
+    }
     if (st->done) return;
     const double beta = st->beta;
     const Range R = flat_range(n);
@@ -1129,6 +1515,15 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__
                                                      const double *__restrict__ t, double *__restrict__ r,
                                                      double *__restrict__ x, CGState *st, Reduction red, int fin)
 {
+    {
+        const int64_t co = col_offset(n);
+        f += co;
+        if (t) t += co;
+        r += co;
+        if (x) x += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     const Range R = block_range(n);
     double acc = 0;
     for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
@@ -1145,6 +1540,13 @@ template <typename Ta, typename Tb>
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict__ a, const Tb *__restrict__ b,
                                                 CGState *st, Reduction red, int fin, int honor_done)
 {
+    {
+        const int64_t co = col_offset(n);
+        a += co;
+        b += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
     if (honor_done && st->done) return;
     const Range R = flat_range(n);
     double acc = 0;
@@ -1167,13 +1569,28 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict_
 template <typename Ts, typename Td>
 __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts *__restrict__ s, Td *__restrict__ d)
 {
+    s += col_offset(n);
+    d += col_offset(n);
     const Range R = block_range(n);
     for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) d[i] = Td(s[i]);
 }
 
-__global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop)
+// deterministic pseudo-random fill in [-0.5, 0.5) (autotuner input)
+__global__ __launch_bounds__(kBlock) void k_fill_hash(int64_t n, float *__restrict__ v, uint32_t seed)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        uint32_t h = (uint32_t)i * 2654435761u ^ seed;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        v[i] = (float)(h >> 8) * (1.0f / 16777216.0f) - 0.5f;
+    }
+}
+
+__global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int ncols)
+{
+    if (blockIdx.x == 0 && (int)threadIdx.x < ncols) {
+        st += threadIdx.x;
         st->no_stop = no_stop;
         st->iter = 0;
         st->done = 0;
@@ -1224,6 +1641,25 @@ int grid_for_flat(int64_t n)
     return (int)((g + 7) / 8 * 8);
 }
 
+// Tile shapes of the multi-RHS matvec the autotuner chooses from: elements per
+// layer (TEX x TEY) and node planes per block (MZ).
+struct BlockCfg {
+    int tex, tey, mz;
+};
+const BlockCfg kBlockCfgs[] = {{16, 4, 16}, {16, 4, 8}, {16, 4, 32}, {8, 8, 16}, {8, 8, 8},  {8, 8, 32},
+                               {8, 4, 16},  {8, 4, 8},  {8, 4, 32},  {16, 2, 16}, {16, 2, 32}};
+constexpr int kNumBlockCfgs = sizeof(kBlockCfgs) / sizeof(kBlockCfgs[0]);
+
+BlockGrid block_grid(const DevLevel &L, const BlockCfg &c)
+{
+    BlockGrid G;
+    G.tx = (L.nx + c.tex - 2) / (c.tex - 1);
+    G.ty = (L.ny + c.tey - 2) / (c.tey - 1);
+    G.tz = (L.nz + c.mz - 1) / c.mz;
+    G.mz = c.mz;
+    return G;
+}
+
 }  // namespace
 
 struct vfem_system {
@@ -1246,6 +1682,7 @@ struct vfem_system {
     // level-0 vectors
     double *x = nullptr, *r = nullptr, *f = nullptr, *t64 = nullptr, *p64 = nullptr, *ap64 = nullptr, *z64 = nullptr;
     float *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr, *zold32 = nullptr;
+    double *zold64 = nullptr;
     // coarse-level vectors (index = level), t = operator scratch of the damped cycle
     double *rl64[kMaxLevels] = {}, *zl64[kMaxLevels] = {}, *tl64[kMaxLevels] = {};
     float *rl32[kMaxLevels] = {}, *zl32[kMaxLevels] = {}, *tl32[kMaxLevels] = {};
@@ -1257,12 +1694,18 @@ struct vfem_system {
     double setup_ms = 0;
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
     int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
+    int block_cfg = 0;          // kBlockCfgs entry of the multi-RHS level-0 matvec (set by the autotuner)
+    bool l0_mfma = false;       // FP64 level-0 matvecs on the matrix cores (set while enqueueing a VFEM_MIXED plan)
+    bool use_block = true;      // batched FP32 level-0 matvec: one launch per 16 columns on the matrix cores
+    int nb = 0;                 // right-hand-side columns the vectors are allocated for
+    int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
+    std::vector<void *> vec_allocs;
 
-    template <typename T> int alloc(T **p, size_t n, bool zero = true)
+    template <typename T> int alloc(T **p, size_t n, bool zero = true, std::vector<void *> *list = nullptr)
     {
         void *q = nullptr;
         HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
-        allocs.push_back(q);
+        (list ? list : &allocs)->push_back(q);
         if (zero) HIP_TRY(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
         *p = (T *)q;
         return 0;
@@ -1278,6 +1721,7 @@ struct vfem_system {
     {
         for (auto &g : graphs) (void)hipGraphExecDestroy(g.second);
         for (void *q : allocs) (void)hipFree(q);
+        for (void *q : vec_allocs) (void)hipFree(q);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1291,6 +1735,64 @@ template <typename T> std::vector<T> convert(const std::vector<double> &v)
     std::vector<T> o(v.size());
     for (size_t i = 0; i < v.size(); i++) o[i] = (T)v[i];
     return o;
+}
+
+// Solver vectors for `nb` right-hand sides, column-major (column c of a level
+// vector at c * 3 * nodes).  Reallocating drops the captured graphs, which hold
+// the old addresses.
+int alloc_vectors(vfem_system *S, int nb)
+{
+    if (nb < 1 || nb > kMaxCols) {
+        g_last_error = "batch width out of range";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (nb == S->nb) return 0;
+    if (S->stream) HIP_TRY(hipStreamSynchronize(S->stream));
+    for (auto &g : S->graphs) (void)hipGraphExecDestroy(g.second);
+    S->graphs.clear();
+    for (void *q : S->vec_allocs) (void)hipFree(q);
+    S->vec_allocs.clear();
+    S->nb = 0;
+    Hierarchy &H = S->H;
+    auto *V = &S->vec_allocs;
+    const int64_t n3 = S->n3 * nb;
+    if (int rc = S->alloc(&S->x, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->r, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->f, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->t64, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->p64, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->ap64, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->z64, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->p32, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->ap32, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->z32, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->zold32, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->zold64, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->t0_32, n3, true, V)) return rc;
+    for (int l = 1; l < H.nlev; l++) {
+        int64_t m = 3 * H.L[l].nn * nb;
+        if (int rc = S->alloc(&S->rl64[l], m, true, V)) return rc;
+        if (int rc = S->alloc(&S->zl64[l], m, true, V)) return rc;
+        if (int rc = S->alloc(&S->tl64[l], m, true, V)) return rc;
+        if (int rc = S->alloc(&S->rl32[l], m, true, V)) return rc;
+        if (int rc = S->alloc(&S->zl32[l], m, true, V)) return rc;
+        if (int rc = S->alloc(&S->tl32[l], m, true, V)) return rc;
+    }
+    if (int rc = S->alloc(&S->st, nb, true, V)) return rc;
+    {
+        const MfmaTiles t = mfma_tiles(S->dl[0]);
+        int64_t cap = std::max<int64_t>(kMaxPartials, (int64_t)t.tx * t.ty * t.tz);
+        if (nb > 1)
+            for (const BlockCfg &c : kBlockCfgs) {
+                const BlockGrid G = block_grid(S->dl[0], c);
+                cap = std::max<int64_t>(cap, (int64_t)G.tx * G.ty * G.tz);
+            }
+        S->red.cap = (int)cap;
+    }
+    if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap * nb, true, V)) return rc;
+    if (int rc = S->alloc(&S->red.counter, nb, true, V)) return rc;
+    S->nb = nb;
+    return 0;
 }
 
 int upload_system(vfem_system *S)
@@ -1344,35 +1846,7 @@ int upload_system(vfem_system *S)
     if (int rc = S->upload(&S->kinv_gal32, convert<float>(H.kcinv_gal))) return rc;
     if (int rc = S->upload(&S->freedof, H.freedof)) return rc;
     S->n3 = 3 * H.L[0].nn;
-    int64_t n3 = S->n3;
-    if (int rc = S->alloc(&S->x, n3)) return rc;
-    if (int rc = S->alloc(&S->r, n3)) return rc;
-    if (int rc = S->alloc(&S->f, n3)) return rc;
-    if (int rc = S->alloc(&S->t64, n3)) return rc;
-    if (int rc = S->alloc(&S->p64, n3)) return rc;
-    if (int rc = S->alloc(&S->ap64, n3)) return rc;
-    if (int rc = S->alloc(&S->z64, n3)) return rc;
-    if (int rc = S->alloc(&S->p32, n3)) return rc;
-    if (int rc = S->alloc(&S->ap32, n3)) return rc;
-    if (int rc = S->alloc(&S->z32, n3)) return rc;
-    if (int rc = S->alloc(&S->zold32, n3)) return rc;
-    if (int rc = S->alloc(&S->t0_32, n3)) return rc;
-    for (int l = 1; l < H.nlev; l++) {
-        int64_t m = 3 * H.L[l].nn;
-        if (int rc = S->alloc(&S->rl64[l], m)) return rc;
-        if (int rc = S->alloc(&S->zl64[l], m)) return rc;
-        if (int rc = S->alloc(&S->tl64[l], m)) return rc;
-        if (int rc = S->alloc(&S->rl32[l], m)) return rc;
-        if (int rc = S->alloc(&S->zl32[l], m)) return rc;
-        if (int rc = S->alloc(&S->tl32[l], m)) return rc;
-    }
-    if (int rc = S->alloc(&S->st, 1)) return rc;
-    {
-        const MfmaTiles t = mfma_tiles(S->dl[0]);
-        S->red.cap = std::max<int>(kMaxPartials, t.tx * t.ty * t.tz);
-    }
-    if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap)) return rc;
-    if (int rc = S->alloc(&S->red.counter, 1)) return rc;
+    return alloc_vectors(S, 1);
     return 0;
 }
 
@@ -1405,27 +1879,76 @@ template <> const float *kinv<float>(vfem_system *S, bool gal) { return gal ? S-
 // level-0 matvec.  FP64: node-centric VALU kernel.  FP32: matrix cores with
 // `passes` BF16 products -- 6 (three-way split, about FP32 accuracy) for the CG
 // operator, 3 inside the FP32 cycle, 1 inside the BF16 cycle.
+template <int P, int TEX, int TEY>
+void launch_block_shape(vfem_system *S, const float *x, float *y, int fin, int honor_done, const BlockCfg &c)
+{
+    const BlockGrid G = block_grid(S->dl[0], c);
+    const int groups = (S->ncols + BCOLS - 1) / BCOLS;
+    hipLaunchKernelGGL((k_matvec_block<P, TEX, TEY>), dim3(G.tx * G.ty * G.tz, groups), dim3(kBlock), 0, S->stream,
+                       S->dl[0], S->ke32, x, y, S->st, S->red, fin, honor_done, S->ncols, G);
+}
+
+template <int P>
+void launch_block_passes(vfem_system *S, const float *x, float *y, int fin, int honor_done, const BlockCfg &c)
+{
+    if (c.tex == 16 && c.tey == 4) launch_block_shape<P, 16, 4>(S, x, y, fin, honor_done, c);
+    else if (c.tex == 8 && c.tey == 8) launch_block_shape<P, 8, 8>(S, x, y, fin, honor_done, c);
+    else if (c.tex == 8 && c.tey == 4) launch_block_shape<P, 8, 4>(S, x, y, fin, honor_done, c);
+    else launch_block_shape<P, 16, 2>(S, x, y, fin, honor_done, c);
+}
+
+// multi-RHS level-0 matvec (S->ncols columns, 16 per MFMA N tile)
+void launch_matvec_block(vfem_system *S, const float *x, float *y, int fin, int honor_done, int passes, int cfg)
+{
+    const BlockCfg &c = kBlockCfgs[cfg];
+    if (passes == 1) launch_block_passes<1>(S, x, y, fin, honor_done, c);
+    else if (passes == 3) launch_block_passes<3>(S, x, y, fin, honor_done, c);
+    else launch_block_passes<6>(S, x, y, fin, honor_done, c);
+}
+
+// level-0 matvec.  FP64 vectors: the VALU kernel in FP64 arithmetic, or (VFEM_MIXED
+// plans, unless `exact`) the matrix-core kernel on FP64 centre differences.
 template <typename T>
-void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done, int passes = 6)
+void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done, int passes = 6, bool exact = false)
 {
     const DevLevel &L = S->dl[0];
+    if constexpr (sizeof(T) == 8) {
+        if (S->l0_mfma && !exact) {
+            const MfmaTiles t = mfma_tiles(L);
+            const int grid = t.tx * t.ty * t.tz;
+            if (passes == 1)
+                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+            else if (passes == 3)
+                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+            else
+                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+            return;
+        }
+    }
     if constexpr (sizeof(T) == 4) {
+        if (S->use_mfma && S->use_block && S->ncols > 1) {
+            launch_matvec_block(S, (const float *)x, (float *)y, fin, honor_done, passes, S->block_cfg);
+            return;
+        }
         if (S->use_mfma) {
             const MfmaTiles t = mfma_tiles(L);
             const int grid = t.tx * t.ty * t.tz;
             if (passes == 1)
-                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
                                    y, S->st, S->red, fin, honor_done, t);
             else if (passes == 3)
-                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
                                    y, S->st, S->red, fin, honor_done, t);
             else
-                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
                                    y, S->st, S->red, fin, honor_done, t);
             return;
         }
     }
-    hipLaunchKernelGGL(k_matvec_fine<T>, dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, kefine<T>(S), x, y,
+    hipLaunchKernelGGL(k_matvec_fine<T>, dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, kefine<T>(S), x, y,
                        S->st, S->red, fin, honor_done);
 }
 
@@ -1436,7 +1959,7 @@ template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x
         return;
     }
     const DevLevel &L = S->dl[l];
-    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
+    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
                        S->st);
 }
 
@@ -1444,8 +1967,8 @@ template <typename T> void launch_coarse_solve(vfem_system *S, const T *rc, T *z
 {
     int n = (int)S->H.nfree;
     if (n == 0) return;
-    hipLaunchKernelGGL((k_coarse_solve<T, T>), dim3((n + kWaves - 1) / kWaves), dim3(kBlock), 0, S->stream,
-                       kinv<T>(S, gal), n, S->freedof, rc, zc, S->st);
+    hipLaunchKernelGGL((k_coarse_solve<T, T>), dim3((n + kWaves - 1) / kWaves, S->ncols), dim3(kBlock), 0, S->stream,
+                       kinv<T>(S, gal), n, S->freedof, rc, zc, S->st, (int64_t)3 * S->dl[S->H.nlev - 1].nn);
 }
 
 // Reference additive V-cycle (MultigridPreconditioner.hpp:158-193) on the
@@ -1458,21 +1981,21 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
     for (int l = 0; l < nl - 1; l++) {
         const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
         if (l == 0)
-            hipLaunchKernelGGL((k_restrict<double, double, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, F,
+            hipLaunchKernelGGL((k_restrict<double, double, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, F,
                                C, r, (const double *)nullptr, level_r<T>(S, 1), 0, S->st);
         else
-            hipLaunchKernelGGL((k_restrict<T, T, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, F, C,
+            hipLaunchKernelGGL((k_restrict<T, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                                (const T *)level_r<T>(S, l), (const T *)nullptr, level_r<T>(S, l + 1), 0, S->st);
     }
     launch_coarse_solve<T>(S, level_r<T>(S, nl - 1), level_z<T>(S, nl - 1), false);
     for (int l = nl - 2; l >= 1; l--) {
         const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
-        hipLaunchKernelGGL((k_prolong<T, T, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
+        hipLaunchKernelGGL((k_prolong<T, T, T, T>), dim3(items_grid(F), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                            (const T *)level_z<T>(S, l + 1), (const T *)level_r<T>(S, l), dinv<T>(S, l),
                            level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1, (const T *)nullptr);
     }
     const DevLevel &F = S->dl[0], &C = S->dl[1];
-    hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F)), dim3(kBlock), 0, S->stream, F, C,
+    hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                        (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin,
                        zold);
 }
@@ -1490,19 +2013,19 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
     const int64_t n3l = 3 * L.nn;
-    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l)), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
                        (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
-    hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C)), dim3(kBlock), 0, S->stream, L, C, rl,
+    hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
                        (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
     enqueue_vcycle_damped_level<T, T>(S, l + 1, level_r<T>(S, l + 1), level_z<T>(S, l + 1), level_t<T>(S, l + 1),
                                       omega, -1, (const T *)nullptr);
-    hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L)), dim3(kBlock), 0, S->stream, L, C,
+    hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
                        (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
                        -1, (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
-    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l)), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
+    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
 }
 
@@ -1513,25 +2036,45 @@ struct Plan {
     double omega;
     int op_passes;     // BF16 MFMA products of the reduced-precision CG operator (3 or 6)
     int cycle_passes;  // BF16 MFMA products of the cycle's level-0 matvecs (1, 3 or 6)
+    bool mfma64;       // VFEM_MIXED: FP64 vectors, level-0 matvecs on the matrix cores
+};
+
+// While alive: level-0 FP64 matvecs of the plan go to the matrix cores (VFEM_MIXED).
+struct L0Mode {
+    vfem_system *S;
+    bool old;
+    L0Mode(vfem_system *s, const Plan &P) : S(s), old(s->l0_mfma) { S->l0_mfma = P.mfma64; }
+    ~L0Mode() { S->l0_mfma = old; }
 };
 
 // z = M r on the level-0 FP64 residual r, reducing r.z with `fin`.  The
 // reduced-precision cycles run flexible CG: FIN_RZ becomes FIN_RZ_FLEX, which
 // also reduces r.z_old.
-bool flexible(const vfem_system *S, const Plan &P) { return P.precision != VFEM_FP64 && S->H.nlev > 1; }
+bool flexible(const vfem_system *S, const Plan &P)
+{
+    return (P.precision != VFEM_FP64 || P.mfma64) && S->H.nlev > 1;
+}
 
 void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 {
     if (S->H.nlev == 1) {
         // single level: the cycle is the exact coarse solve, z = Kc^-1 r (FP64), then r.z
         launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
-        hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(S->n3)), dim3(kBlock), 0, S->stream, S->n3, r,
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(S->n3), S->ncols), dim3(kBlock), 0, S->stream, S->n3, r,
                            S->z64, S->st, S->red, fin, 1);
         return;
     }
     if (P.precision == VFEM_FP64) {
-        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<double>(S, r, S->z64, fin, nullptr);
-        else enqueue_vcycle_damped_level<double, double>(S, 0, r, S->z64, S->t64, P.omega, fin, nullptr);
+        const double *zold = nullptr;
+        if (fin == FIN_RZ && flexible(S, P)) {
+            fin = FIN_RZ_FLEX;
+            zold = S->zold64;
+        }
+        L0Mode mode(S, P);
+        S->cycle_passes = P.cycle_passes;
+        if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<double>(S, r, S->z64, fin, zold);
+        else enqueue_vcycle_damped_level<double, double>(S, 0, r, S->z64, S->t64, P.omega, fin, zold);
+        S->cycle_passes = 6;
     } else {
         const float *zold = nullptr;
         if (fin == FIN_RZ && flexible(S, P)) {
@@ -1548,8 +2091,8 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 // FP64 true residual r = f - A x and |r|^2 (FIN_RR_TRUE)
 void enqueue_true_residual(vfem_system *S)
 {
-    launch_matvec_fine<double>(S, S->x, S->t64, -1, 0);
-    hipLaunchKernelGGL(k_residual, dim3(grid_for_flat(S->n3)), dim3(kBlock), 0, S->stream, S->n3, S->f, S->t64,
+    launch_matvec_fine<double>(S, S->x, S->t64, -1, 0, 6, true);
+    hipLaunchKernelGGL(k_residual, dim3(grid_for_flat(S->n3), S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->f, S->t64,
                        S->r, (double *)nullptr, S->st, S->red, (int)FIN_RR_TRUE);
 }
 
@@ -1559,13 +2102,13 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
     if (P.precision == VFEM_FP64)
-        hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           (double *)nullptr, S->st);
+        hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
+                           flexible(S, P) ? S->zold64 : (double *)nullptr, S->st);
     else if (S->H.nlev == 1)
-        hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
+        hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
                            (double *)nullptr, S->st);
     else
-        hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
+        hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
                            S->zold32, S->st);
 }
 
@@ -1577,12 +2120,13 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
     if (P.precision == VFEM_FP64) {
-        launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1);
-        hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
+        L0Mode mode(S, P);
+        launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes);
+        hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p64, S->ap64, S->st, S->red);
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
-        hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
+        hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p32, S->ap32, S->st, S->red);
     }
     if (with_true) enqueue_true_residual(S);
@@ -1592,11 +2136,12 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
 
 int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
 {
-    if (!(o.tol >= 0) || o.max_iter < 0 || o.precision < 0 || o.precision > 2 || o.cycle < 0 || o.cycle > 1) {
+    if (!(o.tol >= 0) || o.max_iter < 0 || o.precision < 0 || o.precision > 3 || o.cycle < 0 || o.cycle > 1) {
         g_last_error = "invalid solver options";
         return VFEM_ERR_ARGUMENT;
     }
-    P.precision = o.precision == VFEM_FP64 ? VFEM_FP64 : VFEM_FP32;
+    P.precision = (o.precision == VFEM_FP64 || o.precision == VFEM_MIXED) ? VFEM_FP64 : VFEM_FP32;
+    P.mfma64 = o.precision == VFEM_MIXED;
     P.bf16 = o.precision == VFEM_BF16;
     P.cycle = o.cycle;
     P.omega = o.omega;
@@ -1623,6 +2168,8 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     key += 1024 * (P.op_passes + 8 * P.cycle_passes);
     // omega is baked into the graph's kernel arguments
     key += std::llround(P.omega * 1000) * 1024 * 64;
+    key += (int64_t)S->ncols << 40;
+    key += (int64_t)(P.mfma64 ? 1 : 0) << 50;
     auto it = S->graphs.find(key);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -1647,13 +2194,27 @@ int read_state(vfem_system *S, CGState &h)
     return 0;
 }
 
+int read_states(vfem_system *S, std::vector<CGState> &h)
+{
+    h.resize(S->ncols);
+    HIP_TRY(hipMemcpyAsync(h.data(), S->st, S->ncols * sizeof(CGState), hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+// PCG on the device for S->ncols right-hand sides at once (column c of f, x in
+// column c of the level-0 vectors; each column has its own CG state and stops
+// on its own).  stats: S->ncols entries (may be null).  Returns the worst
+// column status.
 int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, bool warm)
 {
     Plan P;
     if (int rc = check_plan(S, o, P)) return rc;
+    const int nc = S->ncols;
     const int64_t n3 = S->n3;
+    const int64_t n3all = n3 * nc;
     const int gf = grid_for_flat(n3);
-    const bool mixed = P.precision != VFEM_FP64;
+    const bool mixed = P.precision != VFEM_FP64 || P.mfma64;
     hipGraphExec_t g_norm = nullptr, g_true = nullptr;
     if (o.use_graph) {
         if (int rc = get_iteration_graph(S, P, false, &g_norm)) return rc;
@@ -1661,46 +2222,58 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             if (int rc = get_iteration_graph(S, P, true, &g_true)) return rc;
     }
     HIP_TRY(hipEventRecord(S->ev0, S->stream));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, S->stream, S->st, o.max_iter, o.tol, 0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, o.max_iter, o.tol, 0, nc);
     // |f|^2 and the threshold; r = f with x = 0, or r = f - A x0
-    hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr, S->r,
-                       warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
+    hipLaunchKernelGGL(k_residual, dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr,
+                       S->r, warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
     if (warm)
         enqueue_true_residual(S);
     else  // initial r.r test (Eigen: residualNorm2 < threshold -> 0 iterations)
-        hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->r, S->r, S->st,
-                           S->red, (int)FIN_RR, 0);
-    HIP_TRY(hipMemsetAsync(S->p64, 0, n3 * sizeof(double), S->stream));
-    HIP_TRY(hipMemsetAsync(S->p32, 0, n3 * sizeof(float), S->stream));
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->r, S->r,
+                           S->st, S->red, (int)FIN_RR, 0);
+    HIP_TRY(hipMemsetAsync(S->p64, 0, n3all * sizeof(double), S->stream));
+    HIP_TRY(hipMemsetAsync(S->p32, 0, n3all * sizeof(float), S->stream));
     // p = M r, absNew = r.p
     enqueue_precond(S, P, S->r, FIN_RZ_INIT);
     enqueue_pupdate(S, P);
-    CGState h{};
-    if (int rc = read_state(S, h)) return rc;
-    int true_count = warm ? 1 : 0;
+    std::vector<CGState> h;
+    if (int rc = read_states(S, h)) return rc;
+    std::vector<int> true_count(nc, warm ? 1 : 0);
     const int chunk = mixed ? std::max(1, o.true_res_every) : std::max(1, o.check_every);
     int64_t launched = 0;
     const int64_t cap = (int64_t)o.max_iter + 2 * chunk + 8;
-    if (o.max_iter == 0 && !h.done) h.done = 2;
+    std::vector<int> capped(nc, 0);
+    if (o.max_iter == 0)
+        for (int c = 0; c < nc; c++)
+            if (!h[c].done) capped[c] = 1;
+    auto all_done = [&]() {
+        for (int c = 0; c < nc; c++)
+            if (!h[c].done && !capped[c]) return false;
+        return true;
+    };
     while (true) {
-        if (h.done) {
+        if (all_done()) {
             // mixed precision stops on the FP64 true residual only
-            if (h.done == 1 && mixed && !h.last_true) {
-                enqueue_true_residual(S);
-                true_count++;
-                if (int rc = read_state(S, h)) return rc;
-                if (!h.done) {
-                    // finish the interrupted iteration: z = M r, beta, p
-                    enqueue_precond(S, P, S->r, FIN_RZ);
-                    enqueue_pupdate(S, P);
-                    if (int rc = read_state(S, h)) return rc;
-                    continue;
+            bool verify = false;
+            for (int c = 0; c < nc; c++)
+                if (h[c].done == 1 && mixed && !h[c].last_true) {
+                    verify = true;
+                    true_count[c]++;
                 }
+            if (!verify) break;
+            enqueue_true_residual(S);
+            if (int rc = read_states(S, h)) return rc;
+            if (!all_done()) {
+                // finish the interrupted iteration of the reopened columns: z = M r, beta, p
+                enqueue_precond(S, P, S->r, FIN_RZ);
+                enqueue_pupdate(S, P);
+                if (int rc = read_states(S, h)) return rc;
             }
-            break;
+            continue;
         }
         if (launched >= cap) {
-            h.done = 2;
+            for (int c = 0; c < nc; c++)
+                if (!h[c].done) capped[c] = 1;
             break;
         }
         for (int k = 0; k < chunk; k++) {
@@ -1708,38 +2281,44 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             hipGraphExec_t g = wt ? g_true : g_norm;
             if (g) HIP_TRY(hipGraphLaunch(g, S->stream));
             else enqueue_iteration(S, P, wt);
-            if (wt) true_count++;
+            if (wt)
+                for (int c = 0; c < nc; c++)
+                    if (!h[c].done) true_count[c]++;
         }
         launched += chunk;
-        if (int rc = read_state(S, h)) return rc;
+        if (int rc = read_states(S, h)) return rc;
     }
     HIP_TRY(hipEventRecord(S->ev1, S->stream));
     HIP_TRY(hipEventSynchronize(S->ev1));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, S->ev0, S->ev1));
-    double rec = h.rhs2 > 0 ? std::sqrt(h.rr / h.rhs2) : 0.0;
     // true residual at exit (FP64), outside the timed region
     launch_matvec_fine<double>(S, S->x, S->t64, -1, 0);
-    hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, S->stream, n3, S->f, S->t64, S->t64,
+    hipLaunchKernelGGL(k_residual, dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->f, S->t64, S->t64,
                        (double *)nullptr, S->st, S->red, (int)FIN_STORE);
-    CGState h2{};
-    if (int rc = read_state(S, h2)) return rc;
-    int status = VFEM_OK;
-    if (h.done == 2) status = VFEM_ERR_NOT_CONVERGED;
-    if (h.done == 3) status = VFEM_ERR_BREAKDOWN;
-    if (stats) {
-        stats->iterations = h.iter;
-        stats->converged = h.done == 1;
-        stats->rec_residual = rec;
-        stats->rel_residual = h.rhs2 > 0 ? std::sqrt(h2.result / h.rhs2) : 0.0;
-        stats->solve_ms = ms;
-        stats->setup_ms = S->setup_ms;
-        stats->true_res_count = true_count;
-        stats->status = status;
+    std::vector<CGState> h2;
+    if (int rc = read_states(S, h2)) return rc;
+    int worst = VFEM_OK;
+    for (int c = 0; c < nc; c++) {
+        int status = VFEM_OK;
+        if (h[c].done == 2 || capped[c]) status = VFEM_ERR_NOT_CONVERGED;
+        if (h[c].done == 3) status = VFEM_ERR_BREAKDOWN;
+        if (status == VFEM_ERR_BREAKDOWN || (status && worst == VFEM_OK)) worst = status;
+        if (stats) {
+            vfem_stats &s = stats[c];
+            s.iterations = h[c].iter;
+            s.converged = h[c].done == 1;
+            s.rec_residual = h[c].rhs2 > 0 ? std::sqrt(h[c].rr / h[c].rhs2) : 0.0;
+            s.rel_residual = h[c].rhs2 > 0 ? std::sqrt(h2[c].result / h[c].rhs2) : 0.0;
+            s.solve_ms = ms;
+            s.setup_ms = S->setup_ms;
+            s.true_res_count = true_count[c];
+            s.status = status;
+        }
     }
-    if (status == VFEM_ERR_NOT_CONVERGED) g_last_error = "max_iter reached before the tolerance";
-    if (status == VFEM_ERR_BREAKDOWN) g_last_error = "CG breakdown (non-positive p.Ap or non-finite scalar)";
-    return status;
+    if (worst == VFEM_ERR_NOT_CONVERGED) g_last_error = "max_iter reached before the tolerance";
+    if (worst == VFEM_ERR_BREAKDOWN) g_last_error = "CG breakdown (non-positive p.Ap or non-finite scalar)";
+    return worst;
 }
 
 }  // namespace
@@ -1931,6 +2510,46 @@ int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options 
     return rc2 ? rc2 : rc;
 }
 
+int vfem_set_rhs_block(vfem_system *sys, const double *f, int32_t nrhs)
+{
+    if (!sys || !f || nrhs < 1 || nrhs > kMaxCols) {
+        g_last_error = "invalid argument (nrhs must be 1..256)";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (nrhs > sys->nb)
+        if (int rc = alloc_vectors(sys, nrhs)) return rc;
+    HIP_TRY(hipMemcpy(sys->f, f, (size_t)nrhs * sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    return VFEM_OK;
+}
+
+int vfem_solve_resident_block(vfem_system *sys, int32_t nrhs, const vfem_options *opts, vfem_stats *stats)
+{
+    if (!sys || nrhs < 1 || nrhs > sys->nb) {
+        g_last_error = "invalid argument (nrhs exceeds the columns set by vfem_set_rhs_block)";
+        return VFEM_ERR_ARGUMENT;
+    }
+    vfem_options o;
+    vfem_default_options(&o);
+    if (opts) o = *opts;
+    struct Cols {
+        vfem_system *S;
+        ~Cols() { S->ncols = 1; }
+    } guard{sys};
+    sys->ncols = nrhs;
+    return solve_resident(sys, o, stats, o.warm_start != 0);
+}
+
+int vfem_get_solution_block(vfem_system *sys, double *u, int32_t nrhs)
+{
+    if (!sys || !u || nrhs < 1 || nrhs > sys->nb) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(u, sys->x, (size_t)nrhs * sys->n3 * sizeof(double), hipMemcpyDeviceToHost));
+    return VFEM_OK;
+}
+
 int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrhs, const vfem_options *opts,
                        vfem_stats *stats)
 {
@@ -1938,29 +2557,148 @@ int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrh
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
+    vfem_options o;
+    vfem_default_options(&o);
+    if (opts) o = *opts;
     int worst = VFEM_OK;
-    for (int k = 0; k < nrhs; k++) {
-        vfem_stats s{};
-        int rc = vfem_solve(sys, f + (int64_t)k * sys->n3, u + (int64_t)k * sys->n3, opts, &s);
-        if (stats) stats[k] = s;
-        if (rc != VFEM_OK && worst == VFEM_OK) worst = rc;
-        if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED) return rc;
+    for (int c0 = 0; c0 < nrhs; c0 += kMaxCols) {
+        const int nc = std::min(kMaxCols, nrhs - c0);
+        const int64_t off = (int64_t)c0 * sys->n3;
+        if (int rc = vfem_set_rhs_block(sys, f + off, nc)) return rc;
+        if (o.warm_start)
+            HIP_TRY(hipMemcpy(sys->x, u + off, (size_t)nc * sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+        int rc = vfem_solve_resident_block(sys, nc, &o, stats ? stats + c0 : nullptr);
+        if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED && rc != VFEM_ERR_BREAKDOWN) return rc;
+        if (rc != VFEM_OK && (worst == VFEM_OK || rc == VFEM_ERR_BREAKDOWN)) worst = rc;
+        if (int rc2 = vfem_get_solution_block(sys, u + off, nc)) return rc2;
     }
     return worst;
 }
 
+int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs, int32_t precision)
+{
+    if (!sys || !x || !y || nrhs < 1 || nrhs > kMaxCols || precision < 0 || precision > 2) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (nrhs > sys->nb)
+        if (int rc = alloc_vectors(sys, nrhs)) return rc;
+    const int64_t n3 = sys->n3, nall = n3 * nrhs;
+    struct Cols {
+        vfem_system *S;
+        ~Cols() { S->ncols = 1; }
+    } guard{sys};
+    sys->ncols = nrhs;
+    const int gf = grid_for_flat(n3);
+    HIP_TRY(hipMemcpy(sys->t64, x, nall * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs);
+    if (precision == VFEM_FP64) {
+        launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
+    } else {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(gf, nrhs), dim3(kBlock), 0, sys->stream, n3, sys->t64,
+                           sys->p32);
+        launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, precision == VFEM_BF16 ? 1 : 6);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gf, nrhs), dim3(kBlock), 0, sys->stream, n3,
+                           sys->ap32, sys->ap64);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(y, sys->ap64, nall * sizeof(double), hipMemcpyDeviceToHost));
+    return VFEM_OK;
+}
+
+int32_t vfem_block_config_count(void) { return kNumBlockCfgs; }
+
+int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz)
+{
+    if (index < 0 || index >= kNumBlockCfgs || !tex || !tey || !mz) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    *tex = kBlockCfgs[index].tex;
+    *tey = kBlockCfgs[index].tey;
+    *mz = kBlockCfgs[index].mz;
+    return VFEM_OK;
+}
+
+int vfem_set_block_config(vfem_system *sys, int32_t index)
+{
+    if (!sys || index < 0 || index >= kNumBlockCfgs) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    sys->block_cfg = index;
+    for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
+    sys->graphs.clear();
+    return VFEM_OK;
+}
+
+int vfem_autotune_block(vfem_system *sys, int32_t nrhs, int32_t passes, int32_t reps, double *ms_out,
+                        int32_t *best_out)
+{
+    if (!sys || nrhs < 2 || nrhs > kMaxCols || reps < 1 || (passes != 1 && passes != 3 && passes != 6)) {
+        g_last_error = "invalid argument (nrhs 2..256, passes 1, 3 or 6, reps >= 1)";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (nrhs > sys->nb)
+        if (int rc = alloc_vectors(sys, nrhs)) return rc;
+    struct Cols {
+        vfem_system *S;
+        ~Cols() { S->ncols = 1; }
+    } guard{sys};
+    sys->ncols = nrhs;
+    const int64_t nall = sys->n3 * nrhs;
+    hipLaunchKernelGGL(k_fill_hash, dim3(2048), dim3(kBlock), 0, sys->stream, nall, sys->p32, 12345u);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs);
+    // reference output: configuration 0
+    launch_matvec_block(sys, sys->p32, sys->z32, -1, 0, passes, 0);
+    std::vector<float> ref(nall), out(nall);
+    HIP_TRY(hipStreamSynchronize(sys->stream));
+    HIP_TRY(hipMemcpy(ref.data(), sys->z32, nall * sizeof(float), hipMemcpyDeviceToHost));
+    int best = -1;
+    double best_ms = 1e30;
+    for (int c = 0; c < kNumBlockCfgs; c++) {
+        launch_matvec_block(sys, sys->p32, sys->ap32, -1, 0, passes, c);
+        HIP_TRY(hipEventRecord(sys->ev0, sys->stream));
+        for (int r = 0; r < reps; r++) launch_matvec_block(sys, sys->p32, sys->ap32, -1, 0, passes, c);
+        HIP_TRY(hipEventRecord(sys->ev1, sys->stream));
+        HIP_TRY(hipEventSynchronize(sys->ev1));
+        HIP_TRY(hipGetLastError());
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        // every tile shape must give the same product: the element contractions and the
+        // gather order do not depend on the tiling
+        HIP_TRY(hipMemcpy(out.data(), sys->ap32, nall * sizeof(float), hipMemcpyDeviceToHost));
+        const bool same = std::memcmp(out.data(), ref.data(), nall * sizeof(float)) == 0;
+        const double avg = same ? ms / reps : -1.0;
+        if (ms_out) ms_out[c] = avg;
+        if (same && avg < best_ms) {
+            best_ms = avg;
+            best = c;
+        }
+    }
+    if (best < 0) {
+        g_last_error = "no tile shape reproduced the reference product";
+        return VFEM_ERR_DEVICE;
+    }
+    if (best_out) *best_out = best;
+    return vfem_set_block_config(sys, best);
+}
+
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
 {
-    if (!sys || !x || !y || precision < 0 || precision > 2) {
+    if (!sys || !x || !y || precision < 0 || precision > 3) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(hipMemcpy(sys->t64, x, n3 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0, 0);
-    if (precision == VFEM_FP64) {
-        launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
+    if (precision == VFEM_FP64 || precision == VFEM_MIXED) {
+        sys->l0_mfma = precision == VFEM_MIXED;
+        launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0, 6);
+        sys->l0_mfma = false;
     } else {
         hipLaunchKernelGGL((k_convert<double, float>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->t64, sys->p32);
         launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, precision == VFEM_BF16 ? 1 : 6);
@@ -1987,7 +2725,7 @@ int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_o
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(hipMemcpy(sys->r, r, n3 * sizeof(double), hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1, 0.0, 0);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
     enqueue_precond(sys, P, sys->r, FIN_STORE);
     if (P.precision != VFEM_FP64 && sys->H.nlev > 1)
         hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->z32,
@@ -2026,10 +2764,13 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     o.precision = precision;
     Plan P;
     if (int rc = check_plan(sys, o, P)) return rc;
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(64), 0, sys->stream, sys->st, 1 << 30, 0.0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1 << 30, 0.0, 1, 1);
     auto one = [&]() {
         if (which == 0) {
-            if (P.precision == VFEM_FP64) launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0);
+            if (P.precision == VFEM_FP64) {
+                L0Mode mode(sys, P);
+                launch_matvec_fine<double>(sys, sys->p64, sys->ap64, -1, 0, P.op_passes);
+            }
             else launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, P.bf16 ? 1 : P.op_passes);
         } else if (which == 3) {
             launch_matvec_fine<float>(sys, sys->p32, sys->ap32, -1, 0, 3);

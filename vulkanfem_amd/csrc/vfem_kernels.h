@@ -59,6 +59,7 @@ enum Finalize {
 };
 
 constexpr int kMaxPartials = 4096;
+constexpr int kMaxCols = 256;  // right-hand sides of one batched solve
 
 struct Reduction {
     double *partials;   // [2 * cap]: block partials of up to two sums
