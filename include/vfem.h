@@ -30,6 +30,7 @@ extern "C" {
 #endif
 
 typedef struct vfem_system vfem_system;
+typedef struct vfem_unet vfem_unet;
 
 enum vfem_status {
     VFEM_OK = 0,
@@ -72,6 +73,10 @@ typedef struct {
     int32_t op_passes;       /* reduced precision: BF16 products of the CG operator matvec, 3 or 6; 0 = 6 */
     int32_t cycle_passes;    /* reduced precision: BF16 products of the level-0 cycle matvec, 1, 3 or 6;
                                 0 = 6 (VFEM_FP32) or 1 (VFEM_BF16) */
+    int32_t learned;         /* 1: damped cycle with the attached learned coarse correction (flexible CG) */
+    double stall_ratio;      /* learned: fall back to the plain cycle for the rest of the solve when the
+                                residual norm falls by less than this factor per iteration over a
+                                polling interval; default 0.9 */
 } vfem_options;
 
 typedef struct {
@@ -83,6 +88,8 @@ typedef struct {
     double setup_ms;         /* host + device setup time of vfem_assemble */
     int32_t true_res_count;  /* FP64 true-residual recomputations done */
     int32_t status;
+    int32_t learned_iters;   /* iterations preconditioned with the learned correction */
+    int32_t fell_back;       /* 1 if the solve fell back to the plain cycle */
 } vfem_stats;
 
 /* Fill *o with the defaults listed above. */
@@ -105,6 +112,10 @@ int64_t vfem_num_dofs(const vfem_system *sys);
 
 /* MatrixFreeSparse::numLevels after the early stop (include/MatrixFreeSparse.hpp:86). */
 int32_t vfem_num_levels(const vfem_system *sys);
+
+/* Node-grid extents of level `level` (no reference counterpart: the reference's
+ * level dims are only known inside FEM.cpp:253-259). */
+int vfem_level_extent(const vfem_system *sys, int32_t level, int32_t *nx, int32_t *ny, int32_t *nz);
 
 /* Per-level sizes: elements, used nodes, fixed nodes of level `level`. */
 int vfem_level_info(const vfem_system *sys, int32_t level, int64_t *n_elems, int64_t *n_nodes, int64_t *n_fixed);
@@ -197,6 +208,47 @@ int vfem_inspect_hierarchy(const int32_t *voxels, int32_t nx, int32_t ny, int32_
                            const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
                            int32_t level, int32_t *n_levels_out, int64_t *n_nodes, int64_t *n_fixed_out,
                            double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out);
+
+/* ---- learned coarse-grid correction (no reference counterpart) --------------
+ * A 3-D conv U-Net (4 resolutions, 16-64 channels, 274,800 weights, BF16
+ * arithmetic with FP32 accumulation, FP32 master weights) that maps a coarse
+ * level's residual and occupancy to the correction the damped cycle would get
+ * from the levels below; trained and run by the library's own HIP kernels.
+ * Tensors are [batch][channel][z][y][x] float32.  Input channels: the level
+ * residual's three components divided by their RMS over the free DOFs, and the
+ * node code 1 (free), -1 (fixed), 0 (unused); output: the error estimate in
+ * the same scaling.  Weights are layer by layer, each [cout][cin][k][k][k]. */
+int64_t vfem_unet_num_params(void);
+int vfem_unet_create(uint64_t seed, vfem_unet **out);
+void vfem_unet_destroy(vfem_unet *net);
+int vfem_unet_get_weights(vfem_unet *net, float *w);
+int vfem_unet_set_weights(vfem_unet *net, const float *w);
+/* test hooks: forward of a host batch, then the weight gradients (and the input
+ * gradient when gx is non-null) for the upstream gradient gout of that batch */
+int vfem_unet_forward(vfem_unet *net, const float *x, int32_t batch, int32_t d, int32_t h, int32_t w, float *out);
+int vfem_unet_backward(vfem_unet *net, const float *gout, float *gw, float *gx);
+/* Adam training on (x, y) pairs with a node mask [n][d][h][w]: loss per sample
+ * |mask (N(x) - y)|^2 / |mask y|^2; cosine-decayed learning rate; epoch_loss
+ * receives the mean loss of each epoch. */
+int vfem_unet_train(vfem_unet *net, const float *x, const float *y, const float *mask, int64_t n_samples, int32_t d,
+                    int32_t h, int32_t w, int32_t epochs, int32_t batch, float lr, uint64_t seed, float *epoch_loss);
+
+/* Training pairs produced by the solver: runs up to n_iter FP64 PCG iterations
+ * of the damped cycle on right-hand side f and, after every iteration, takes
+ * the residual r_l that the cycle restricted to `level` (1 <= level < levels-1)
+ * and writes, on that level's node grid (d, h, w = level node extents):
+ *   x[k]   = [r_l / s, node code]          (4 channels, s = RMS of r_l on free DOFs)
+ *   y[k]   = A_l^-1 r_l / s                (3 channels; exact, dense Cholesky of
+ *                                           the level operator on its free DOFs)
+ *   yvc[k] = (the cycle's own correction from level l down) / s   (3 channels)
+ *   mask[k] = 1 on free used nodes
+ * *count receives the number of pairs (<= n_iter); grid receives d, h, w. */
+int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, int32_t n_iter, float *x, float *y,
+                              float *yvc, float *mask, int32_t *count, int32_t *grid);
+
+/* Use `net` as the coarse-grid correction of the damped cycle at `level`
+ * (options.learned = 1 turns it on per solve); net = NULL detaches. */
+int vfem_attach_unet(vfem_system *sys, vfem_unet *net, int32_t level);
 
 /* Name of a status code, and the last error message of the calling thread. */
 const char *vfem_status_string(int status);

@@ -157,3 +157,11 @@ def test_cxx_shim_compiles_and_links(tmp_path):
                     "-L", libdir, "-lvfem", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     assert out.split() == ["1e-08", "30000"]
+
+
+def test_generators_reject_empty_models():
+    empty = np.zeros((8, 8, 8), np.int32)
+    with pytest.raises(ValueError):
+        problems.patch_load_cases(empty, 1, seed=0)
+    with pytest.raises(ValueError):
+        problems.batched_problem(32, 20, 2.0, 9.0, seed=107, nrhs=1)   # no ellipsoid reaches z = 0

@@ -38,13 +38,15 @@ class Options(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("max_iter", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("cycle", ctypes.c_int32), ("omega", ctypes.c_double), ("true_res_every", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32),
-                ("op_passes", ctypes.c_int32), ("cycle_passes", ctypes.c_int32)]
+                ("op_passes", ctypes.c_int32), ("cycle_passes", ctypes.c_int32), ("learned", ctypes.c_int32),
+                ("stall_ratio", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_residual", ctypes.c_double),
                 ("rec_residual", ctypes.c_double), ("solve_ms", ctypes.c_double), ("setup_ms", ctypes.c_double),
-                ("true_res_count", ctypes.c_int32), ("status", ctypes.c_int32)]
+                ("true_res_count", ctypes.c_int32), ("status", ctypes.c_int32), ("learned_iters", ctypes.c_int32),
+                ("fell_back", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -111,6 +113,22 @@ def lib():
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                                          ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.POINTER(ctypes.c_int64)]
+    _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.vfem_unet_num_params.restype = ctypes.c_int64
+    L.vfem_unet_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.vfem_unet_destroy.argtypes = [vp]
+    L.vfem_unet_destroy.restype = None
+    L.vfem_unet_get_weights.argtypes = [vp, _f32p]
+    L.vfem_unet_set_weights.argtypes = [vp, _f32p]
+    L.vfem_unet_forward.argtypes = [vp, _f32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f32p]
+    L.vfem_unet_backward.argtypes = [vp, _f32p, _f32p, ctypes.c_void_p]
+    L.vfem_unet_train.argtypes = [vp, _f32p, _f32p, _f32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, ctypes.c_uint64,
+                                  _f32p]
+    L.vfem_collect_coarse_pairs.argtypes = [vp, _f64p, ctypes.c_int32, ctypes.c_int32, _f32p, _f32p, _f32p, _f32p,
+                                            ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+    L.vfem_attach_unet.argtypes = [vp, vp, ctypes.c_int32]
+    L.vfem_level_extent.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
     L.vfem_status_string.argtypes = [ctypes.c_int]
     L.vfem_status_string.restype = ctypes.c_char_p
     L.vfem_last_error.restype = ctypes.c_char_p
@@ -127,6 +145,68 @@ def block_configs():
         L.vfem_block_config(i, ctypes.byref(t), ctypes.byref(u), ctypes.byref(m))
         out.append((t.value, u.value, m.value))
     return out
+
+
+def inspect_level_dims(mat, level):
+    """Node grid (nz, ny, nx) of `level` of an assembled system."""
+    nx, ny, nz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _check(mat._lib.vfem_level_extent(mat._h, int(level), ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+    return (nz.value, ny.value, nx.value)
+
+
+class UNet:
+    """The learned coarse-grid correction (include/vfem.h, vfem_unet_*)."""
+
+    def __init__(self, seed=0):
+        self._lib = lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.vfem_unet_create(int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.vfem_unet_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def num_params():
+        return int(lib().vfem_unet_num_params())
+
+    def get_weights(self):
+        w = np.zeros(self.num_params(), np.float32)
+        _check(self._lib.vfem_unet_get_weights(self._h, w))
+        return w
+
+    def set_weights(self, w):
+        _check(self._lib.vfem_unet_set_weights(self._h, np.ascontiguousarray(w, np.float32)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        B, C, D, H, W = x.shape
+        out = np.zeros((B, 3, D, H, W), np.float32)
+        _check(self._lib.vfem_unet_forward(self._h, x.reshape(-1), B, D, H, W, out.reshape(-1)))
+        return out
+
+    def backward(self, gout, input_grad=True):
+        """Weight gradients (flat, layer by layer) and the input gradient of the
+        last forward batch for upstream gradient gout."""
+        gout = np.ascontiguousarray(gout, np.float32)
+        gw = np.zeros(self.num_params(), np.float32)
+        B, _, D, H, W = gout.shape
+        gx = np.zeros((B, 4, D, H, W), np.float32) if input_grad else None
+        _check(self._lib.vfem_unet_backward(self._h, gout.reshape(-1), gw,
+                                            gx.ctypes.data_as(ctypes.c_void_p) if gx is not None else None))
+        return gw, gx
+
+    def train(self, x, y, mask, epochs=10, batch=32, lr=1e-3, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        mask = np.ascontiguousarray(mask, np.float32)
+        n, _, D, H, W = x.shape
+        losses = np.zeros(epochs, np.float32)
+        _check(self._lib.vfem_unet_train(self._h, x.reshape(-1), y.reshape(-1), mask.reshape(-1), n, D, H, W,
+                                         int(epochs), int(batch), float(lr), int(seed), losses))
+        return losses
 
 
 def _check(rc):
@@ -290,6 +370,29 @@ class MatrixFreeSparse:
         best = ctypes.c_int32()
         _check(self._lib.vfem_autotune_block(self._h, int(nrhs), int(passes), int(reps), ms, ctypes.byref(best)))
         return best.value, {i: block_configs()[i] + (float(ms[i]),) for i in range(n)}
+
+    def collect_coarse_pairs(self, f, level, n_iter):
+        """(x, y, yvc, mask) training pairs on `level`'s node grid from up to n_iter
+        FP64 PCG iterations on right-hand side f (vfem_collect_coarse_pairs)."""
+        L = self._lib
+        hl = inspect_level_dims(self, level)
+        V = int(np.prod(hl))
+        x = np.zeros((n_iter, 4) + hl, np.float32)
+        y = np.zeros((n_iter, 3) + hl, np.float32)
+        yvc = np.zeros((n_iter, 3) + hl, np.float32)
+        m = np.zeros((n_iter,) + hl, np.float32)
+        cnt, grid = ctypes.c_int32(), (ctypes.c_int32 * 3)()
+        _check(L.vfem_collect_coarse_pairs(self._h, np.ascontiguousarray(f, np.float64), int(level), int(n_iter),
+                                           x.reshape(-1), y.reshape(-1), yvc.reshape(-1), m.reshape(-1),
+                                           ctypes.byref(cnt), grid))
+        assert tuple(grid) == hl and V > 0
+        k = cnt.value
+        return x[:k], y[:k], yvc[:k], m[:k]
+
+    def attach_unet(self, net, level):
+        """Use `net` (UNet) as the learned coarse correction at `level`; None detaches."""
+        self._unet = net
+        _check(self._lib.vfem_attach_unet(self._h, net._h if net is not None else None, int(level)))
 
     def time_kernel(self, which, precision=FP64, reps=10):
         out = ctypes.c_double()

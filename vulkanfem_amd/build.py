@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvfem.so")
 
-SOURCES = ["vfem_setup.cpp", "vfem_device.hip"]
+SOURCES = ["vfem_setup.cpp", "vfem_device.hip", "vfem_unet.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

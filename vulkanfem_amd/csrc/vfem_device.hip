@@ -38,6 +38,7 @@
 #include "vfem.h"
 #include "vfem_internal.h"
 #include "vfem_kernels.h"
+#include "vfem_unet.h"
 
 namespace vfem {
 
@@ -1575,6 +1576,96 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const Ts *__restr
     for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) d[i] = Td(s[i]);
 }
 
+// ---- learned coarse correction: level vector <-> network grid ---------------
+
+// s = RMS of r over the level's free DOFs (1 when r vanishes there); one block
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unet_scale(int64_t n3, const uint32_t *__restrict__ fixc,
+                                                       const T *__restrict__ r, float *__restrict__ scale,
+                                                       const CGState *st)
+{
+    st = col_state(st);
+    if (st->done) return;
+    r += col_offset(n3);
+    double s2 = 0, cnt = 0;
+    for (int64_t k = threadIdx.x; k < n3; k += kBlock) {
+        const int64_t node = k / 3;
+        if ((fixc[node >> 5] >> (node & 31)) & 1) continue;
+        s2 += (double)r[k] * (double)r[k];
+        cnt += 1;
+    }
+    __shared__ double sh[2][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    s2 = wave_sum(s2);
+    cnt = wave_sum(cnt);
+    if (lane == 0) {
+        sh[0][wave] = s2;
+        sh[1][wave] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int w = 0; w < kWaves; w++) {
+            a += sh[0][w];
+            c += sh[1][w];
+        }
+        const double rms = c > 0 ? sqrt(a / c) : 0.0;
+        scale[blockIdx.y] = rms > 0 ? (float)rms : 1.0f;
+    }
+}
+
+// network input on the level's node grid: [r / s (3 channels), node code]
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unet_in(DevLevel L, const T *__restrict__ r,
+                                                    const float *__restrict__ scale, float *__restrict__ in,
+                                                    const CGState *st)
+{
+    st = col_state(st);
+    if (st->done) return;
+    r += col_offset(3 * L.nn);
+    const int64_t V = (int64_t)L.nx * L.ny * L.nz;
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const int x = (int)(v % L.nx), y = (int)((v / L.nx) % L.ny), z = (int)(v / ((int64_t)L.nx * L.ny));
+    const int id = lookup(L, x, y, z);
+    const float inv = 1.0f / scale[blockIdx.y];
+    float code = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+    if (id >= 0) {
+        const bool fx = (L.fbits[((int64_t)z * L.ny + y) * L.S + (x >> 6)] >> (x & 63)) & 1;
+        code = fx ? -1.0f : 1.0f;
+        c0 = (float)r[3 * id + 0] * inv;
+        c1 = (float)r[3 * id + 1] * inv;
+        c2 = (float)r[3 * id + 2] * inv;
+    }
+    float *o = in + (int64_t)blockIdx.y * 4 * V;
+    o[v] = c0;
+    o[V + v] = c1;
+    o[2 * V + v] = c2;
+    o[3 * V + v] = code;
+}
+
+// z = s * network output at used nodes, zero at fixed nodes
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unet_out(DevLevel L, const float *__restrict__ out,
+                                                     const float *__restrict__ scale, T *__restrict__ z,
+                                                     const CGState *st)
+{
+    st = col_state(st);
+    if (st->done) return;
+    z += col_offset(3 * L.nn);
+    const int64_t V = (int64_t)L.nx * L.ny * L.nz;
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const int x = (int)(v % L.nx), y = (int)((v / L.nx) % L.ny), zz = (int)(v / ((int64_t)L.nx * L.ny));
+    const int id = lookup(L, x, y, zz);
+    if (id < 0) return;
+    const bool fx = (L.fbits[((int64_t)zz * L.ny + y) * L.S + (x >> 6)] >> (x & 63)) & 1;
+    const float s = scale[blockIdx.y];
+    const float *o = out + (int64_t)blockIdx.y * 3 * V;
+#pragma unroll
+    for (int c = 0; c < 3; c++) z[3 * id + c] = fx ? T(0) : T(o[c * V + v] * s);
+}
+
 // deterministic pseudo-random fill in [-0.5, 0.5) (autotuner input)
 __global__ __launch_bounds__(kBlock) void k_fill_hash(int64_t n, float *__restrict__ v, uint32_t seed)
 {
@@ -1696,6 +1787,10 @@ struct vfem_system {
     int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
     int block_cfg = 0;          // kBlockCfgs entry of the multi-RHS level-0 matvec (set by the autotuner)
     bool l0_mfma = false;       // FP64 level-0 matvecs on the matrix cores (set while enqueueing a VFEM_MIXED plan)
+    vfem_unet *unet = nullptr;  // learned coarse correction (not owned)
+    int unet_level = -1;
+    bool unet_on = false;       // set while enqueueing a learned plan
+    float *unet_scale = nullptr;
     bool use_block = true;      // batched FP32 level-0 matvec: one launch per 16 columns on the matrix cores
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
@@ -2011,6 +2106,19 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         launch_coarse_solve<T>(S, (const T *)rl, zl, true);
         return;
     }
+    if (S->unet_on && l == S->unet_level) {
+        // learned coarse correction: z_l = s N(r_l / s) in place of the cycle from level l down
+        const DevLevel &L = S->dl[l];
+        const int64_t V = (int64_t)L.nx * L.ny * L.nz;
+        hipLaunchKernelGGL((k_unet_scale<TR>), dim3(1, S->ncols), dim3(kBlock), 0, S->stream, 3 * L.nn, S->fixc[l], rl,
+                           S->unet_scale, S->st);
+        hipLaunchKernelGGL((k_unet_in<TR>), dim3((V + kBlock - 1) / kBlock, S->ncols), dim3(kBlock), 0, S->stream, L,
+                           rl, S->unet_scale, unet_input(S->unet), S->st);
+        unet_forward_enqueue(S->unet, S->stream);
+        hipLaunchKernelGGL((k_unet_out<T>), dim3((V + kBlock - 1) / kBlock, S->ncols), dim3(kBlock), 0, S->stream, L,
+                           unet_output(S->unet), S->unet_scale, zl, S->st);
+        return;
+    }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
     const int64_t n3l = 3 * L.nn;
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
@@ -2037,6 +2145,8 @@ struct Plan {
     int op_passes;     // BF16 MFMA products of the reduced-precision CG operator (3 or 6)
     int cycle_passes;  // BF16 MFMA products of the cycle's level-0 matvecs (1, 3 or 6)
     bool mfma64;       // VFEM_MIXED: FP64 vectors, level-0 matvecs on the matrix cores
+    bool learned;      // damped cycle with the learned coarse correction
+    bool force_flex;   // flexible CG although the cycle is fixed (after a learned phase)
 };
 
 // While alive: level-0 FP64 matvecs of the plan go to the matrix cores (VFEM_MIXED).
@@ -2047,16 +2157,25 @@ struct L0Mode {
     ~L0Mode() { S->l0_mfma = old; }
 };
 
+// While alive: the damped cycle uses the learned coarse correction of the plan.
+struct LearnedMode {
+    vfem_system *S;
+    bool old;
+    LearnedMode(vfem_system *s, const Plan &P) : S(s), old(s->unet_on) { S->unet_on = P.learned; }
+    ~LearnedMode() { S->unet_on = old; }
+};
+
 // z = M r on the level-0 FP64 residual r, reducing r.z with `fin`.  The
 // reduced-precision cycles run flexible CG: FIN_RZ becomes FIN_RZ_FLEX, which
 // also reduces r.z_old.
 bool flexible(const vfem_system *S, const Plan &P)
 {
-    return (P.precision != VFEM_FP64 || P.mfma64) && S->H.nlev > 1;
+    return (P.precision != VFEM_FP64 || P.mfma64 || P.learned || P.force_flex) && S->H.nlev > 1;
 }
 
 void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 {
+    LearnedMode learned(S, P);
     if (S->H.nlev == 1) {
         // single level: the cycle is the exact coarse solve, z = Kc^-1 r (FP64), then r.z
         launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
@@ -2143,6 +2262,12 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     P.precision = (o.precision == VFEM_FP64 || o.precision == VFEM_MIXED) ? VFEM_FP64 : VFEM_FP32;
     P.mfma64 = o.precision == VFEM_MIXED;
     P.bf16 = o.precision == VFEM_BF16;
+    P.learned = o.learned != 0;
+    P.force_flex = false;
+    if (P.learned && (!S->unet || o.cycle != VFEM_CYCLE_DAMPED || S->unet_level >= S->H.nlev || S->ncols > 1)) {
+        g_last_error = "learned correction needs an attached network, the damped cycle and one right-hand side";
+        return VFEM_ERR_UNSUPPORTED;
+    }
     P.cycle = o.cycle;
     P.omega = o.omega;
     P.op_passes = o.op_passes ? o.op_passes : 6;
@@ -2170,6 +2295,8 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     key += std::llround(P.omega * 1000) * 1024 * 64;
     key += (int64_t)S->ncols << 40;
     key += (int64_t)(P.mfma64 ? 1 : 0) << 50;
+    key += (int64_t)(P.learned ? 1 : 0) << 51;
+    key += (int64_t)(P.force_flex ? 1 : 0) << 52;
     auto it = S->graphs.find(key);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -2243,6 +2370,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     int64_t launched = 0;
     const int64_t cap = (int64_t)o.max_iter + 2 * chunk + 8;
     std::vector<int> capped(nc, 0);
+    int learned_iters = 0, fell_back = 0;
     if (o.max_iter == 0)
         for (int c = 0; c < nc; c++)
             if (!h[c].done) capped[c] = 1;
@@ -2276,6 +2404,8 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
                 if (!h[c].done) capped[c] = 1;
             break;
         }
+        const int iter_before = h[0].iter;
+        const double rr_before = h[0].rr;
         for (int k = 0; k < chunk; k++) {
             const bool wt = mixed && k == chunk - 1;
             hipGraphExec_t g = wt ? g_true : g_norm;
@@ -2287,6 +2417,28 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
         }
         launched += chunk;
         if (int rc = read_states(S, h)) return rc;
+        if (P.learned) {
+            const int done_its = h[0].iter - iter_before;
+            learned_iters += done_its;
+            // the plain cycle judges the learned one: a chunk that reduced the residual
+            // by less than stall_ratio per iteration switches to the plain cycle
+            if (!h[0].done && done_its > 0 && rr_before > 0 &&
+                h[0].rr > rr_before * std::pow(o.stall_ratio, 2.0 * done_its)) {
+                // restart with the plain cycle: p = z = M r, beta = 0, and keep the
+                // flexible beta (Fletcher-Reeves does not recover from the learned phase)
+                P.learned = false;
+                P.force_flex = true;
+                fell_back = 1;
+                enqueue_precond(S, P, S->r, FIN_RZ_INIT);
+                enqueue_pupdate(S, P);
+                g_norm = g_true = nullptr;
+                if (o.use_graph) {
+                    if (int rc = get_iteration_graph(S, P, false, &g_norm)) return rc;
+                    if (mixed)
+                        if (int rc = get_iteration_graph(S, P, true, &g_true)) return rc;
+                }
+            }
+        }
     }
     HIP_TRY(hipEventRecord(S->ev1, S->stream));
     HIP_TRY(hipEventSynchronize(S->ev1));
@@ -2314,6 +2466,8 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             s.setup_ms = S->setup_ms;
             s.true_res_count = true_count[c];
             s.status = status;
+            s.learned_iters = learned_iters;
+            s.fell_back = fell_back;
         }
     }
     if (worst == VFEM_ERR_NOT_CONVERGED) g_last_error = "max_iter reached before the tolerance";
@@ -2343,6 +2497,8 @@ void vfem_default_options(vfem_options *o)
     o->warm_start = 0;
     o->op_passes = 0;
     o->cycle_passes = 0;
+    o->learned = 0;
+    o->stall_ratio = 0.9;
 }
 
 const char *vfem_status_string(int status)
@@ -2573,6 +2729,148 @@ int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrh
         if (int rc2 = vfem_get_solution_block(sys, u + off, nc)) return rc2;
     }
     return worst;
+}
+
+int vfem_level_extent(const vfem_system *sys, int32_t level, int32_t *nx, int32_t *ny, int32_t *nz)
+{
+    if (!sys || !nx || !ny || !nz || level < 0 || level >= sys->H.nlev) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    *nx = sys->H.L[level].nx;
+    *ny = sys->H.L[level].ny;
+    *nz = sys->H.L[level].nz;
+    return VFEM_OK;
+}
+
+int vfem_attach_unet(vfem_system *sys, vfem_unet *net, int32_t level)
+{
+    if (!sys) {
+        g_last_error = "null system";
+        return VFEM_ERR_ARGUMENT;
+    }
+    for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
+    sys->graphs.clear();
+    if (!net) {
+        sys->unet = nullptr;
+        sys->unet_level = -1;
+        return VFEM_OK;
+    }
+    if (level < 1 || level >= sys->H.nlev - 1 || sys->H.L[level].div != 2) {
+        g_last_error = "the learned correction needs a div-2 level between the finest and the coarsest";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const HostLevel &L = sys->H.L[level];
+    if (int rc = unet_prepare(net, L.nz, L.ny, L.nx)) {
+        g_last_error = unet_last_error();
+        return rc;
+    }
+    if (!sys->unet_scale)
+        if (int rc = sys->alloc(&sys->unet_scale, kMaxCols)) return rc;
+    sys->unet = net;
+    sys->unet_level = level;
+    return VFEM_OK;
+}
+
+int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, int32_t n_iter, float *x, float *y,
+                              float *yvc, float *mask, int32_t *count, int32_t *grid)
+{
+    if (!sys || !f || !x || !y || !count || n_iter < 1 || level < 1 || level >= sys->H.nlev - 1 ||
+        sys->H.L[level].div != 2) {
+        g_last_error = "invalid argument (level must lie strictly between the finest and the coarsest)";
+        return VFEM_ERR_ARGUMENT;
+    }
+    *count = 0;
+    const HostLevel &HL = sys->H.L[level];
+    const int D = HL.nz, Hh = HL.ny, W = HL.nx;
+    const int64_t V = (int64_t)D * Hh * W, n3l = 3 * HL.nn;
+    if (grid) {
+        grid[0] = D;
+        grid[1] = Hh;
+        grid[2] = W;
+    }
+    // exact level operator on its free DOFs
+    std::vector<double> A;
+    std::vector<int32_t> freedof;
+    if (assemble_level_dense(sys->H, level, A, freedof) != 0 || !cholesky_factor(A, (int64_t)freedof.size())) {
+        g_last_error = "level operator not positive definite";
+        return VFEM_ERR_SINGULAR;
+    }
+    const int64_t nf = (int64_t)freedof.size();
+    // grid position of every level node
+    std::vector<int64_t> pos(HL.nn, -1);
+    std::vector<char> fixedn(HL.nn, 0);
+    for (int z = 0; z < D; z++)
+        for (int yy = 0; yy < Hh; yy++)
+            for (int xx = 0; xx < W; xx++) {
+                const int id = node_id(HL, xx, yy, z);
+                if (id < 0) continue;
+                pos[id] = ((int64_t)z * Hh + yy) * W + xx;
+                fixedn[id] = node_fixed(HL, xx, yy, z);
+            }
+    if (sys->nb < 1)
+        if (int rc = alloc_vectors(sys, 1)) return rc;
+    sys->ncols = 1;
+    vfem_options o;
+    vfem_default_options(&o);
+    o.cycle = VFEM_CYCLE_DAMPED;
+    Plan P;
+    if (int rc = check_plan(sys, o, P)) return rc;
+    const int64_t n3 = sys->n3;
+    const int gf = grid_for_flat(n3);
+    HIP_TRY(hipMemcpy(sys->f, f, n3 * sizeof(double), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, n_iter + 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->f, (const double *)nullptr,
+                       sys->r, sys->x, sys->st, sys->red, (int)FIN_RHS);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->r, sys->r, sys->st,
+                       sys->red, (int)FIN_RR, 0);
+    HIP_TRY(hipMemsetAsync(sys->p64, 0, n3 * sizeof(double), sys->stream));
+    enqueue_precond(sys, P, sys->r, FIN_RZ_INIT);
+    enqueue_pupdate(sys, P);
+    std::vector<double> r(n3l), zc(n3l), e(nf);
+    int got = 0;
+    for (int it = 0; it < n_iter; it++) {
+        enqueue_iteration(sys, P, false);
+        CGState h{};
+        if (int rc = read_state(sys, h)) return rc;
+        if (h.done) break;
+        // the residual the last cycle restricted to `level`, and the cycle's own correction there
+        HIP_TRY(hipMemcpy(r.data(), sys->rl64[level], n3l * sizeof(double), hipMemcpyDeviceToHost));
+        enqueue_vcycle_damped_level<double, double>(sys, level, sys->rl64[level], sys->zl64[level],
+                                                    sys->tl64[level], o.omega, -1, (const double *)nullptr);
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        HIP_TRY(hipMemcpy(zc.data(), sys->zl64[level], n3l * sizeof(double), hipMemcpyDeviceToHost));
+        double s2 = 0;
+        for (int64_t i = 0; i < nf; i++) s2 += r[freedof[i]] * r[freedof[i]];
+        if (!(s2 > 0)) continue;
+        const double sc = std::sqrt(s2 / (double)nf), inv = 1.0 / sc;
+        for (int64_t i = 0; i < nf; i++) e[i] = r[freedof[i]];
+        cholesky_solve(A, nf, e.data());
+        float *X = x + (int64_t)got * 4 * V, *Y = y + (int64_t)got * 3 * V;
+        float *Yv = yvc ? yvc + (int64_t)got * 3 * V : nullptr;
+        float *M = mask ? mask + (int64_t)got * V : nullptr;
+        std::fill(X, X + 4 * V, 0.0f);
+        std::fill(Y, Y + 3 * V, 0.0f);
+        if (Yv) std::fill(Yv, Yv + 3 * V, 0.0f);
+        if (M) std::fill(M, M + V, 0.0f);
+        for (int64_t id = 0; id < HL.nn; id++) {
+            const int64_t q = pos[id];
+            X[3 * V + q] = fixedn[id] ? -1.0f : 1.0f;
+            if (fixedn[id]) continue;
+            if (M) M[q] = 1.0f;
+            for (int c = 0; c < 3; c++) {
+                X[c * V + q] = (float)(r[3 * id + c] * inv);
+                if (Yv) Yv[c * V + q] = (float)(zc[3 * id + c] * inv);
+            }
+        }
+        for (int64_t i = 0; i < nf; i++) {
+            const int64_t dof = freedof[i];
+            Y[(dof % 3) * V + pos[dof / 3]] = (float)(e[i] * inv);
+        }
+        got++;
+    }
+    *count = got;
+    return VFEM_OK;
 }
 
 int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs, int32_t precision)

@@ -54,4 +54,13 @@ int build_hierarchy(const int32_t *voxels, int nx, int ny, int nz, const double 
 // element's corners to child c's corners (div 2: 8 children, div 4: 64).
 void galerkin_element(const double *ke, int div, uint64_t mask, double *K);
 
+// Dense operator of a div-2 coarse level l >= 1 on its free DOFs (the damped
+// cycle's level operator, from the element tables); freedof[i] = level DOF of
+// row i.  Returns 0, or -1 when the level has no element tables.
+int assemble_level_dense(const Hierarchy &H, int l, std::vector<double> &A, std::vector<int32_t> &freedof);
+bool cholesky_factor(std::vector<double> &A, int64_t n);          // in place, lower, row-major
+void cholesky_solve(const std::vector<double> &Lf, int64_t n, double *b);
+int node_id(const HostLevel &L, int x, int y, int z);               // compact id or -1
+bool node_fixed(const HostLevel &L, int x, int y, int z);
+
 }  // namespace vfem

@@ -153,6 +153,78 @@ std::vector<double> spd_inverse(std::vector<double> Lf, int64_t n, bool &ok)
 
 }  // namespace
 
+bool cholesky_factor(std::vector<double> &A, int64_t n) { return cholesky(A, n); }
+
+void cholesky_solve(const std::vector<double> &Lf, int64_t n, double *b)
+{
+    for (int64_t i = 0; i < n; i++) {
+        double v = b[i];
+        const double *Li = &Lf[i * n];
+        for (int64_t k = 0; k < i; k++) v -= Li[k] * b[k];
+        b[i] = v / Li[i];
+    }
+    for (int64_t i = n - 1; i >= 0; i--) {
+        double v = b[i];
+        for (int64_t k = i + 1; k < n; k++) v -= Lf[k * n + i] * b[k];
+        b[i] = v / Lf[i * n + i];
+    }
+}
+
+int node_id(const HostLevel &L, int x, int y, int z)
+{
+    if (x < 0 || y < 0 || z < 0 || x >= L.nx || y >= L.ny || z >= L.nz) return -1;
+    const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
+    const uint64_t w = L.nbits[seg];
+    if (!((w >> (x & 63)) & 1)) return -1;
+    return L.nbase[seg] + __builtin_popcountll(w & ((1ull << (x & 63)) - 1));
+}
+
+bool node_fixed(const HostLevel &L, int x, int y, int z)
+{
+    const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
+    return (L.fbits[seg] >> (x & 63)) & 1;
+}
+
+int assemble_level_dense(const Hierarchy &H, int l, std::vector<double> &A, std::vector<int32_t> &freedof)
+{
+    if (l < 1 || l >= H.nlev || H.ktab[l].empty()) return -1;
+    const HostLevel &L = H.L[l];
+    const int64_t n3 = 3 * L.nn;
+    std::vector<int32_t> map(n3, -1);
+    freedof.clear();
+    for (int z = 0; z < L.nz; z++)
+        for (int y = 0; y < L.ny; y++)
+            for (int x = 0; x < L.nx; x++) {
+                const int id = node_id(L, x, y, z);
+                if (id < 0 || node_fixed(L, x, y, z)) continue;
+                for (int c = 0; c < 3; c++) {
+                    map[3 * id + c] = (int32_t)freedof.size();
+                    freedof.push_back(3 * id + c);
+                }
+            }
+    const int64_t nf = (int64_t)freedof.size();
+    A.assign((size_t)nf * nf, 0.0);
+    const std::vector<double> &T = H.ktab[l];
+    for (int ez = 0; ez < L.ez; ez++)
+        for (int ey = 0; ey < L.ey; ey++)
+            for (int ex = 0; ex < L.ex; ex++) {
+                const int p = L.epat[((size_t)ez * L.ey + ey) * L.ex + ex];
+                if (!p) continue;
+                int dof[24];
+                for (int n = 0; n < 8; n++) {
+                    const int id = node_id(L, ex + (n & 1), ey + ((n >> 1) & 1), ez + (n >> 2));
+                    for (int c = 0; c < 3; c++) dof[3 * n + c] = id < 0 ? -1 : map[3 * id + c];
+                }
+                const double *K = &T[(size_t)p * 576];
+                for (int i = 0; i < 24; i++) {
+                    if (dof[i] < 0) continue;
+                    for (int j = 0; j < 24; j++)
+                        if (dof[j] >= 0) A[(size_t)dof[i] * nf + dof[j]] += K[i * 24 + j];
+                }
+            }
+    return 0;
+}
+
 void galerkin_element(const double *ke, int div, uint64_t mask, double *K)
 {
     int nch = div * div * div;

@@ -178,6 +178,8 @@ def patch_load_cases(voxels, nrhs, seed, patch=8, total_force=1.0):
     nzn, nyn, nxn = ids.shape
     used = ids >= 0
     has = used.any(axis=0)
+    if not has.any():
+        raise ValueError("the voxel model has no solid voxel")
     ztop = nzn - 1 - np.argmax(used[::-1], axis=0)
     ndof = 3 * int(ids.max() + 1)
     F = np.zeros((nrhs, ndof))
@@ -202,6 +204,8 @@ def patch_load_cases(voxels, nrhs, seed, patch=8, total_force=1.0):
 def batched_problem(n, count, amin, amax, seed, nrhs=16, rhs_seed=None):
     """Ellipsoid geometry with the bottom fixed and `nrhs` patch load cases."""
     vox, info = ellipsoid_union(n, count, amin, amax, seed)
+    if vox.sum() == 0:
+        raise ValueError("no solid component touches z = 0")
     ids = compact_ids(vox)
     fixed = np.sort(ids[0][ids[0] >= 0])
     F = patch_load_cases(vox, nrhs, seed if rhs_seed is None else rhs_seed)
