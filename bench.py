@@ -47,6 +47,49 @@ def measured_traffic(kernel, workload):
         return None
 
 
+def learned_solve(A, f, nlev, level=3):
+    """FP64 damped-cycle solve with the trained U-Net as the level-3 coarse
+    correction (falls back to the plain cycle when a polling interval stalls)."""
+    path = os.path.join(ROOT, "vulkanfem_amd", "data", "unet_level3.npy")
+    if not os.path.exists(path) or nlev <= level + 1:
+        return {}
+    net = V.UNet(0)
+    net.set_weights(np.load(path))
+    A.attach_unet(net, level)
+    A.set_rhs(f)
+    kw = dict(tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.6, learned=1)
+    A.solve_resident(**kw)
+    st = A.solve_resident(**kw)
+    A.attach_unet(None, 0)
+    log("learned", st)
+    return {"learned_fp64_time_s": round(st["solve_ms"] / 1e3, 5), "learned_fp64_iterations": st["iterations"],
+            "learned_iterations_with_network": st["learned_iters"], "learned_fell_back": st["fell_back"],
+            "learned_rel_residual": st["rel_residual"]}
+
+
+def batched_solves():
+    """batched128x16: 16 patch load cases on one 128^3 geometry as one block
+    (tile shape from the autotuner), FP32 cycle and FP64 cycle, FP64 block CG."""
+    p = problems.workload("batched128x16")
+    A = V.assemble_system_matrix(p.voxels, problems.ke0(1.0, 0.3), p.fixed, problems.WORKLOAD_LEVELS["batched128x16"])
+    best, table = A.autotune_block(nrhs=p.loads.shape[0], passes=6, reps=3)
+    out = {"batched_workload": p.name, "batched_nrhs": int(p.loads.shape[0]),
+           "batched_autotuned_tile": list(table[best][:3]), "batched_tile_ms": round(table[best][3], 4)}
+    A.set_rhs_block(p.loads)
+    for name, prec in (("fp64", V.FP64), ("fp32", V.FP32)):
+        kw = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=0.6)
+        A.solve_resident_block(**kw)
+        t0 = time.perf_counter()
+        st = A.solve_resident_block(**kw)
+        t = time.perf_counter() - t0
+        its = [s["iterations"] for s in st]
+        log(f"batched {name}: {t:.3f} s, iterations {its}")
+        out.update({f"batched_{name}_time_s": round(t, 4), f"batched_{name}_max_iterations": max(its),
+                    f"batched_{name}_all_converged": all(s["converged"] == 1 for s in st),
+                    f"batched_{name}_gdof_iter_per_s": round(A.n * sum(its) / t / 1e9, 2)})
+    return out
+
+
 def cpu_baseline(p, nlev, iterations, sample_iters, threads):
     """The oracle (C restatement of the reference CPU path, additive cycle, FP64)
     on the host cores: setup plus `sample_iters` PCG iterations, scaled to the
@@ -79,6 +122,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction solve")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 16-RHS batched128x16 block solve")
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only launch the level-0 matvec variants and the copy reference this many times each "
                          "(for rocprofv3 counter runs), then exit")
@@ -134,6 +179,13 @@ def main():
         err = float(np.linalg.norm(u_mixed - u_ref) / np.linalg.norm(u_ref))
         # the reference's own additive cycle in FP64 (iteration count for the CPU baseline)
         st_ref = A.solve_resident(tol=1e-8, precision=V.FP64, cycle=V.CYCLE_REFERENCE)
+        # FP64 vectors with the level-0 matvecs on the matrix cores (VFEM_MIXED)
+        mx = dict(tol=1e-8, precision=V.MIXED, cycle=V.CYCLE_DAMPED, omega=0.6)
+        A.solve_resident(**mx)
+        stmx = A.solve_resident(**mx)
+        log("mixed-fp64-vectors", stmx)
+        out_extra.update({"mixed_fp64_vectors_time_s": round(stmx["solve_ms"] / 1e3, 5),
+                          "mixed_fp64_vectors_iterations": stmx["iterations"]})
         out_extra.update({"fp64_time_s": round(t_fp64, 5), "fp64_iterations": st64["iterations"],
                           "speedup_vs_fp64": round(t_fp64 / t_mixed, 3), "disp_rel_err_vs_fp64": err,
                           "ref_cycle_fp64_iterations": st_ref["iterations"],
@@ -156,6 +208,10 @@ def main():
                       "rel_residual": stats[-1]["rel_residual"], "gdof_iter_per_s":
                           round(A.n * stats[-1]["iterations"] / t_mixed / 1e9, 2),
                       "setup_s": round(t_setup, 3), "ndof": A.n})
+    if not args.no_learned:
+        out_extra.update(learned_solve(A, p.loads[0], nlev))
+    if not args.no_batched:
+        out_extra.update(batched_solves())
     kernel = "k_matvec_mfma<6,float,float>"
     traffic = measured_traffic(kernel, p.name)
     cpu = None

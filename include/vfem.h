@@ -191,8 +191,9 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
 /* Kernel timing hook (no reference counterpart): runs `reps` launches of one
  * kernel on the library's stream between two hipEvents and returns the mean
  * duration.  which: 0 = level-0 matvec of the CG operator (FP64: VALU kernel;
- * FP32: matrix cores, three-way BF16 split, 6 products; BF16: one BF16 product),
- * 1 = one V-cycle, 2 = one CG iteration, 3 = level-0 matvec with the two-way
+ * FP32: matrix cores, three-way BF16 split, 6 products; BF16: one BF16 product;
+ * MIXED: matrix cores on FP64 input), 1 = one damped V(1,1) cycle (omega 0.6),
+ * 2 = one CG iteration with that cycle, 3 = level-0 matvec with the two-way
  * BF16 split (3 products), 4 = plain copy of one FP32 level-0 vector (12 bytes
  * read and 12 written per node: the reference for HBM byte counters). */
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);

@@ -3060,6 +3060,7 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     vfem_options o;
     vfem_default_options(&o);
     o.precision = precision;
+    o.cycle = VFEM_CYCLE_DAMPED;
     Plan P;
     if (int rc = check_plan(sys, o, P)) return rc;
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1 << 30, 0.0, 1, 1);
