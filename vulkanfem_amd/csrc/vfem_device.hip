@@ -427,13 +427,20 @@ constexpr int YS_STRIDE = 25;          // padded row of an element's 24 results
 constexpr int MROWS_PER_WAVE = (MN_Y + kWaves - 1) / kWaves;  // 3
 constexpr int MOWN = (MT_X * MT_Y + kBlock - 1) / kBlock;      // 2 output nodes per thread
 
+constexpr int MSTEPS = ((MZ + 1) + 2) / 3 * 3;  // layer steps per tile, rounded to the 3-step unroll
+
+// Element schedule of the level-0 tiles: for tile t and layer step s, the
+// present element slots (eyl << 6 | exl) are list[off[t * MSTEPS + s] ..
+// off[t * MSTEPS + s + 1]), so the matrix cores only see solid elements.
 struct MfmaTiles {
     int tx, ty, tz;  // tile counts in x, y, z
+    const uint32_t *off;
+    const uint16_t *list;
 };
 
 inline MfmaTiles mfma_tiles(const DevLevel &L)
 {
-    return {(L.nx + MT_X - 1) / MT_X, (L.ny + MT_Y - 1) / MT_Y, (L.nz + MZ - 1) / MZ};
+    return {(L.nx + MT_X - 1) / MT_X, (L.ny + MT_Y - 1) / MT_Y, (L.nz + MZ - 1) / MZ, nullptr, nullptr};
 }
 
 // compact id of node (gx, gy, gz) where gx = x_first + lane, from the row's bit
@@ -511,6 +518,9 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     // product's error scales with the local differences, not with |x|
     using XT = typename std::conditional<sizeof(Tx) == 8, double, float>::type;
     __shared__ XT Xs[2][MN_Y][64][3];
+    constexpr int MSLOTS = ME_Y * (ME_X - 1);      // element slots per layer (504)
+    __shared__ uint32_t soffs[MSTEPS + 1];          // schedule offsets of this tile's steps
+    __shared__ uint16_t slist[2][MSLOTS];           // element lists of the current and the next step
     __shared__ float Ys[ME_Y * ME_X * YS_STRIDE];
     __shared__ MfmaMasks K;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -521,6 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     const int tz = b / T.ty;
     const int x0 = tx * MT_X, y0 = ty * MT_Y, zb = tz * MZ;
     const int ze = min(zb + MZ, L.nz);
+    const uint32_t *soff = T.off + (int64_t)((tz * T.ty + ty) * T.tx + tx) * MSTEPS;
     const int ew0 = (x0 - 1) >> 6;  // element word of the tile's first element column (-1 when x0 = 0)
     const int nw0 = x0 >> 6;
     const int rs0 = (x0 - 1) >> 6;  // node word of the staged rows' first node (-1 when x0 = 0)
@@ -555,6 +566,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         K.fw[pl][r][wsel] = ok ? fv : 0;
         K.nb[pl][r][wsel] = ok ? bv : 0;
     }
+
+    if ((int)threadIdx.x <= MSTEPS) soffs[threadIdx.x] = soff[threadIdx.x];
+    __syncthreads();
+    for (int q = threadIdx.x; q < (int)(soffs[1] - soffs[0]); q += kBlock) slist[0][q] = T.list[soffs[0] + q];
 
     // A operand: Ke rows 16*t + (lane % 16).  The K index is ordered by node pairs:
     // k = 8 g + 3 m + c holds DOF c of local node 2 g + m (m = 0, 1), k % 8 = 6, 7
@@ -643,10 +658,17 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         const int bl = ez & 1, bu = (ez + 1) & 1;
         const int ly = ez - (zb - 1);  // local element layer / staged plane of ez
         const bool live = ez < ze;
-        bool layer = false;
-#pragma unroll
-        for (int r = 0; r < ME_Y; r++) layer |= (K.ew[min(ly, MZ)][r][0] | K.ew[min(ly, MZ)][r][1]) != 0;
-        layer = layer && live;
+        const int sidx = min(ez - (zb - 1), MSTEPS - 1);
+        const int cnt = live ? (int)(soffs[sidx + 1] - soffs[sidx]) : 0;
+        const bool layer = cnt > 0;
+        const uint16_t *lst = slist[sidx & 1];
+        // the next step's element list, loaded now and stored in LDS at the end of the step
+        const int sn = min(sidx + 1, MSTEPS - 1);
+        const uint32_t on = soffs[sn];
+        const int cn = (int)(soffs[sn + 1] - on);
+        uint16_t nl0 = 0, nl1 = 0;
+        if ((int)threadIdx.x < cn) nl0 = T.list[on + threadIdx.x];
+        if ((int)threadIdx.x + kBlock < cn) nl1 = T.list[on + threadIdx.x + kBlock];
         // centre values of the owned nodes on plane ez (for the fused x.Ax)
 #pragma unroll
         for (int k = 0; k < MOWN; k++) {
@@ -659,28 +681,27 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             }
         }
         if (layer) {
-            // wave w: element rows 0..ME_Y-1, element columns 16 w .. 16 w + 15.  Lane
-            // (j, g) builds DOFs of local nodes 2g, 2g+1 of element j: plane g >> 1,
-            // row offset g & 1, the two x-adjacent nodes cx, cx + 1.
+            // the layer's present elements in groups of 16 (the MFMA N dimension); wave w
+            // takes groups w, w + 4, ...  Lane (j, g) builds DOFs of local nodes 2g, 2g+1
+            // of element j of the group: plane g >> 1, row offset g & 1, the two
+            // x-adjacent nodes.
             const int j = lane & 15, g = lane >> 4;
-            const int exl = 16 * wave + j;
-            const int ex = x0 - 1 + exl;
-            const int ewl = (ex >> 6) - ew0, ebit = ex & 63;
-            const int cx = min(exl, ME_X - 2);
-            const XT *pb = &Xs[(g >> 1) ? bu : bl][g & 1][cx][0];  // node pair of row eyl = 0
-            const XT *pc = &Xs[bl][0][cx][0];                      // centre (local node 0)
-            constexpr int RSTR = 64 * 3;                              // floats per staged row
-            for (int eyl = 0; eyl < ME_Y; eyl++) {
-                const bool pres = exl < ME_X - 1 && ex >= 0 && ((K.ew[ly][eyl][ewl] >> ebit) & 1);
-                const XT *q = pb + eyl * RSTR, *qc = pc + eyl * RSTR;
+            constexpr int RSTR = 64 * 3;  // floats per staged row
+            for (int grp = wave; grp * 16 < cnt; grp += kWaves) {
+                const int q = grp * 16 + j;
+                const bool pres = q < cnt;
+                const int e = lst[min(q, cnt - 1)];
+                const int exl = e & 63, eyl = e >> 6;
+                const XT *qp = &Xs[(g >> 1) ? bu : bl][(g & 1) + eyl][exl][0];
+                const XT *qc = &Xs[bl][eyl][exl][0];
                 const XT c0 = qc[0], c1 = qc[1], c2 = qc[2];
                 float vv[8];
-                vv[0] = (float)(q[0] - c0);
-                vv[1] = (float)(q[1] - c1);
-                vv[2] = (float)(q[2] - c2);
-                vv[3] = (float)(q[3] - c0);
-                vv[4] = (float)(q[4] - c1);
-                vv[5] = (float)(q[5] - c2);
+                vv[0] = (float)(qp[0] - c0);
+                vv[1] = (float)(qp[1] - c1);
+                vv[2] = (float)(qp[2] - c2);
+                vv[3] = (float)(qp[3] - c0);
+                vv[4] = (float)(qp[4] - c1);
+                vv[5] = (float)(qp[5] - c2);
                 vv[6] = 0.0f;
                 vv[7] = 0.0f;
                 bf16x8 b1, b2, b3;
@@ -693,6 +714,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                     if (PARTS > 1) b2[t] = m;
                     if (PARTS > 2) b3[t] = l;
                 }
+                (void)RSTR;
 #pragma unroll
                 for (int t = 0; t < 2; t++) {
                     f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -708,10 +730,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                     }
                     d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b1, d, 0, 0, 0);
                     // lane (j, g) holds rows 16 t + 4 g + q of element j
+                    if (pres && 16 * t + 4 * g < 24) {
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int row = 16 * t + 4 * g + q;
-                        if (row < 24) Ys[(eyl * ME_X + exl) * YS_STRIDE + row] = d[q];
+                        for (int qq = 0; qq < 4; qq++)
+                            Ys[(eyl * ME_X + exl) * YS_STRIDE + 16 * t + 4 * g + qq] = d[qq];
                     }
                 }
             }
@@ -726,15 +748,27 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             const int a = own ? n % MT_X : 0, bb = own ? n / MT_X : 0;
             float lo[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
             if (layer && own) {
+                // present slots of element rows bb, bb+1 (slot exl <-> element x0 - 1 + exl):
+                // only elements of the schedule wrote their results
+                const int lyc = min(ly, MZ);
+                const int sh = (x0 - 1) & 63;
+                uint64_t bm[2];
+#pragma unroll
+                for (int ey = 0; ey < 2; ey++) {
+                    const uint64_t w0 = K.ew[lyc][bb + ey][0], w1 = K.ew[lyc][bb + ey][1];
+                    bm[ey] = sh ? ((w0 >> sh) | (w1 << (64 - sh))) : w0;
+                }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     const int ex = e & 1, ey = e >> 1;
-                    const float *Yp = &Ys[((bb + ey) * ME_X + (a + ex)) * YS_STRIDE];
-                    const int Ll = (1 - ex) + 2 * (1 - ey);
+                    if ((bm[ey] >> (a + ex)) & 1) {
+                        const float *Yp = &Ys[((bb + ey) * ME_X + (a + ex)) * YS_STRIDE];
+                        const int Ll = (1 - ex) + 2 * (1 - ey);
 #pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        lo[c] += Yp[3 * Ll + c];
-                        up[c] += Yp[3 * (Ll + 4) + c];
+                        for (int c = 0; c < 3; c++) {
+                            lo[c] += Yp[3 * Ll + c];
+                            up[c] += Yp[3 * (Ll + 4) + c];
+                        }
                     }
                 }
             }
@@ -759,6 +793,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         // plane ez+2 (prefetched) replaces plane ez in LDS; refill the set with plane ez+2+PRE
         store(bl, slot, sv);
         fetch(ez + 2 + PRE, slot, sv);
+        if (sn != sidx) {
+            if ((int)threadIdx.x < cn) slist[sn & 1][threadIdx.x] = nl0;
+            if ((int)threadIdx.x + kBlock < cn) slist[sn & 1][threadIdx.x + kBlock] = nl1;
+        }
         lds_barrier();
     };
 
@@ -1786,6 +1824,7 @@ struct vfem_system {
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
     int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
     int block_cfg = 0;          // kBlockCfgs entry of the multi-RHS level-0 matvec (set by the autotuner)
+    MfmaTiles tiles0{};         // level-0 tiles of the matrix-core matvec with their element schedule
     bool l0_mfma = false;       // FP64 level-0 matvecs on the matrix cores (set while enqueueing a VFEM_MIXED plan)
     vfem_unet *unet = nullptr;  // learned coarse correction (not owned)
     int unet_level = -1;
@@ -1830,6 +1869,50 @@ template <typename T> std::vector<T> convert(const std::vector<double> &v)
     std::vector<T> o(v.size());
     for (size_t i = 0; i < v.size(); i++) o[i] = (T)v[i];
     return o;
+}
+
+// Element schedule of the level-0 matrix-core matvec (see MfmaTiles): the present
+// elements of every tile and layer step, in slot order.
+int build_schedule(vfem_system *S)
+{
+    const HostLevel &L = S->H.L[0];
+    MfmaTiles T = mfma_tiles(S->dl[0]);
+    const int64_t ntiles = (int64_t)T.tx * T.ty * T.tz;
+    std::vector<uint32_t> off(ntiles * MSTEPS + 1, 0);
+    std::vector<uint16_t> list;
+    list.reserve((size_t)L.ne * 5 / 4 + 64);
+    auto present = [&](int ex, int ey, int ez) {
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return false;
+        return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) != 0;
+    };
+    for (int tz = 0; tz < T.tz; tz++)
+        for (int ty = 0; ty < T.ty; ty++)
+            for (int tx = 0; tx < T.tx; tx++) {
+                const int64_t t = ((int64_t)tz * T.ty + ty) * T.tx + tx;
+                const int x0 = tx * MT_X, y0 = ty * MT_Y, zb = tz * MZ, ze = std::min(zb + MZ, L.nz);
+                for (int st = 0; st < MSTEPS; st++) {
+                    off[t * MSTEPS + st] = (uint32_t)list.size();
+                    const int ez = zb - 1 + st;
+                    if (ez >= ze) continue;
+                    for (int eyl = 0; eyl < ME_Y; eyl++)
+                        for (int exl = 0; exl < ME_X - 1; exl++)
+                            if (present(x0 - 1 + exl, y0 - 1 + eyl, ez)) list.push_back((uint16_t)(eyl << 6 | exl));
+                }
+            }
+    off[ntiles * MSTEPS] = (uint32_t)list.size();
+    if (list.size() >= (size_t)UINT32_MAX) {
+        g_last_error = "element schedule too large";
+        return VFEM_ERR_ARGUMENT;
+    }
+    uint32_t *doff;
+    uint16_t *dlist;
+    if (int rc = S->upload(&doff, off)) return rc;
+    if (list.empty()) list.push_back(0);
+    if (int rc = S->upload(&dlist, list)) return rc;
+    T.off = doff;
+    T.list = dlist;
+    S->tiles0 = T;
+    return 0;
 }
 
 // Solver vectors for `nb` right-hand sides, column-major (column c of a level
@@ -1935,6 +2018,7 @@ int upload_system(vfem_system *S)
     std::vector<double> ke(H.ke, H.ke + 576);
     if (int rc = S->upload(&S->ke64, ke)) return rc;
     if (int rc = S->upload(&S->ke32, convert<float>(ke))) return rc;
+    if (int rc = build_schedule(S)) return rc;
     if (int rc = S->upload(&S->kinv_ref64, H.kcinv_ref)) return rc;
     if (int rc = S->upload(&S->kinv_gal64, H.kcinv_gal)) return rc;
     if (int rc = S->upload(&S->kinv_ref32, convert<float>(H.kcinv_ref))) return rc;
@@ -2009,7 +2093,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
     const DevLevel &L = S->dl[0];
     if constexpr (sizeof(T) == 8) {
         if (S->l0_mfma && !exact) {
-            const MfmaTiles t = mfma_tiles(L);
+            const MfmaTiles t = S->tiles0;
             const int grid = t.tx * t.ty * t.tz;
             if (passes == 1)
                 hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
@@ -2029,7 +2113,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
             return;
         }
         if (S->use_mfma) {
-            const MfmaTiles t = mfma_tiles(L);
+            const MfmaTiles t = S->tiles0;
             const int grid = t.tx * t.ty * t.tz;
             if (passes == 1)
                 hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
