@@ -1756,10 +1756,14 @@ template <typename T> struct DBuf {
     size_t n = 0;
 };
 
+// Row-segment kernels: one segment per wavefront where the level is small (the
+// coarse levels are latency-bound: a longer serial loop per wave costs more than
+// extra waves), at most kMaxPartials blocks (the reduction partials).
 int grid_for_items(int64_t W)
 {
-    int64_t g = (W + 4 * kWaves - 1) / (4 * kWaves);
-    g = std::max<int64_t>(8, std::min<int64_t>(4096, g));
+    const int64_t per_wave = W > 16384 ? 4 : 1;
+    int64_t g = (W + kWaves * per_wave - 1) / (kWaves * per_wave);
+    g = std::max<int64_t>(8, std::min<int64_t>(kMaxPartials, g));
     return (int)((g + 7) / 8 * 8);
 }
 
