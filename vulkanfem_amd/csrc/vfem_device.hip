@@ -416,6 +416,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
 // ---------------------------------------------------------------------------
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) double f64x4;
 
 constexpr int MT_X = 62;               // output nodes per tile row
 constexpr int MT_Y = 7;                // output node rows per tile
@@ -499,6 +500,7 @@ struct MfmaMasks {
 
 template <int PASSES, typename Tx, typename Ty>
 __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float *__restrict__ ke,
+                                                         const double *__restrict__ ke64,
                                                          const Tx *__restrict__ x, Ty *__restrict__ y, CGState *st,
                                                          Reduction red, int fin, int honor_done, MfmaTiles T)
 {
@@ -512,7 +514,11 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     if (honor_done && st->done) return;
     // PASSES 1: hi*hi.  3: two-way split, hi*hi + hi*lo + lo*hi.  6: three-way split,
     // all products of combined order < 3 (about FP32 accuracy).
+    // PASSES 64: exact FP64 products on v_mfma_f64_16x16x4_f64 (FP64 vectors only).
+    constexpr bool F64 = PASSES == 64;
     constexpr int PARTS = PASSES == 1 ? 1 : (PASSES == 3 ? 2 : 3);
+    static_assert(!F64 || sizeof(Tx) == 8, "FP64 products need FP64 input");
+    using YT = typename std::conditional<F64, double, float>::type;
     // staged node values keep the input precision: with FP64 input the element
     // differences x - x_centre are formed in FP64 before the BF16 split, so the
     // product's error scales with the local differences, not with |x|
@@ -521,7 +527,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     constexpr int MSLOTS = ME_Y * (ME_X - 1);      // element slots per layer (504)
     __shared__ uint32_t soffs[MSTEPS + 1];          // schedule offsets of this tile's steps
     __shared__ uint16_t slist[2][MSLOTS];           // element lists of the current and the next step
-    __shared__ float Ys[ME_Y * ME_X * YS_STRIDE];
+    __shared__ YT Ys[ME_Y * ME_X * YS_STRIDE];
     __shared__ MfmaMasks K;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     int b = remap_block();
@@ -576,6 +582,17 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
     // are zero padding -- so a B lane group g reads two x-adjacent nodes of one
     // plane row.  Split in three.
     bf16x8 a1[2], a2[2], a3[2];
+    // FP64: A[i][k] for K step s in lane i + 16 (k - 4 s), natural DOF order k = 3 L + c
+    double a64[2][6];
+    if constexpr (F64) {
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int s6 = 0; s6 < 6; s6++) {
+                const int row = 16 * t + (lane & 15), col = 4 * s6 + (lane >> 4);
+                a64[t][s6] = row < 24 ? ke64[row * 24 + col] : 0.0;
+            }
+    }
 #pragma unroll
     for (int t = 0; t < 2; t++) {
 #pragma unroll
@@ -630,10 +647,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         }
     };
 
-    float acc[MOWN][3];
+    YT acc[MOWN][3];
     XT xc[MOWN][3];
 #pragma unroll
-    for (int k = 0; k < MOWN; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0f;
+    for (int k = 0; k < MOWN; k++) acc[k][0] = acc[k][1] = acc[k][2] = YT(0);
     double dacc = 0;
 
     // prefetch ring of PRE planes in three fixed register sets (no register moves:
@@ -692,6 +709,26 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                 const bool pres = q < cnt;
                 const int e = lst[min(q, cnt - 1)];
                 const int exl = e & 63, eyl = e >> 6;
+                if constexpr (F64) {
+                    // B[k][j] = x of DOF k of element j, lane j + 16 (k - 4 s)
+                    f64x4 d[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+                    for (int s6 = 0; s6 < 6; s6++) {
+                        const int k = 4 * s6 + g, Ln = k / 3, c = k % 3;
+                        const double bv = Xs[(Ln >> 2) ? bu : bl][eyl + ((Ln >> 1) & 1)][exl + (Ln & 1)][c];
+                        const double bk = pres ? bv : 0.0;
+                        d[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[0][s6], bk, d[0], 0, 0, 0);
+                        d[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[1][s6], bk, d[1], 0, 0, 0);
+                    }
+                    // the FP64 product's lane (j, g) holds rows 16 t + g + 4 qq (register qq)
+#pragma unroll
+                    for (int t = 0; t < 2; t++)
+#pragma unroll
+                        for (int qq = 0; qq < 4; qq++)
+                            if (pres && 16 * t + g + 4 * qq < 24)
+                                Ys[(eyl * ME_X + exl) * YS_STRIDE + 16 * t + g + 4 * qq] = d[t][qq];
+                    continue;
+                }
                 const XT *qp = &Xs[(g >> 1) ? bu : bl][(g & 1) + eyl][exl][0];
                 const XT *qc = &Xs[bl][eyl][exl][0];
                 const XT c0 = qc[0], c1 = qc[1], c2 = qc[2];
@@ -718,6 +755,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
 #pragma unroll
                 for (int t = 0; t < 2; t++) {
                     f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if constexpr (F64) break;
                     // small terms first
                     if (PASSES == 6) {
                         d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], b3, d, 0, 0, 0);
@@ -746,7 +784,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
             const int n = threadIdx.x + kBlock * k;
             const bool own = n < MT_X * MT_Y;
             const int a = own ? n % MT_X : 0, bb = own ? n / MT_X : 0;
-            float lo[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 0.0f, 0.0f};
+            YT lo[3] = {YT(0), YT(0), YT(0)}, up[3] = {YT(0), YT(0), YT(0)};
             if (layer && own) {
                 // present slots of element rows bb, bb+1 (slot exl <-> element x0 - 1 + exl):
                 // only elements of the schedule wrote their results
@@ -762,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                 for (int e = 0; e < 4; e++) {
                     const int ex = e & 1, ey = e >> 1;
                     if ((bm[ey] >> (a + ex)) & 1) {
-                        const float *Yp = &Ys[((bb + ey) * ME_X + (a + ex)) * YS_STRIDE];
+                        const YT *Yp = &Ys[((bb + ey) * ME_X + (a + ex)) * YS_STRIDE];
                         const int Ll = (1 - ex) + 2 * (1 - ey);
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
@@ -781,7 +819,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
                     const bool fx = (K.fw[pz][bb][wl] >> bit) & 1;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        const float o = fx ? 0.0f : acc[k][c] + lo[c];
+                        const YT o = fx ? YT(0) : acc[k][c] + lo[c];
                         y[3 * id + c] = (Ty)o;
                         dacc += (double)xc[k][c] * (double)o;
                     }
@@ -1826,6 +1864,7 @@ struct vfem_system {
     std::map<int64_t, hipGraphExec_t> graphs;
     double setup_ms = 0;
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
+    bool use_mfma64 = true;     // FP64 level-0 matvec on the matrix cores (exact FP64 products)
     int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
     int block_cfg = 0;          // kBlockCfgs entry of the multi-RHS level-0 matvec (set by the autotuner)
     MfmaTiles tiles0{};         // level-0 tiles of the matrix-core matvec with their element schedule
@@ -2089,25 +2128,33 @@ void launch_matvec_block(vfem_system *S, const float *x, float *y, int fin, int 
     else launch_block_passes<6>(S, x, y, fin, honor_done, c);
 }
 
-// level-0 matvec.  FP64 vectors: the VALU kernel in FP64 arithmetic, or (VFEM_MIXED
-// plans, unless `exact`) the matrix-core kernel on FP64 centre differences.
+// level-0 matvec.  FP64 vectors: exact FP64 products on the matrix cores
+// (v_mfma_f64_16x16x4_f64; the VALU kernel when use_mfma64 is off), or (VFEM_MIXED
+// plans, unless `exact`) BF16 products of FP64 centre differences.
 template <typename T>
 void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done, int passes = 6, bool exact = false)
 {
     const DevLevel &L = S->dl[0];
     if constexpr (sizeof(T) == 8) {
+        if ((!S->l0_mfma || exact) && S->use_mfma64) {
+            const MfmaTiles t = S->tiles0;
+            const int grid = t.tx * t.ty * t.tz;
+            hipLaunchKernelGGL((k_matvec_mfma<64, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
+            return;
+        }
         if (S->l0_mfma && !exact) {
             const MfmaTiles t = S->tiles0;
             const int grid = t.tx * t.ty * t.tz;
             if (passes == 1)
                 hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
-                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+                                   S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
             else if (passes == 3)
                 hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
-                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+                                   S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
             else
                 hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
-                                   S->ke32, x, y, S->st, S->red, fin, honor_done, t);
+                                   S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
             return;
         }
     }
@@ -2120,13 +2167,13 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
             const MfmaTiles t = S->tiles0;
             const int grid = t.tx * t.ty * t.tz;
             if (passes == 1)
-                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<1, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, S->ke64, x,
                                    y, S->st, S->red, fin, honor_done, t);
             else if (passes == 3)
-                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<3, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, S->ke64, x,
                                    y, S->st, S->red, fin, honor_done, t);
             else
-                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, x,
+                hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, S->ke64, x,
                                    y, S->st, S->red, fin, honor_done, t);
             return;
         }
