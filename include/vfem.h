@@ -195,7 +195,8 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
  * MIXED: matrix cores on FP64 input), 1 = one damped V(1,1) cycle (omega 0.6),
  * 2 = one CG iteration with that cycle, 3 = level-0 matvec with the two-way
  * BF16 split (3 products), 4 = plain copy of one FP32 level-0 vector (12 bytes
- * read and 12 written per node: the reference for HBM byte counters). */
+ * read and 12 written per node: the reference for HBM byte counters), 5 = copy
+ * of one FP64 level-0 vector, 6 = FP64 dot product of two level-0 vectors. */
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);
 
 /* Host-only inspection of the hierarchy vfem_assemble builds (no device call; no

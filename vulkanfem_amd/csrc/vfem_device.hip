@@ -3,7 +3,7 @@
 //
 // Reference kernels replaced (shaders/*.comp of hirademirkol/VulkanFEM):
 //   MatxVec.comp + FixedNodes.comp      -> k_matvec (node-centric, atomic-free, fixed rows fused)
-//   VecDotVec.comp                      -> block_reduce_finalize (single pass, deterministic)
+//   VecDotVec.comp                      -> block_partials + k_finalize (deterministic)
 //   ConjugateGradient_1.comp            -> k_cg_update (x, r update fused with r.r)
 //   ConjugateGradient_2.comp            -> k_cg_pupdate
 //   CWiseMult.comp + Restrict(Double)   -> k_restrict (coarse-node gather; mu cancels, see below)
@@ -172,13 +172,15 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
     }
 }
 
-// Single-pass grid reduction of up to two sums: block partials in fixed order,
-// the last block to arrive sums them in fixed order (deterministic) and runs
-// the finalizer.
-__device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGState *st, int fin)
+// Grid reductions in two launches: every block writes its partial sums (fixed
+// in-block order) and k_finalize, one block per column launched right after on
+// the same stream, adds them in fixed order (deterministic) and runs the
+// finalizer.  A single-pass "last block finalizes" scheme needs an agent-scope
+// release fence per block, which writes back the XCD's L2 and doubled the time of
+// a streaming kernel here; the kernel boundary orders the partials for free.
+__device__ void block_partials2(double v, double v2, Reduction red)
 {
     __shared__ double sh[2][kWaves];
-    __shared__ int am_last;
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     v = wave_sum(v);
     v2 = wave_sum(v2);
@@ -193,23 +195,29 @@ __device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGSta
             s += sh[0][w];
             s2 += sh[1][w];
         }
-        __hip_atomic_store(&red.partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&red.partials[red.cap + blockIdx.x], s2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned t = atomicAdd(red.counter, 1u);
-        am_last = (t == gridDim.x - 1);
+        red.partials[blockIdx.x] = s;
+        red.partials[red.cap + blockIdx.x] = s2;
     }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
+}
+
+__device__ __forceinline__ void block_partials(double v, Reduction red) { block_partials2(v, 0.0, red); }
+
+// sums nblocks partials of column blockIdx.y and runs finalizer `fin` on its state
+// (skipped for a finished column when the producer skips finished columns)
+__global__ __launch_bounds__(kBlock) void k_finalize(Reduction red, int nblocks, CGState *st, int fin, int honor_done)
+{
+    red = col_red(red);
+    st = col_state(st);
+    if (honor_done && st->done) return;
+    __shared__ double sh[2][kWaves];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     double s = 0, s2 = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock) {
-        s += __hip_atomic_load(&red.partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s2 += __hip_atomic_load(&red.partials[red.cap + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) {
+        s += red.partials[i];
+        s2 += red.partials[red.cap + i];
     }
     s = wave_sum(s);
     s2 = wave_sum(s2);
-    __syncthreads();
     if (lane == 0) {
         sh[0][wave] = s;
         sh[1][wave] = s2;
@@ -221,14 +229,8 @@ __device__ void block_reduce_finalize2(double v, double v2, Reduction red, CGSta
             tot += sh[0][w];
             tot2 += sh[1][w];
         }
-        __hip_atomic_store(red.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         finalize(st, fin, tot, tot2);
     }
-}
-
-__device__ __forceinline__ void block_reduce_finalize(double v, Reduction red, CGState *st, int fin)
-{
-    block_reduce_finalize2(v, 0.0, red, st, fin);
 }
 
 // compact id of node (x, y, z) of level L, or -1 when unused / outside
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
             dacc += (double)ctr[0] * (double)acc0 + (double)ctr[1] * (double)acc1 + (double)ctr[2] * (double)acc2;
         }
     }
-    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+    if (fin >= 0) block_partials(dacc, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -844,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_mfma(DevLevel L, const float 
         step(ez + 1, s2, v2);
         step(ez + 2, s0, v0);
     }
-    if (fin >= 0) block_reduce_finalize(dacc, red, st, fin);
+    if (fin >= 0) block_partials(dacc, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -894,7 +896,6 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
     __shared__ int ecount;
     __shared__ unsigned long long epres;
     __shared__ double dsh[TL::NO * BCOLS];
-    __shared__ int am_last;
 
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c0 = blockIdx.y * BCOLS;  // first column of this group
@@ -1109,7 +1110,8 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
     }
     if (fin < 0) return;
 
-    // per-column x.Ax: fixed-order sums over the block's items, then over blocks
+    // per-column x.Ax: fixed-order sums over the block's items into the column's
+    // partials (k_finalize adds the blocks)
 #pragma unroll
     for (int k = 0; k < TL::ITEMS; k++) {
         const int q = threadIdx.x + kBlock * k;
@@ -1120,35 +1122,9 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
         const int j = threadIdx.x;
         double s = 0;
         for (int o = 0; o < TL::NO; o++) s += dsh[o * BCOLS + j];
-        __hip_atomic_store(&red.partials[(size_t)(c0 + j) * 2 * red.cap + blockIdx.x], s, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        red.partials[(size_t)(c0 + j) * 2 * red.cap + blockIdx.x] = s;
+        red.partials[(size_t)(c0 + j) * 2 * red.cap + red.cap + blockIdx.x] = 0.0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned t = atomicAdd(red.counter + c0, 1u);
-        am_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    for (int j = 0; j < nc; j++) {
-        CGState *sj = st + c0 + j;
-        double s = 0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
-            s += __hip_atomic_load(&red.partials[(size_t)(c0 + j) * 2 * red.cap + i], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        s = wave_sum(s);
-        __syncthreads();
-        if (lane == 0) dsh[wave] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double tot = 0;
-            for (int w = 0; w < kWaves; w++) tot += dsh[w];
-            if (!sj->done) finalize(sj, fin, tot, 0.0);
-        }
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(red.counter + c0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Coarse-level operator of the damped cycle: element matrices from a table
@@ -1243,29 +1219,33 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
         const int Y = (int)(row % C.ny), Z = (int)(row / C.ny), X = 64 * s + lane;
         Tc a0 = 0, a1 = 0, a2 = 0;
         if (div == 2) {
-            // fine x = 2X + {-1, 0, +1}; 2X lies in fine word 2s (lanes < 32) or 2s+1
+            // fine x = 2X + {-1, 0, +1}; 2X lies in fine word 2s (lanes < 32) or 2s+1.
+            // All nine fine rows are unrolled with clamped, unconditional loads (rows
+            // outside the grid get weight 0), so their loads are in flight together.
             const int b = (2 * lane) & 63;
             const uint64_t below = (1ull << b) - 1;
+            const int fs0 = 2 * s, fs1 = 2 * s + 1;
+            const int cs0 = min(fs0, F.S - 1), cs1 = min(fs1, F.S - 1), csp = max(fs0 - 1, 0);
+#pragma unroll
             for (int dz = -1; dz <= 1; dz++) {
-                const int fz = 2 * Z + dz;
-                if (fz < 0 || fz >= F.nz) continue;
-                const Tc wz = dz ? Tc(0.5) : Tc(1);
+#pragma unroll
                 for (int dy = -1; dy <= 1; dy++) {
-                    const int fy = 2 * Y + dy;
-                    if (fy < 0 || fy >= F.ny) continue;
-                    const Tc wzy = wz * (dy ? Tc(0.5) : Tc(1));
-                    const int64_t sb = ((int64_t)fz * F.ny + fy) * F.S;
-                    const int fs0 = 2 * s, fs1 = 2 * s + 1;
-                    const uint64_t wA = fs0 < F.S ? F.nbits[sb + fs0] : 0;
-                    const uint64_t wB = fs1 < F.S ? F.nbits[sb + fs1] : 0;
-                    const uint64_t wP = fs0 > 0 ? F.nbits[sb + fs0 - 1] : 0;
-                    const int bA = fs0 < F.S ? F.nbase[sb + fs0] : 0;
-                    const int bB = fs1 < F.S ? F.nbase[sb + fs1] : 0;
+                    const int fz = 2 * Z + dz, fy = 2 * Y + dy;
+                    const bool rowok = fz >= 0 && fz < F.nz && fy >= 0 && fy < F.ny;
+                    const Tc wzy = rowok ? (dz ? Tc(0.5) : Tc(1)) * (dy ? Tc(0.5) : Tc(1)) : Tc(0);
+                    const int64_t sb = rowok ? ((int64_t)fz * F.ny + fy) * F.S : 0;
+                    const uint64_t wA0 = F.nbits[sb + cs0], wB0 = F.nbits[sb + cs1], wP0 = F.nbits[sb + csp];
+                    const int bA0 = F.nbase[sb + cs0], bB0 = F.nbase[sb + cs1];
+                    const uint64_t wA = rowok && fs0 < F.S ? wA0 : 0, wB = rowok && fs1 < F.S ? wB0 : 0;
+                    const uint64_t wP = rowok && fs0 > 0 ? wP0 : 0;
                     const uint64_t w = lane < 32 ? wA : wB;
-                    const int id0 = (lane < 32 ? bA : bB) + __popcll(w & below);
+                    const int id0 = (lane < 32 ? bA0 : bB0) + __popcll(w & below);
                     const bool v0 = (w >> b) & 1, vp = (w >> (b + 1)) & 1;
                     const bool vm = b ? ((w >> (b - 1)) & 1) : (lane < 32 ? (wP >> 63) : (wA >> 63));
-                    const int i0 = v0 ? id0 : 0, ip = vp ? id0 + (v0 ? 1 : 0) : 0, im = vm ? id0 - 1 : 0;
+                    // clamped: lanes past the coarse row end load valid (discarded) entries
+                    const int hi = (int)F.nn - 1;
+                    const int i0 = v0 ? min(id0, hi) : 0, ip = vp ? min(id0 + (v0 ? 1 : 0), hi) : 0;
+                    const int im = vm ? min(max(id0 - 1, 0), hi) : 0;
                     Tc q[3][3];
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
@@ -1357,19 +1337,23 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
         const int rx = x - div * X0, ry = y - div * Y0, rz = z - div * Z0;
         Tz a0 = 0, a1 = 0, a2 = 0;
         if (div == 2) {
-            // coarse x = X0 (and X0+1 when x is odd); X0 lies in coarse word s/2
+            // coarse x = X0 (and X0+1 when x is odd); X0 lies in coarse word s/2.  The
+            // (up to) four coarse rows are unrolled with unconditional loads; rows the
+            // fine node does not use get weight 0.
             const int cs = s >> 1, b = X0 & 63;
             const uint64_t below = (1ull << b) - 1;
-            for (int cz = 0; cz <= rz; cz++) {
-                const Tz wz = rz ? Tz(0.5) : Tz(1);
-                for (int cy = 0; cy <= ry; cy++) {
-                    const Tz wzy = wz * (ry ? Tz(0.5) : Tz(1));
-                    const int64_t seg = ((int64_t)(Z0 + cz) * C.ny + (Y0 + cy)) * C.S + cs;
+#pragma unroll
+            for (int cz = 0; cz < 2; cz++) {
+#pragma unroll
+                for (int cy = 0; cy < 2; cy++) {
+                    const bool use = cz <= rz && cy <= ry;
+                    const Tz wzy = use ? (rz ? Tz(0.5) : Tz(1)) * (ry ? Tz(0.5) : Tz(1)) : Tz(0);
+                    const int64_t seg = ((int64_t)min(Z0 + cz, C.nz - 1) * C.ny + min(Y0 + cy, C.ny - 1)) * C.S + cs;
                     const uint64_t w = C.nbits[seg];
-                    const uint64_t wn = cs + 1 < C.S ? C.nbits[seg + 1] : 0;
+                    const uint64_t wn = C.nbits[cs + 1 < C.S ? seg + 1 : seg];
                     const int id0 = C.nbase[seg] + __popcll(w & below);
                     const bool v0 = (w >> b) & 1;
-                    const bool v1 = rx && (b < 63 ? ((w >> (b + 1)) & 1) : (wn & 1));
+                    const bool v1 = rx && (b < 63 ? ((w >> (b + 1)) & 1) : (cs + 1 < C.S && (wn & 1)));
                     const int i0 = v0 ? id0 : 0, i1 = v1 ? id0 + (v0 ? 1 : 0) : 0;
                     const Tz w0 = v0 ? wzy * (rx ? Tz(0.5) : Tz(1)) : Tz(0), w1 = v1 ? wzy * Tz(0.5) : Tz(0);
                     a0 += w0 * Tz(zc[3 * i0 + 0]) + w1 * Tz(zc[3 * i1 + 0]);
@@ -1417,7 +1401,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
             }
         }
     }
-    if (fin >= 0) block_reduce_finalize2(dacc, dacc2, red, st, fin);
+    if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
 // Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
@@ -1456,7 +1440,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
             const int64_t k = min(k0 + u * kBlock, R.end - 1);
             rv[u] = r[k];
             dv[u] = dinv[k];
-            fw[u] = fixc[(k / 3) >> 5];
+            fw[u] = fixc[((uint32_t)k / 3u) >> 5];  // n3 < 2^32 (grids up to 1022^3)
             if (mode != SMOOTH_INIT) {
                 zv[u] = z[k];
                 tv[u] = t[k];
@@ -1467,7 +1451,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t k = k0 + u * kBlock;
             if (k >= R.end) break;
-            const bool fx = (fw[u] >> ((k / 3) & 31)) & 1;
+            const bool fx = (fw[u] >> (((uint32_t)k / 3u) & 31)) & 1;
             Tz v;
             if (mode == SMOOTH_INIT) v = om * Tz(dv[u]) * Tz(rv[u]);
             else v = zv[u] + om * Tz(dv[u]) * (Tz(rv[u]) - tv[u]);
@@ -1479,7 +1463,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
             }
         }
     }
-    if (fin >= 0) block_reduce_finalize2(dacc, dacc2, red, st, fin);
+    if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
 // Coarsest level: z[freedof[i]] = sum_k Kc^-1[i][k] r[freedof[k]] (replaces the
@@ -1549,7 +1533,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
             acc += rn * rn;
         }
     }
-    block_reduce_finalize(acc, red, st, FIN_RR);
+    block_partials(acc, red);
 }
 
 // ConjugateGradient_2.comp: p = z + beta p
@@ -1610,7 +1594,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__
         if (x) x[i] = 0;
         acc += v * v;
     }
-    block_reduce_finalize(acc, red, st, fin);
+    block_partials(acc, red);
 }
 
 template <typename Ta, typename Tb>
@@ -1640,7 +1624,7 @@ __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const Ta *__restrict_
         for (int u = 0; u < FLAT_U; u++)
             if (i0 + u * kBlock < R.end) acc += (double)av[u] * (double)bv[u];
     }
-    block_reduce_finalize(acc, red, st, fin);
+    block_partials(acc, red);
 }
 
 template <typename Ts, typename Td>
@@ -2074,6 +2058,14 @@ int upload_system(vfem_system *S)
 
 // ---- launch helpers -------------------------------------------------------
 
+// finalizer of a reduction whose producer (nblocks blocks per column) was just
+// enqueued on stream `sm`
+void enqueue_finalize(vfem_system *S, hipStream_t sm, int nblocks, int ncols, int fin, int honor_done)
+{
+    if (fin < 0) return;
+    hipLaunchKernelGGL(k_finalize, dim3(1, ncols), dim3(kBlock), 0, sm, S->red, nblocks, S->st, fin, honor_done);
+}
+
 int items_grid(const DevLevel &L) { return grid_for_items((int64_t)L.nz * L.ny * L.S); }
 
 template <typename T> T *level_r(vfem_system *S, int l);
@@ -2108,6 +2100,7 @@ void launch_block_shape(vfem_system *S, const float *x, float *y, int fin, int h
     const int groups = (S->ncols + BCOLS - 1) / BCOLS;
     hipLaunchKernelGGL((k_matvec_block<P, TEX, TEY>), dim3(G.tx * G.ty * G.tz, groups), dim3(kBlock), 0, S->stream,
                        S->dl[0], S->ke32, x, y, S->st, S->red, fin, honor_done, S->ncols, G);
+    enqueue_finalize(S, S->stream, G.tx * G.ty * G.tz, S->ncols, fin, honor_done);
 }
 
 template <int P>
@@ -2141,6 +2134,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
             const int grid = t.tx * t.ty * t.tz;
             hipLaunchKernelGGL((k_matvec_mfma<64, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
                                S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
+            enqueue_finalize(S, S->stream, grid, S->ncols, fin, honor_done);
             return;
         }
         if (S->l0_mfma && !exact) {
@@ -2155,6 +2149,7 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
             else
                 hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L,
                                    S->ke32, S->ke64, x, y, S->st, S->red, fin, honor_done, t);
+            enqueue_finalize(S, S->stream, grid, S->ncols, fin, honor_done);
             return;
         }
     }
@@ -2175,11 +2170,13 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
             else
                 hipLaunchKernelGGL((k_matvec_mfma<6, T, T>), dim3(grid, S->ncols), dim3(kBlock), 0, S->stream, L, S->ke32, S->ke64, x,
                                    y, S->st, S->red, fin, honor_done, t);
+            enqueue_finalize(S, S->stream, grid, S->ncols, fin, honor_done);
             return;
         }
     }
     hipLaunchKernelGGL(k_matvec_fine<T>, dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, kefine<T>(S), x, y,
                        S->st, S->red, fin, honor_done);
+    enqueue_finalize(S, S->stream, items_grid(L), S->ncols, fin, honor_done);
 }
 
 template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x, T *y)
@@ -2228,6 +2225,7 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
     hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                        (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin,
                        zold);
+    enqueue_finalize(S, S->stream, items_grid(F), S->ncols, fin, 1);
 }
 
 // Damped multiplicative V(1,1) cycle, level l >= 0.  rl: level residual
@@ -2270,6 +2268,7 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
+    enqueue_finalize(S, S->stream, grid_for_flat(n3l), S->ncols, fin, 1);
 }
 
 struct Plan {
@@ -2316,6 +2315,7 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
         launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
         hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(S->n3), S->ncols), dim3(kBlock), 0, S->stream, S->n3, r,
                            S->z64, S->st, S->red, fin, 1);
+        enqueue_finalize(S, S->stream, grid_for_flat(S->n3), S->ncols, fin, 1);
         return;
     }
     if (P.precision == VFEM_FP64) {
@@ -2348,6 +2348,7 @@ void enqueue_true_residual(vfem_system *S)
     launch_matvec_fine<double>(S, S->x, S->t64, -1, 0, 6, true);
     hipLaunchKernelGGL(k_residual, dim3(grid_for_flat(S->n3), S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->f, S->t64,
                        S->r, (double *)nullptr, S->st, S->red, (int)FIN_RR_TRUE);
+    enqueue_finalize(S, S->stream, grid_for_flat(S->n3), S->ncols, FIN_RR_TRUE, 0);
 }
 
 // p = z + beta p after the preconditioner
@@ -2378,10 +2379,12 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes);
         hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p64, S->ap64, S->st, S->red);
+        enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p32, S->ap32, S->st, S->red);
+        enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
     }
     if (with_true) enqueue_true_residual(S);
     enqueue_precond(S, P, S->r, FIN_RZ);
@@ -2488,11 +2491,13 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     // |f|^2 and the threshold; r = f with x = 0, or r = f - A x0
     hipLaunchKernelGGL(k_residual, dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr,
                        S->r, warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
+    enqueue_finalize(S, S->stream, gf, nc, FIN_RHS, 0);
     if (warm)
         enqueue_true_residual(S);
     else  // initial r.r test (Eigen: residualNorm2 < threshold -> 0 iterations)
         hipLaunchKernelGGL((k_dot<double, double>), dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->r, S->r,
                            S->st, S->red, (int)FIN_RR, 0);
+    if (!warm) enqueue_finalize(S, S->stream, gf, nc, FIN_RR, 0);
     HIP_TRY(hipMemsetAsync(S->p64, 0, n3all * sizeof(double), S->stream));
     HIP_TRY(hipMemsetAsync(S->p32, 0, n3all * sizeof(float), S->stream));
     // p = M r, absNew = r.p
@@ -2583,6 +2588,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     launch_matvec_fine<double>(S, S->x, S->t64, -1, 0);
     hipLaunchKernelGGL(k_residual, dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->f, S->t64, S->t64,
                        (double *)nullptr, S->st, S->red, (int)FIN_STORE);
+    enqueue_finalize(S, S->stream, gf, nc, FIN_STORE, 0);
     std::vector<CGState> h2;
     if (int rc = read_states(S, h2)) return rc;
     int worst = VFEM_OK;
@@ -2957,8 +2963,10 @@ int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, 
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, n_iter + 1, 0.0, 0, 1);
     hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->f, (const double *)nullptr,
                        sys->r, sys->x, sys->st, sys->red, (int)FIN_RHS);
+    enqueue_finalize(sys, sys->stream, gf, 1, FIN_RHS, 0);
     hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->r, sys->r, sys->st,
                        sys->red, (int)FIN_RR, 0);
+    enqueue_finalize(sys, sys->stream, gf, 1, FIN_RR, 0);
     HIP_TRY(hipMemsetAsync(sys->p64, 0, n3 * sizeof(double), sys->stream));
     enqueue_precond(sys, P, sys->r, FIN_RZ_INIT);
     enqueue_pupdate(sys, P);
@@ -3180,6 +3188,7 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
     HIP_TRY(hipMemcpy(sys->ap64, b, n3 * sizeof(double), hipMemcpyHostToDevice));
     hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, sys->stream, n3, sys->t64,
                        sys->ap64, sys->st, sys->red, (int)FIN_STORE, 0);
+    enqueue_finalize(sys, sys->stream, grid_for_flat(n3), 1, FIN_STORE, 0);
     CGState h{};
     if (int rc = read_state(sys, h)) return rc;
     *out = h.result;
@@ -3188,7 +3197,7 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
 
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms)
 {
-    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 4) {
+    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 6) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
@@ -3211,6 +3220,13 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
         } else if (which == 4) {
             hipLaunchKernelGGL((k_convert<float, float>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0, sys->stream,
                                sys->n3, sys->p32, sys->ap32);
+        } else if (which == 5) {
+            hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0,
+                               sys->stream, sys->n3, sys->p64, sys->ap64);
+        } else if (which == 6) {
+            hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0, sys->stream,
+                               sys->n3, sys->t64, sys->ap64, sys->st, sys->red, (int)FIN_STORE, 0);
+            enqueue_finalize(sys, sys->stream, grid_for_flat(sys->n3), 1, FIN_STORE, 0);
         } else if (which == 1) {
             enqueue_precond(sys, P, sys->r, -1);
         } else {
