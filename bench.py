@@ -146,8 +146,8 @@ def main():
     A.set_rhs(p.loads[0])
     if args.profile_kernels:
         for which, prec_k, name in ((0, V.FP32, "matvec fp32 6-pass"), (3, V.FP32, "matvec fp32 3-pass"),
-                                    (0, V.BF16, "matvec bf16 1-pass"), (0, V.FP64, "matvec fp64 valu"),
-                                    (4, V.FP32, "copy fp32 vector")):
+                                    (0, V.BF16, "matvec bf16 1-pass"), (0, V.FP64, "matvec fp64 mfma"),
+                                    (4, V.FP32, "copy fp32 vector"), (5, V.FP32, "copy fp64 vector")):
             ms = A.time_kernel(which, prec_k, reps=args.profile_kernels)
             log(f"{name}: {ms * 1e3:.1f} us")
         return
