@@ -54,6 +54,13 @@ enum vfem_precision {
                        true_res_every iterations; flexible CG */
 };
 
+/* form of the level-0 operator (vfem_set_level0_kernel) */
+enum vfem_level0 {
+    VFEM_L0_AUTO = 0,     /* the stencil form when Ke has its structure, else the element form (default) */
+    VFEM_L0_ELEMENT = 1,  /* element form: Ke times each element's 24 values, on the matrix cores */
+    VFEM_L0_STENCIL = 2   /* assembled 27-point stencil plus a correction of the surface nodes */
+};
+
 /* multigrid cycle */
 enum vfem_cycle {
     VFEM_CYCLE_REFERENCE = 0, /* additive cycle of MultigridPreconditioner.hpp:158-193 (omega = 1) */
@@ -155,10 +162,19 @@ int vfem_get_solution(vfem_system *sys, double *u);
 
 /* Replaces the matrix-free product y = A x with fixed rows zeroed
  * (include/MatrixFreeSparse.hpp:191-215; GPU: shaders/MatxVec.comp +
- * shaders/FixedNodes.comp).  precision selects the kernel (vfem_precision):
- * FP64 VALU, FP32 (6 BF16 products), BF16 (1 product), MIXED (6 products on
- * FP64 input differences). */
+ * shaders/FixedNodes.comp).  precision selects the arithmetic (vfem_precision):
+ * FP64, FP32, BF16 (1 BF16 product on the matrix cores), MIXED (6 BF16 products
+ * on FP64 input differences).  FP64 and FP32 use the level-0 form chosen with
+ * vfem_set_level0_kernel. */
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision);
+
+/* Level-0 operator form (enum vfem_level0; no reference counterpart, whose
+ * MatxVec.comp is the element form).  VFEM_L0_STENCIL fails with
+ * VFEM_ERR_UNSUPPORTED when Ke does not assemble to the structured stencil.
+ * vfem_level0_kernel returns the form in effect (VFEM_L0_ELEMENT or
+ * VFEM_L0_STENCIL), or -1 for a null system. */
+int vfem_set_level0_kernel(vfem_system *sys, int32_t kind);
+int32_t vfem_level0_kernel(const vfem_system *sys);
 
 /* Multi-RHS form of vfem_matvec (no reference counterpart): y_k = A x_k for nrhs
  * columns (x, y nrhs x ndof row-major).  FP32/BF16 use the multi-RHS matrix-core

@@ -63,6 +63,49 @@ def test_matvec_parity(systems, name):
     assert emx < 1e-6
 
 
+@pytest.mark.parametrize("name", list(CASES))
+def test_stencil_form_matches_element_form(systems, name):
+    """The assembled-stencil level-0 operator plus its surface correction against
+    the element form and the oracle (FP64 to rounding, FP32 to single precision)."""
+    A, ora, _, _ = systems[name]
+    assert A.level0_kernel == V.L0_STENCIL
+    x = np.random.default_rng(5).standard_normal(A.n)
+    ref = ora.matvec(x)
+    out = {}
+    try:
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+            A.set_level0_kernel(kind)
+            assert A.level0_kernel == kind
+            out[kind] = (A.matvec(x, V.FP64), A.matvec(x, V.FP32))
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
+    s64, s32 = out[V.L0_STENCIL]
+    e64, e32 = out[V.L0_ELEMENT]
+    say("stencil %s: fp64 %.2e (element %.2e), fp32 %.2e (element %.2e)"
+        % (name, rel(s64, ref), rel(e64, ref), rel(s32, ref), rel(e32, ref)))
+    assert rel(s64, ref) < 1e-14
+    assert rel(s64, e64) < 1e-14
+    assert rel(s32, ref) < 2e-6
+
+
+def test_unstructured_ke_uses_element_form(ke0):
+    """A Ke without the stencil structure keeps the element-form kernels, and
+    asking for the stencil form is refused."""
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((24, 24)) * 1e-2
+    ke = ke0 + B @ B.T
+    vox = np.ones((6, 7, 8), np.int32)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    A = V.assemble_system_matrix(vox, ke, fixed, 2)
+    assert A.level0_kernel == V.L0_ELEMENT
+    with pytest.raises(V.VfemError):
+        A.set_level0_kernel(V.L0_STENCIL)
+    x = rng.standard_normal(A.n)
+    ref = O.OracleSystem(vox, ke, fixed, 2).matvec(x)
+    assert rel(A.matvec(x, V.FP64), ref) < 1e-13
+
+
 def test_matvec_fp32_on_smooth_field(systems, goldens):
     # K u = f on the direct solution: extreme cancellation |K||u| >> |K u|
     A, ora, _, _ = systems["box"]
@@ -245,8 +288,12 @@ def test_block_matvec_parity(systems, cfg):
     X = np.random.default_rng(5).standard_normal((20, A.n))
     A.set_block_config(cfg)
     say("block matvec config", cfg, V.api.block_configs()[cfg])
-    Y32 = A.matvec_block(X, V.FP32)
-    Y16 = A.matvec_block(X, V.BF16)
+    A.set_level0_kernel(V.L0_ELEMENT)  # FP32 at 6 products goes to the stencil form otherwise
+    try:
+        Y32 = A.matvec_block(X, V.FP32)
+        Y16 = A.matvec_block(X, V.BF16)
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
     e32 = max(rel(Y32[k], ora.matvec(X[k])) for k in range(20))
     e16 = max(rel(Y16[k], ora.matvec(X[k])) for k in range(20))
     say("block matvec rel err fp32 %.2e bf16 %.2e" % (e32, e16))
@@ -254,6 +301,8 @@ def test_block_matvec_parity(systems, cfg):
     if cfg == 0:
         Y64 = A.matvec_block(X[:3], V.FP64)
         assert max(rel(Y64[k], ora.matvec(X[k])) for k in range(3)) < 1e-13
+        Ys = A.matvec_block(X, V.FP32)  # stencil form, one launch per batch (blockIdx.y = column)
+        assert max(rel(Ys[k], ora.matvec(X[k])) for k in range(20)) < 2e-6
     A.set_block_config(0)
 
 

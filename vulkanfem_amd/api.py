@@ -24,6 +24,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "vfem.h")
 
 FP64, FP32, BF16, MIXED = 0, 1, 2, 3
 CYCLE_REFERENCE, CYCLE_DAMPED = 0, 1
+L0_AUTO, L0_ELEMENT, L0_STENCIL = 0, 1, 2
 
 OK, ERR_ARGUMENT, ERR_EMPTY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_UNSUPPORTED = range(8)
 
@@ -99,6 +100,9 @@ def lib():
     L.vfem_get_solution.argtypes = [vp, _f64p]
     L.vfem_matvec.argtypes = [vp, _f64p, _f64p, ctypes.c_int32]
     L.vfem_matvec_block.argtypes = [vp, _f64p, _f64p, ctypes.c_int32, ctypes.c_int32]
+    L.vfem_set_level0_kernel.argtypes = [vp, ctypes.c_int32]
+    L.vfem_level0_kernel.argtypes = [vp]
+    L.vfem_level0_kernel.restype = ctypes.c_int32
     L.vfem_block_config_count.restype = ctypes.c_int32
     L.vfem_block_config.argtypes = [ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
     L.vfem_set_block_config.argtypes = [vp, ctypes.c_int32]
@@ -351,6 +355,15 @@ class MatrixFreeSparse:
         U = np.zeros((nrhs, self.n))
         _check(self._lib.vfem_get_solution_block(self._h, U.reshape(-1), nrhs))
         return U
+
+    def set_level0_kernel(self, kind):
+        """Level-0 operator form: L0_AUTO, L0_ELEMENT or L0_STENCIL (vfem_set_level0_kernel)."""
+        _check(self._lib.vfem_set_level0_kernel(self._h, int(kind)))
+
+    @property
+    def level0_kernel(self):
+        """Level-0 form in effect: L0_ELEMENT or L0_STENCIL."""
+        return int(self._lib.vfem_level0_kernel(self._h))
 
     def matvec_block(self, X, precision=FP64):
         """Y[k] = A X[k] for every row of X (multi-RHS kernel for FP32/BF16)."""

@@ -100,6 +100,11 @@ __device__ __forceinline__ Range flat_range(int64_t n)
 // not stored.
 constexpr int FLAT_U = 4;
 
+// Workgroup barrier that only orders LDS: waits for this wave's LDS (and scalar)
+// accesses, not for its outstanding global loads, so prefetches stay in flight
+// across the barrier (__syncthreads() would drain them).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -397,6 +402,350 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
 }
 
 // ---------------------------------------------------------------------------
+// Level-0 matvec as an assembled stencil.
+//
+// Around a node whose 8 elements are all present, K x is a 27-point stencil of
+// 3x3 blocks, y_i = sum_d S_d x_{i+d}, S_d = sum over the elements holding both i
+// and i+d of the matching Ke block.  For the trilinear hexahedron of an
+// orthotropic material (the reference's Ke0, inout.cpp:43-72, included) S_d has
+// a fixed structure: S_d[c][c] depends only on whether d moves along the axis of
+// component c and on how many other offsets are nonzero (6 classes), and
+// S_d[c][c2] (c != c2) is zero unless d moves along both components' axes, where
+// it is sign(d_a d_b) times one of two values chosen by the third offset.  153 of
+// the 243 entries are nonzero, from 24 distinct values -- against 576 products
+// per element in the element form.  The host verifies this structure on the
+// symmetrised Ke and extracts the 24 values (StencilCoef); the kernels keep them
+// in scalar registers and the zero pattern is resolved at compile time.
+//
+// Nodes with an absent element (surface nodes, 6% of the 256^3 ellipsoid
+// union) are then recomputed in element form by k_stencil_fix, node-centric
+// over a precomputed list, so the two kernels together give the element-form
+// operator (fixed rows are zero, MatrixFreeSparse.hpp:198-214).
+//
+// k_matvec_stencil: a block owns ST_X x ST_Y output nodes in x-y and marches
+// through ST_Z node planes.  Plane p (with a one-node halo) is staged in LDS
+// (double-buffered, one barrier per plane); each thread reads its 3x3
+// neighbourhood once and adds the contributions of that plane to its outputs on
+// planes p-1, p and p+1, so the output of plane p-1 is complete and stored.  A
+// wave whose neighbourhoods hold no used node skips the products.
+// ---------------------------------------------------------------------------
+constexpr int ST_X = 16, ST_Y = 16, ST_Z = 32;  // output nodes per block
+constexpr int ST_HX = ST_X + 2;                  // staged plane row with halo
+constexpr int ST_HN = (ST_X + 2) * (ST_Y + 2);   // staged nodes per plane (324)
+static_assert(ST_X * ST_Y == kBlock, "one output column per thread");
+static_assert(ST_HN <= 2 * kBlock, "two staged slots per thread");
+
+// Coefficient slot of S_d[c][c2] for d = (dx, dy, dz) in {-1, 0, 1}^3 under the
+// component-to-axis map `perm` (0: component c moves along axis c; 1: components
+// 0 and 1 move along y and x).  Diagonal slots: 6 c + 3 [d moves along c's axis]
+// + (number of the other nonzero offsets); coupling slots: 18 + 2 pair + [third
+// offset nonzero], pairs (0,1), (0,2), (1,2), with sign d_a d_b.  idx < 0: zero.
+struct StencilSlot {
+    int idx, sign;
+};
+__host__ __device__ constexpr int st_axis(int perm, int c) { return (perm == 1 && c < 2) ? 1 - c : c; }
+__host__ __device__ constexpr StencilSlot st_slot(int perm, int dx, int dy, int dz, int c, int c2)
+{
+    const int d0 = dx, d1 = dy, d2 = dz;
+    if (c == c2) {
+        const int a = st_axis(perm, c);
+        const int da = a == 0 ? d0 : (a == 1 ? d1 : d2);
+        const int na = da != 0 ? 1 : 0;
+        const int no = (d0 != 0) + (d1 != 0) + (d2 != 0) - na;
+        return {c * 6 + na * 3 + no, 1};
+    }
+    const int a = st_axis(perm, c), b = st_axis(perm, c2);
+    const int da = a == 0 ? d0 : (a == 1 ? d1 : d2);
+    const int db = b == 0 ? d0 : (b == 1 ? d1 : d2);
+    if (da == 0 || db == 0) return {-1, 0};
+    const int t = 3 - a - b;
+    const int dt = t == 0 ? d0 : (t == 1 ? d1 : d2);
+    const int pair = (c + c2 == 1) ? 0 : (c + c2 == 2 ? 1 : 2);
+    return {18 + pair * 2 + (dt != 0 ? 1 : 0), da * db};
+}
+
+template <typename T> struct StencilCoef {
+    T c[24];  // structured stencil values (st_slot)
+    T t[3];   // diagonal of T = sum_{dx,dy} S_(dx,dy,1) (centred form)
+};
+
+// Blocks of the stencil matvec that own at least one used node: tile index
+// tx | ty << 11 | tz << 22.
+struct StencilTiles {
+    int n;
+    const uint32_t *tile;
+};
+
+// Used, non-fixed level-0 nodes with at least one absent element around them:
+// {compact id, x, y, z | absent << 24}, absent bit a + 2b + 4c for the element
+// (x-1+a, y-1+b, z-1+c).
+struct SurfaceList {
+    int64_t n;
+    const int4 *e;
+};
+
+// A staged node: its three values and id << 2 | used << 1 | fixed (bit pattern).
+template <typename T> struct alignas(4 * sizeof(T)) StNode {
+    T v[3];
+    T info;
+};
+__device__ __forceinline__ float st_pack(int i, float) { return __int_as_float(i); }
+__device__ __forceinline__ double st_pack(int i, double) { return __longlong_as_double((long long)i); }
+__device__ __forceinline__ int st_unpack(float v) { return __float_as_int(v); }
+__device__ __forceinline__ int st_unpack(double v) { return (int)__double_as_longlong(v); }
+
+// Row words of one staged node on one plane: the used and fixed words and the
+// segment base.  A slot's segment offset within a plane (so) and bit (sb) are
+// fixed; `in` is false outside the grid (clamped in-range loads).
+struct StMeta {
+    uint64_t w, f;
+    int base;
+    bool in;
+};
+__device__ __forceinline__ StMeta st_meta(const DevLevel &L, int so, bool sin, int gz)
+{
+    StMeta m;
+    m.in = sin && (unsigned)gz < (unsigned)L.nz;
+    const int seg = so + (m.in ? gz : 0) * (L.ny * L.S);
+    m.w = L.nbits[seg];
+    m.f = L.fbits[seg];
+    m.base = L.nbase[seg];
+    return m;
+}
+
+// acc[c] += sum_{dx, dy, c2} S_(dx, dy, DZ)[c][c2] v[dy][dx][c2]; the centre
+// column is skipped when v holds differences to it (SKIPC).
+template <typename T, int PERM, int DZ, bool SKIPC>
+__device__ __forceinline__ void st_accum(T (&acc)[3], const T (&v)[3][3][3], const StencilCoef<T> &K)
+{
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            if (SKIPC && dy == 1 && dx == 1) continue;
+#pragma unroll
+            for (int c = 0; c < 3; c++)
+#pragma unroll
+                for (int c2 = 0; c2 < 3; c2++) {
+                    const StencilSlot s = st_slot(PERM, dx - 1, dy - 1, DZ, c, c2);
+                    if (s.idx < 0) continue;
+                    const T k = s.sign > 0 ? K.c[s.idx] : -K.c[s.idx];
+                    acc[c] = fma(k, v[dy][dx][c2], acc[c]);
+                }
+        }
+}
+
+// Stencil matvec.  Loads are pipelined over two planes: while plane p is staged
+// and computed, the values of plane p+1 (addressed from row words loaded one
+// plane earlier) and the row words of plane p+2 are in flight.  Each thread
+// stages the halo-tile slots t and t + 256.  FP32 works on differences: the
+// contributions of plane p use v - r_p, r_p the thread's own column value on
+// plane p, and the output adds T (r_{q-1} + r_{q+1} - 2 r_q), which equals the
+// stencil of the uncentred vector at nodes whose 27 neighbours are used (every
+// node with all 8 elements; the others are recomputed by k_stencil_fix).  So the
+// FP32 rounding scales with the local variation of x, as in the element form.
+template <typename T, int PERM>
+__global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCoef<T> K, StencilTiles tiles,
+                                                            const T *__restrict__ x, T *__restrict__ y, CGState *st,
+                                                            Reduction red, int fin, int honor_done)
+{
+    constexpr bool CENTRE = sizeof(T) == 4;
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
+    if (honor_done && st->done) return;
+    __shared__ StNode<T> pl[2][ST_HN];
+    const uint32_t tl = tiles.tile[blockIdx.x];
+    const int x0 = (int)(tl & 2047) * ST_X, y0 = (int)((tl >> 11) & 2047) * ST_Y, z0 = (int)(tl >> 22) * ST_Z;
+    const int z1 = min(z0 + ST_Z, L.nz);
+    const int t = threadIdx.x, lx = t & (ST_X - 1), ly = t / ST_X;
+    // staged slots of this thread: segment offset within a plane, bit, in grid
+    int so[2], sb[2];
+    bool sok[2], sin[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = t + k * kBlock;
+        sok[k] = s < ST_HN;
+        const int ss = sok[k] ? s : 0;
+        const int gx = x0 - 1 + ss % ST_HX, gy = y0 - 1 + ss / ST_HX;
+        sin[k] = sok[k] && (unsigned)gx < (unsigned)L.nx && (unsigned)gy < (unsigned)L.ny;
+        so[k] = sin[k] ? gy * L.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    // pipeline: row words of plane p+1 (mn), values of plane p (xr, with id/flags)
+    StMeta mn[2];
+    T xr[2][3];
+    int inf[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const StMeta m = st_meta(L, so[k], sin[k], z0 - 1);
+        const bool used = m.in && ((m.w >> sb[k]) & 1);
+        const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
+        inf[k] = id << 2 | (used ? 2 : 0) | ((m.in && ((m.f >> sb[k]) & 1)) ? 1 : 0);
+        const T *q = x + 3 * (int64_t)(used ? id : 0);
+        xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
+        mn[k] = st_meta(L, so[k], sin[k], z0);
+    }
+    T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};  // outputs p-1, p, p+1
+    T rm2[3] = {0, 0, 0}, rm1[3] = {0, 0, 0};                      // own column values, planes p-2, p-1
+    int info_m1 = 0;                                              // own node of plane p-1
+    double dacc = 0;
+    for (int p = z0 - 1; p <= z1; p++) {
+        StNode<T> *buf = pl[p & 1];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const bool used = inf[k] & 2;
+            if (sok[k])
+                buf[k * kBlock + t] = StNode<T>{{used ? xr[k][0] : T(0), used ? xr[k][1] : T(0), used ? xr[k][2] : T(0)},
+                                                st_pack(inf[k], T(0))};
+        }
+        // values of plane p+1, row words of plane p+2
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const StMeta &m = mn[k];
+            const bool used = m.in && ((m.w >> sb[k]) & 1);
+            const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
+            inf[k] = id << 2 | (used ? 2 : 0) | ((m.in && ((m.f >> sb[k]) & 1)) ? 1 : 0);
+            const T *q = x + 3 * (int64_t)(used ? id : 0);
+            xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) mn[k] = st_meta(L, so[k], sin[k] && p + 2 <= z1, p + 2);
+        lds_barrier();
+        T v[3][3][3];
+        int any = 0, info_c = 0;
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const StNode<T> q = buf[(ly + dy) * ST_HX + lx + dx];
+                v[dy][dx][0] = q.v[0];
+                v[dy][dx][1] = q.v[1];
+                v[dy][dx][2] = q.v[2];
+                const int in = st_unpack(q.info);
+                any |= in;
+                if (dy == 1 && dx == 1) info_c = in;
+            }
+        const T r0[3] = {v[1][1][0], v[1][1][1], v[1][1][2]};
+        if (__ballot(any & 2)) {  // some used node in the wave's 3x3 neighbourhoods
+            if constexpr (CENTRE) {
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+                    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) v[dy][dx][c] -= r0[c];
+            }
+            st_accum<T, PERM, 1, CENTRE>(a0, v, K);
+            st_accum<T, PERM, 0, CENTRE>(a1, v, K);
+            st_accum<T, PERM, -1, CENTRE>(a2, v, K);
+        }
+        if (p > z0 && (info_m1 & 2)) {
+            // output plane q = p-1 is complete
+            T r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                r[c] = a0[c];
+                if constexpr (CENTRE) r[c] = fma(K.t[c], fma(T(-2), rm1[c], rm2[c] + r0[c]), r[c]);
+                if (info_m1 & 1) r[c] = T(0);
+            }
+            T *q = y + 3 * (int64_t)(info_m1 >> 2);
+            q[0] = r[0];
+            q[1] = r[1];
+            q[2] = r[2];
+            dacc += (double)rm1[0] * (double)r[0] + (double)rm1[1] * (double)r[1] + (double)rm1[2] * (double)r[2];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a0[c] = a1[c];
+            a1[c] = a2[c];
+            a2[c] = 0;
+            rm2[c] = rm1[c];
+            rm1[c] = r0[c];
+        }
+        info_m1 = info_c;
+    }
+    if (fin >= 0) block_partials(dacc, red);
+}
+
+// Surface nodes of the stencil matvec (used, not fixed, some element absent):
+// y_i is recomputed in element form over the present elements around the node,
+// on differences to x_i (Ke annihilates rigid translations), one list entry per
+// thread, and the reduction gets x_i . (y_new - y_stencil).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__restrict__ ke, SurfaceList sl,
+                                                         const T *__restrict__ x, T *__restrict__ y, CGState *st,
+                                                         Reduction red, int fin, int honor_done)
+{
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
+    if (honor_done && st->done) return;
+    double dacc = 0;
+    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
+        const int64_t i = base + threadIdx.x;
+        const bool act = i < sl.n;
+        const int4 e = sl.e[act ? i : 0];
+        const int gx = e.y, gy = e.z, gz = e.w & 0xffffff;
+        const unsigned present = act ? (~((unsigned)e.w >> 24) & 255u) : 0u;
+        T xv[27][3];
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const int id = lookup(L, gx + k % 3 - 1, gy + (k / 3) % 3 - 1, gz + k / 9 - 1);
+            const T *p = x + 3 * (int64_t)(id >= 0 ? id : 0);
+            const T a0 = p[0], a1 = p[1], a2 = p[2];
+            xv[k][0] = id >= 0 ? a0 : T(0);
+            xv[k][1] = id >= 0 ? a1 : T(0);
+            xv[k][2] = id >= 0 ? a2 : T(0);
+        }
+        const T ctr[3] = {xv[13][0], xv[13][1], xv[13][2]};
+        center_neighbours(xv);
+        T c0 = 0, c1 = 0, c2 = 0;
+#pragma unroll
+        for (int el = 0; el < 8; el++) {
+            if (!__any((present >> el) & 1)) continue;
+            const int a = el & 1, b = (el >> 1) & 1, c = el >> 2;
+            const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+            T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const T v = xv[k][j];
+                    s0 += ke[(3 * Ln + 0) * 24 + 3 * m + j] * v;
+                    s1 += ke[(3 * Ln + 1) * 24 + 3 * m + j] * v;
+                    s2 += ke[(3 * Ln + 2) * 24 + 3 * m + j] * v;
+                }
+            }
+            if ((present >> el) & 1) {
+                c0 += s0;
+                c1 += s1;
+                c2 += s2;
+            }
+        }
+        if (act) {
+            T *q = y + 3 * (int64_t)e.x;
+            const T o0 = q[0], o1 = q[1], o2 = q[2];
+            q[0] = c0;
+            q[1] = c1;
+            q[2] = c2;
+            dacc += (double)ctr[0] * ((double)c0 - (double)o0) + (double)ctr[1] * ((double)c1 - (double)o1) +
+                    (double)ctr[2] * ((double)c2 - (double)o2);
+        }
+    }
+    if (fin >= 0) block_partials(dacc, red);
+}
+
+// ---------------------------------------------------------------------------
 // Level-0 matvec on the matrix cores: element form Y_e = Ke X_e on
 // v_mfma_f32_16x16x32_bf16, then an atomic-free node gather.
 //
@@ -472,10 +821,6 @@ __device__ __forceinline__ bool element_present(const DevLevel &L, int ex, int e
     return (L.ebits[((int64_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1;
 }
 
-// Workgroup barrier that only orders LDS: waits for this wave's LDS (and scalar)
-// accesses, not for its outstanding global loads, so prefetches stay in flight
-// across the barrier (__syncthreads() would drain them).
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 __device__ __forceinline__ void split_bf16(float v, __bf16 &h, __bf16 &m, __bf16 &l, int parts)
 {
@@ -1858,6 +2203,14 @@ struct vfem_system {
     bool unet_on = false;       // set while enqueueing a learned plan
     float *unet_scale = nullptr;
     bool use_block = true;      // batched FP32 level-0 matvec: one launch per 16 columns on the matrix cores
+    int l0_kind = VFEM_L0_AUTO; // level-0 operator: assembled stencil (when Ke has its structure) or element form
+    bool st_ok = false;         // Ke assembles to the structured stencil (build_stencil)
+    int st_perm = 0;            // component-to-axis map of the stencil (st_axis)
+    StencilCoef<double> st64{};
+    StencilCoef<float> st32{};
+    StencilTiles st_tiles{};    // blocks of the stencil matvec
+    SurfaceList st_surf{};      // nodes corrected by k_stencil_fix
+    int st_fix_grid = 0;        // blocks of k_stencil_fix (0: no surface node)
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -1942,6 +2295,119 @@ int build_schedule(vfem_system *S)
     return 0;
 }
 
+// Assembled-stencil form of the level-0 operator (see k_matvec_stencil): the
+// interior stencil of the symmetrised Ke, its 24 structured values under one of
+// the component-to-axis maps the kernels are built for, the blocks that own used
+// nodes and the list of surface nodes.  Leaves st_ok false when Ke does not have
+// the structure (the element-form kernels then serve every level-0 matvec).
+int build_stencil(vfem_system *S)
+{
+    const Hierarchy &H = S->H;
+    const HostLevel &L = H.L[0];
+    S->st_ok = false;
+    double sd[3][3][3][3][3] = {};  // [dz][dy][dx][c][c2]
+    for (int e = 0; e < 8; e++) {
+        const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;        // element (x-1+a, y-1+b, z-1+c)
+        const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);      // the node's corner in it
+        for (int m = 0; m < 8; m++) {
+            const int dx = a + (m & 1) - 1, dy = b + ((m >> 1) & 1) - 1, dz = c + (m >> 2) - 1;
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) sd[dz + 1][dy + 1][dx + 1][r][q] += H.ke[(3 * Ln + r) * 24 + 3 * m + q];
+        }
+    }
+    double smax = 0;
+    for (int i = 0; i < 243; i++) smax = std::max(smax, std::fabs((&sd[0][0][0][0][0])[i]));
+    if (!(smax > 0)) return 0;
+    for (int perm : {1, 0}) {
+        double coef[24];
+        bool seen[24] = {};
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++)
+                    for (int c = 0; c < 3; c++)
+                        for (int c2 = 0; c2 < 3; c2++) {
+                            const StencilSlot sl = st_slot(perm, dx, dy, dz, c, c2);
+                            if (sl.idx < 0 || seen[sl.idx]) continue;
+                            seen[sl.idx] = true;
+                            coef[sl.idx] = sl.sign * sd[dz + 1][dy + 1][dx + 1][c][c2];
+                        }
+        double err = 0;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++)
+                    for (int c = 0; c < 3; c++)
+                        for (int c2 = 0; c2 < 3; c2++) {
+                            const StencilSlot sl = st_slot(perm, dx, dy, dz, c, c2);
+                            const double model = sl.idx < 0 ? 0.0 : sl.sign * coef[sl.idx];
+                            err = std::max(err, std::fabs(model - sd[dz + 1][dy + 1][dx + 1][c][c2]));
+                        }
+        // T = sum_{dx,dy} S_(dx,dy,1): diagonal under this structure (the couplings
+        // are odd in dx or dy)
+        double tm[3][3] = {};
+        for (int dy = 0; dy < 3; dy++)
+            for (int dx = 0; dx < 3; dx++)
+                for (int c = 0; c < 3; c++)
+                    for (int c2 = 0; c2 < 3; c2++) tm[c][c2] += sd[2][dy][dx][c][c2];
+        for (int c = 0; c < 3; c++)
+            for (int c2 = 0; c2 < 3; c2++)
+                if (c != c2) err = std::max(err, std::fabs(tm[c][c2]));
+        if (err > 1e-13 * smax) continue;
+        for (int i = 0; i < 24; i++) {
+            S->st64.c[i] = seen[i] ? coef[i] : 0.0;
+            S->st32.c[i] = (float)S->st64.c[i];
+        }
+        for (int c = 0; c < 3; c++) {
+            S->st64.t[c] = tm[c][c];
+            S->st32.t[c] = (float)tm[c][c];
+        }
+        S->st_perm = perm;
+        S->st_ok = true;
+        break;
+    }
+    if (!S->st_ok) return 0;
+    const int tx = (L.nx + ST_X - 1) / ST_X, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + ST_Z - 1) / ST_Z;
+    if (tx > 2047 || ty > 2047 || tz > 1023 || L.nz >= (1 << 24)) {
+        S->st_ok = false;
+        return 0;
+    }
+    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
+    std::vector<int4> surf;
+    auto present = [&](int ex, int ey, int ez) {
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return false;
+        return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) != 0;
+    };
+    int64_t id = 0;
+    for (int z = 0; z < L.nz; z++)
+        for (int y = 0; y < L.ny; y++)
+            for (int x = 0; x < L.nx; x++) {
+                const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
+                if (!((L.nbits[seg] >> (x & 63)) & 1)) continue;
+                busy[((size_t)(z / ST_Z) * ty + y / ST_Y) * tx + x / ST_X] = 1;
+                const bool fixed = (L.fbits[seg] >> (x & 63)) & 1;
+                unsigned absent = 0;
+                for (int e = 0; e < 8; e++)
+                    if (!present(x - 1 + (e & 1), y - 1 + ((e >> 1) & 1), z - 1 + (e >> 2))) absent |= 1u << e;
+                if (absent && !fixed) surf.push_back(make_int4((int)id, x, y, z | (int)(absent << 24)));
+                id++;
+            }
+    std::vector<uint32_t> tiles;
+    for (int k = 0; k < tz; k++)
+        for (int j = 0; j < ty; j++)
+            for (int i = 0; i < tx; i++)
+                if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
+    if (tiles.empty()) tiles.push_back(0);
+    uint32_t *dt;
+    if (int rc = S->upload(&dt, tiles)) return rc;
+    S->st_tiles = {(int)tiles.size(), dt};
+    S->st_surf.n = (int64_t)surf.size();
+    S->st_fix_grid = (int)std::min<int64_t>(512, (S->st_surf.n + kBlock - 1) / kBlock);
+    if (surf.empty()) surf.push_back(make_int4(0, 0, 0, 0));
+    int4 *ds;
+    if (int rc = S->upload(&ds, surf)) return rc;
+    S->st_surf.e = ds;
+    return 0;
+}
+
 // Solver vectors for `nb` right-hand sides, column-major (column c of a level
 // vector at c * 3 * nodes).  Reallocating drops the captured graphs, which hold
 // the old addresses.
@@ -1992,6 +2458,7 @@ int alloc_vectors(vfem_system *S, int nb)
                 const BlockGrid G = block_grid(S->dl[0], c);
                 cap = std::max<int64_t>(cap, (int64_t)G.tx * G.ty * G.tz);
             }
+        if (S->st_ok) cap = std::max<int64_t>(cap, (int64_t)S->st_tiles.n + S->st_fix_grid);
         S->red.cap = (int)cap;
     }
     if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap * nb, true, V)) return rc;
@@ -2046,6 +2513,7 @@ int upload_system(vfem_system *S)
     if (int rc = S->upload(&S->ke64, ke)) return rc;
     if (int rc = S->upload(&S->ke32, convert<float>(ke))) return rc;
     if (int rc = build_schedule(S)) return rc;
+    if (int rc = build_stencil(S)) return rc;
     if (int rc = S->upload(&S->kinv_ref64, H.kcinv_ref)) return rc;
     if (int rc = S->upload(&S->kinv_gal64, H.kcinv_gal)) return rc;
     if (int rc = S->upload(&S->kinv_ref32, convert<float>(H.kcinv_ref))) return rc;
@@ -2121,13 +2589,55 @@ void launch_matvec_block(vfem_system *S, const float *x, float *y, int fin, int 
     else launch_block_passes<6>(S, x, y, fin, honor_done, c);
 }
 
-// level-0 matvec.  FP64 vectors: exact FP64 products on the matrix cores
-// (v_mfma_f64_16x16x4_f64; the VALU kernel when use_mfma64 is off), or (VFEM_MIXED
-// plans, unless `exact`) BF16 products of FP64 centre differences.
+template <typename T> const StencilCoef<T> &st_coef(vfem_system *S);
+template <> const StencilCoef<double> &st_coef<double>(vfem_system *S) { return S->st64; }
+template <> const StencilCoef<float> &st_coef<float>(vfem_system *S) { return S->st32; }
+
+// stencil matvec of level 0 plus its surface correction; the reduction adds the
+// partials of both kernels
+template <typename T> void launch_matvec_stencil(vfem_system *S, const T *x, T *y, int fin, int honor_done)
+{
+    const DevLevel &L = S->dl[0];
+    const int nt = S->st_tiles.n;
+    if (S->st_perm == 1)
+        hipLaunchKernelGGL((k_matvec_stencil<T, 1>), dim3(nt, S->ncols), dim3(kBlock), 0, S->stream, L, st_coef<T>(S),
+                           S->st_tiles, x, y, S->st, S->red, fin, honor_done);
+    else
+        hipLaunchKernelGGL((k_matvec_stencil<T, 0>), dim3(nt, S->ncols), dim3(kBlock), 0, S->stream, L, st_coef<T>(S),
+                           S->st_tiles, x, y, S->st, S->red, fin, honor_done);
+    if (S->st_fix_grid > 0) {
+        Reduction r2 = S->red;
+        r2.partials += nt;
+        hipLaunchKernelGGL(k_stencil_fix<T>, dim3(S->st_fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L, kefine<T>(S),
+                           S->st_surf, x, y, S->st, r2, fin, honor_done);
+    }
+    enqueue_finalize(S, S->stream, nt + S->st_fix_grid, S->ncols, fin, honor_done);
+}
+
+bool stencil_on(const vfem_system *S) { return S->st_ok && S->l0_kind != VFEM_L0_ELEMENT; }
+
+// level-0 matvec.  With the stencil form on: FP64, and FP32 at full accuracy
+// (6 products), go to k_matvec_stencil; the BF16-product variants stay on the
+// matrix cores.  Element form: FP64 vectors get exact FP64 products on the
+// matrix cores (v_mfma_f64_16x16x4_f64; the VALU kernel when use_mfma64 is off),
+// or (VFEM_MIXED plans, unless `exact`) BF16 products of FP64 centre differences.
 template <typename T>
 void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_done, int passes = 6, bool exact = false)
 {
     const DevLevel &L = S->dl[0];
+    if (stencil_on(S)) {
+        if constexpr (sizeof(T) == 8) {
+            if (exact || !S->l0_mfma) {
+                launch_matvec_stencil<T>(S, x, y, fin, honor_done);
+                return;
+            }
+        } else {
+            if (passes == 6) {
+                launch_matvec_stencil<T>(S, x, y, fin, honor_done);
+                return;
+            }
+        }
+    }
     if constexpr (sizeof(T) == 8) {
         if ((!S->l0_mfma || exact) && S->use_mfma64) {
             const MfmaTiles t = S->tiles0;
@@ -3060,6 +3570,31 @@ int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz)
     *tey = kBlockCfgs[index].tey;
     *mz = kBlockCfgs[index].mz;
     return VFEM_OK;
+}
+
+int vfem_set_level0_kernel(vfem_system *sys, int32_t kind)
+{
+    if (!sys || kind < VFEM_L0_AUTO || kind > VFEM_L0_STENCIL) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (kind == VFEM_L0_STENCIL && !sys->st_ok) {
+        g_last_error = "Ke does not assemble to the structured level-0 stencil";
+        return VFEM_ERR_UNSUPPORTED;
+    }
+    if (kind != sys->l0_kind) {
+        HIP_TRY(hipStreamSynchronize(sys->stream));
+        for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
+        sys->graphs.clear();  // captured iterations hold the old kernels
+        sys->l0_kind = kind;
+    }
+    return VFEM_OK;
+}
+
+int32_t vfem_level0_kernel(const vfem_system *sys)
+{
+    if (!sys) return -1;
+    return stencil_on(sys) ? VFEM_L0_STENCIL : VFEM_L0_ELEMENT;
 }
 
 int vfem_set_block_config(vfem_system *sys, int32_t index)
