@@ -56,7 +56,7 @@ enum vfem_precision {
 
 /* form of the level-0 operator (vfem_set_level0_kernel) */
 enum vfem_level0 {
-    VFEM_L0_AUTO = 0,     /* the stencil form when Ke has its structure, else the element form (default) */
+    VFEM_L0_AUTO = 0,     /* the stencil form where it is available and faster (default) */
     VFEM_L0_ELEMENT = 1,  /* element form: Ke times each element's 24 values, on the matrix cores */
     VFEM_L0_STENCIL = 2   /* assembled 27-point stencil plus a correction of the surface nodes */
 };
@@ -168,13 +168,20 @@ int vfem_get_solution(vfem_system *sys, double *u);
  * vfem_set_level0_kernel. */
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision);
 
-/* Level-0 operator form (enum vfem_level0; no reference counterpart, whose
- * MatxVec.comp is the element form).  VFEM_L0_STENCIL fails with
- * VFEM_ERR_UNSUPPORTED when Ke does not assemble to the structured stencil.
- * vfem_level0_kernel returns the form in effect (VFEM_L0_ELEMENT or
- * VFEM_L0_STENCIL), or -1 for a null system. */
+/* Operator form (enum vfem_level0; no reference counterpart, whose MatxVec.comp
+ * is the element form): VFEM_L0_STENCIL uses the assembled stencil on every level
+ * whose full-element matrix has its structure (level 0: Ke; a damped-cycle coarse
+ * level: the Galerkin matrix of the full child pattern); VFEM_L0_AUTO does so on
+ * level 0 and on coarse levels of at least 100000 nodes (smaller ones are
+ * latency-bound and faster in element form); VFEM_L0_ELEMENT uses the element
+ * form everywhere.  VFEM_L0_STENCIL fails with
+ * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form.  vfem_level0_kernel
+ * returns the form in effect on level 0 and vfem_level_operator_form on `level`
+ * (VFEM_L0_ELEMENT or VFEM_L0_STENCIL), or -1 for a null system or a level out
+ * of range. */
 int vfem_set_level0_kernel(vfem_system *sys, int32_t kind);
 int32_t vfem_level0_kernel(const vfem_system *sys);
+int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level);
 
 /* Multi-RHS form of vfem_matvec (no reference counterpart): y_k = A x_k for nrhs
  * columns (x, y nrhs x ndof row-major).  FP32/BF16 use the multi-RHS matrix-core

@@ -88,6 +88,35 @@ def test_stencil_form_matches_element_form(systems, name):
     assert rel(s32, ref) < 2e-6
 
 
+@pytest.mark.parametrize("name", ["ell24", "ell33"])
+def test_stencil_cycle_matches_element_cycle(systems, name):
+    """The damped cycle with the stencil form on every level (the coarse levels'
+    full Galerkin elements included) against the element-form cycle and the
+    oracle's twin."""
+    A, ora, _, _ = systems[name]
+    forms = [A.level_operator_form(l) for l in range(A.num_levels)]
+    say("operator forms (auto)", name, forms)
+    assert forms[0] == V.L0_STENCIL  # coarse levels this small keep the element form under L0_AUTO
+    r = np.random.default_rng(4).standard_normal(A.n)
+    r[np.repeat(ora.fixed(0), 3)] = 0
+    z_d = ora.vcycle(r, O.DAMPED, 0.6)
+    out = {}
+    try:
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+            A.set_level0_kernel(kind)
+            if kind == V.L0_STENCIL:  # every div-2 level with full elements
+                assert all(A.level_operator_form(l) == V.L0_STENCIL for l in range(A.num_levels - 1))
+            out[kind] = (V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r),
+                         V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.FP32).solve(r))
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
+    say("cycle %s: fp64 %.2e fp32 %.2e (element fp32 %.2e)" % (name, rel(out[2][0], z_d), rel(out[2][1], z_d),
+                                                               rel(out[1][1], z_d)))
+    assert rel(out[V.L0_STENCIL][0], z_d) < 1e-12
+    assert rel(out[V.L0_STENCIL][0], out[V.L0_ELEMENT][0]) < 1e-12
+    assert rel(out[V.L0_STENCIL][1], z_d) < 1e-5
+
+
 def test_unstructured_ke_uses_element_form(ke0):
     """A Ke without the stencil structure keeps the element-form kernels, and
     asking for the stencil form is refused."""

@@ -103,6 +103,8 @@ def lib():
     L.vfem_set_level0_kernel.argtypes = [vp, ctypes.c_int32]
     L.vfem_level0_kernel.argtypes = [vp]
     L.vfem_level0_kernel.restype = ctypes.c_int32
+    L.vfem_level_operator_form.argtypes = [vp, ctypes.c_int32]
+    L.vfem_level_operator_form.restype = ctypes.c_int32
     L.vfem_block_config_count.restype = ctypes.c_int32
     L.vfem_block_config.argtypes = [ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int32)] * 3
     L.vfem_set_block_config.argtypes = [vp, ctypes.c_int32]
@@ -364,6 +366,10 @@ class MatrixFreeSparse:
     def level0_kernel(self):
         """Level-0 form in effect: L0_ELEMENT or L0_STENCIL."""
         return int(self._lib.vfem_level0_kernel(self._h))
+
+    def level_operator_form(self, level):
+        """Operator form in effect on `level`: L0_ELEMENT or L0_STENCIL."""
+        return int(self._lib.vfem_level_operator_form(self._h, int(level)))
 
     def matvec_block(self, X, precision=FP64):
         """Y[k] = A X[k] for every row of X (multi-RHS kernel for FP32/BF16)."""

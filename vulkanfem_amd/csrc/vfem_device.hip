@@ -429,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_fine(DevLevel L, const T *__r
 // planes p-1, p and p+1, so the output of plane p-1 is complete and stored.  A
 // wave whose neighbourhoods hold no used node skips the products.
 // ---------------------------------------------------------------------------
-constexpr int ST_X = 16, ST_Y = 16, ST_Z = 32;  // output nodes per block
+constexpr int ST_X = 16, ST_Y = 16;             // output nodes per block in x-y
 constexpr int ST_HX = ST_X + 2;                  // staged plane row with halo
 constexpr int ST_HN = (ST_X + 2) * (ST_Y + 2);   // staged nodes per plane (324)
 static_assert(ST_X * ST_Y == kBlock, "one output column per thread");
@@ -470,18 +470,23 @@ template <typename T> struct StencilCoef {
 };
 
 // Blocks of the stencil matvec that own at least one used node: tile index
-// tx | ty << 11 | tz << 22.
+// tx | ty << 11 | tz << 22 (tz counts chunks of the kernel's ZC planes).
 struct StencilTiles {
     int n;
     const uint32_t *tile;
 };
 
 // Used, non-fixed level-0 nodes with at least one absent element around them:
-// {compact id, x, y, z | absent << 24}, absent bit a + 2b + 4c for the element
-// (x-1+a, y-1+b, z-1+c).
+// e[i] = {compact id, present, used, 0}: present bit a + 2b + 4c for the element
+// (x-1+a, y-1+b, z-1+c), used bit k = (dz * 3 + dy) * 3 + dx for the neighbour
+// (x-1+dx, y-1+dy, z-1+dz); rb[r * n + i] = compact id that position x-1 of
+// neighbour row r = dz * 3 + dy has or would have, so the row's three ids follow
+// from the used bits without touching the level's row words.
 struct SurfaceList {
     int64_t n;
     const int4 *e;
+    const int32_t *rb;
+    const uint2 *pat;  // coarse levels: child pattern of the 8 elements (byte a + 2b + 4c)
 };
 
 // A staged node: its three values and id << 2 | used << 1 | fixed (bit pattern).
@@ -544,7 +549,7 @@ __device__ __forceinline__ void st_accum(T (&acc)[3], const T (&v)[3][3][3], con
 // stencil of the uncentred vector at nodes whose 27 neighbours are used (every
 // node with all 8 elements; the others are recomputed by k_stencil_fix).  So the
 // FP32 rounding scales with the local variation of x, as in the element form.
-template <typename T, int PERM>
+template <typename T, int PERM, int ZC>
 __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCoef<T> K, StencilTiles tiles,
                                                             const T *__restrict__ x, T *__restrict__ y, CGState *st,
                                                             Reduction red, int fin, int honor_done)
@@ -560,8 +565,8 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
     if (honor_done && st->done) return;
     __shared__ StNode<T> pl[2][ST_HN];
     const uint32_t tl = tiles.tile[blockIdx.x];
-    const int x0 = (int)(tl & 2047) * ST_X, y0 = (int)((tl >> 11) & 2047) * ST_Y, z0 = (int)(tl >> 22) * ST_Z;
-    const int z1 = min(z0 + ST_Z, L.nz);
+    const int x0 = (int)(tl & 2047) * ST_X, y0 = (int)((tl >> 11) & 2047) * ST_Y, z0 = (int)(tl >> 22) * ZC;
+    const int z1 = min(z0 + ZC, L.nz);
     const int t = threadIdx.x, lx = t & (ST_X - 1), ly = t / ST_X;
     // staged slots of this thread: segment offset within a plane, bit, in grid
     int so[2], sb[2];
@@ -672,11 +677,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
     if (fin >= 0) block_partials(dacc, red);
 }
 
-// Surface nodes of the stencil matvec (used, not fixed, some element absent):
-// y_i is recomputed in element form over the present elements around the node,
+// Surface nodes of the stencil matvec (used, not fixed, some element absent, or
+// on a coarse level some element partial): y_i is recomputed in element form over
+// the present elements around the node (TAB: each with the Galerkin matrix of its
+// child pattern, tab[pattern * 576], pattern 0 = absent),
 // on differences to x_i (Ke annihilates rigid translations), one list entry per
 // thread, and the reduction gets x_i . (y_new - y_stencil).
-template <typename T>
+template <typename T, bool TAB>
 __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__restrict__ ke, SurfaceList sl,
                                                          const T *__restrict__ x, T *__restrict__ y, CGState *st,
                                                          Reduction red, int fin, int honor_done)
@@ -693,18 +700,26 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
     for (int64_t base = (int64_t)blockIdx.x * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + threadIdx.x;
         const bool act = i < sl.n;
-        const int4 e = sl.e[act ? i : 0];
-        const int gx = e.y, gy = e.z, gz = e.w & 0xffffff;
-        const unsigned present = act ? (~((unsigned)e.w >> 24) & 255u) : 0u;
+        const int64_t ic = act ? i : 0;
+        const int4 e = sl.e[ic];
+        const unsigned present = act ? (unsigned)e.y : 0u, used = (unsigned)e.z;
+        uint2 pat = make_uint2(0, 0);
+        if constexpr (TAB) pat = sl.pat[ic];
         T xv[27][3];
 #pragma unroll
-        for (int k = 0; k < 27; k++) {
-            const int id = lookup(L, gx + k % 3 - 1, gy + (k / 3) % 3 - 1, gz + k / 9 - 1);
-            const T *p = x + 3 * (int64_t)(id >= 0 ? id : 0);
-            const T a0 = p[0], a1 = p[1], a2 = p[2];
-            xv[k][0] = id >= 0 ? a0 : T(0);
-            xv[k][1] = id >= 0 ? a1 : T(0);
-            xv[k][2] = id >= 0 ? a2 : T(0);
+        for (int r = 0; r < 9; r++) {
+            const int b = sl.rb[r * sl.n + ic];
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const int k = 3 * r + dx;
+                const bool u = (used >> k) & 1;
+                const int id = b + (dx > 0 ? (int)((used >> (3 * r)) & 1) : 0) + (dx > 1 ? (int)((used >> (3 * r + 1)) & 1) : 0);
+                const T *p = x + 3 * (int64_t)(u ? id : 0);
+                const T a0 = p[0], a1 = p[1], a2 = p[2];
+                xv[k][0] = u ? a0 : T(0);
+                xv[k][1] = u ? a1 : T(0);
+                xv[k][2] = u ? a2 : T(0);
+            }
         }
         const T ctr[3] = {xv[13][0], xv[13][1], xv[13][2]};
         center_neighbours(xv);
@@ -714,6 +729,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
             if (!__any((present >> el) & 1)) continue;
             const int a = el & 1, b = (el >> 1) & 1, c = el >> 2;
             const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+            const T *K = ke + 3 * Ln * 24;
+            if constexpr (TAB) {  // the element's Galerkin matrix (per lane)
+                const unsigned pe = ((el < 4 ? pat.x : pat.y) >> (8 * (el & 3))) & 255u;
+                K += (int64_t)pe * 576;
+            }
             T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
             for (int m = 0; m < 8; m++) {
@@ -721,9 +741,9 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
                     const T v = xv[k][j];
-                    s0 += ke[(3 * Ln + 0) * 24 + 3 * m + j] * v;
-                    s1 += ke[(3 * Ln + 1) * 24 + 3 * m + j] * v;
-                    s2 += ke[(3 * Ln + 2) * 24 + 3 * m + j] * v;
+                    s0 += K[0 * 24 + 3 * m + j] * v;
+                    s1 += K[1 * 24 + 3 * m + j] * v;
+                    s2 += K[2 * 24 + 3 * m + j] * v;
                 }
             }
             if ((present >> el) & 1) {
@@ -2162,6 +2182,21 @@ BlockGrid block_grid(const DevLevel &L, const BlockCfg &c)
 
 }  // namespace
 
+constexpr int64_t kStencilMinNodes = 100000;  // smallest coarse level given the stencil form
+
+// Stencil form of one level's operator (see k_matvec_stencil).
+struct LevelStencil {
+    bool ok = false;
+    bool auto_on = false;       // VFEM_L0_AUTO uses this level's stencil form
+    int perm = 0;               // component-to-axis map (st_axis)
+    int zc = 32;                // planes per block
+    StencilCoef<double> c64{};
+    StencilCoef<float> c32{};
+    StencilTiles tiles{};       // blocks of the stencil kernel
+    SurfaceList surf{};         // nodes recomputed by k_stencil_fix
+    int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
+};
+
 struct vfem_system {
     Hierarchy H;
     int device = 0;
@@ -2204,13 +2239,8 @@ struct vfem_system {
     float *unet_scale = nullptr;
     bool use_block = true;      // batched FP32 level-0 matvec: one launch per 16 columns on the matrix cores
     int l0_kind = VFEM_L0_AUTO; // level-0 operator: assembled stencil (when Ke has its structure) or element form
-    bool st_ok = false;         // Ke assembles to the structured stencil (build_stencil)
-    int st_perm = 0;            // component-to-axis map of the stencil (st_axis)
-    StencilCoef<double> st64{};
-    StencilCoef<float> st32{};
-    StencilTiles st_tiles{};    // blocks of the stencil matvec
-    SurfaceList st_surf{};      // nodes corrected by k_stencil_fix
-    int st_fix_grid = 0;        // blocks of k_stencil_fix (0: no surface node)
+    bool st_ok = false;         // level 0 has the stencil form (build_stencil)
+    LevelStencil stl[kMaxLevels];  // stencil form of each level
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -2295,29 +2325,25 @@ int build_schedule(vfem_system *S)
     return 0;
 }
 
-// Assembled-stencil form of the level-0 operator (see k_matvec_stencil): the
-// interior stencil of the symmetrised Ke, its 24 structured values under one of
-// the component-to-axis maps the kernels are built for, the blocks that own used
-// nodes and the list of surface nodes.  Leaves st_ok false when Ke does not have
-// the structure (the element-form kernels then serve every level-0 matvec).
-int build_stencil(vfem_system *S)
+// Structured-stencil values of an element matrix: assembles the interior
+// stencil of `ke` and accepts it when it matches the structure of st_slot under
+// one of the component-to-axis maps the kernels are built for (to 1e-13 of the
+// largest entry) and T = sum_{dx,dy} S_(dx,dy,1) is diagonal.
+bool stencil_coef(const double *ke, LevelStencil &LS)
 {
-    const Hierarchy &H = S->H;
-    const HostLevel &L = H.L[0];
-    S->st_ok = false;
     double sd[3][3][3][3][3] = {};  // [dz][dy][dx][c][c2]
     for (int e = 0; e < 8; e++) {
-        const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;        // element (x-1+a, y-1+b, z-1+c)
-        const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);      // the node's corner in it
+        const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;   // element (x-1+a, y-1+b, z-1+c)
+        const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c); // the node's corner in it
         for (int m = 0; m < 8; m++) {
             const int dx = a + (m & 1) - 1, dy = b + ((m >> 1) & 1) - 1, dz = c + (m >> 2) - 1;
             for (int r = 0; r < 3; r++)
-                for (int q = 0; q < 3; q++) sd[dz + 1][dy + 1][dx + 1][r][q] += H.ke[(3 * Ln + r) * 24 + 3 * m + q];
+                for (int q = 0; q < 3; q++) sd[dz + 1][dy + 1][dx + 1][r][q] += ke[(3 * Ln + r) * 24 + 3 * m + q];
         }
     }
     double smax = 0;
     for (int i = 0; i < 243; i++) smax = std::max(smax, std::fabs((&sd[0][0][0][0][0])[i]));
-    if (!(smax > 0)) return 0;
+    if (!(smax > 0)) return false;
     for (int perm : {1, 0}) {
         double coef[24];
         bool seen[24] = {};
@@ -2353,28 +2379,53 @@ int build_stencil(vfem_system *S)
                 if (c != c2) err = std::max(err, std::fabs(tm[c][c2]));
         if (err > 1e-13 * smax) continue;
         for (int i = 0; i < 24; i++) {
-            S->st64.c[i] = seen[i] ? coef[i] : 0.0;
-            S->st32.c[i] = (float)S->st64.c[i];
+            LS.c64.c[i] = seen[i] ? coef[i] : 0.0;
+            LS.c32.c[i] = (float)LS.c64.c[i];
         }
         for (int c = 0; c < 3; c++) {
-            S->st64.t[c] = tm[c][c];
-            S->st32.t[c] = (float)tm[c][c];
+            LS.c64.t[c] = tm[c][c];
+            LS.c32.t[c] = (float)tm[c][c];
         }
-        S->st_perm = perm;
-        S->st_ok = true;
-        break;
+        LS.perm = perm;
+        return true;
     }
-    if (!S->st_ok) return 0;
-    const int tx = (L.nx + ST_X - 1) / ST_X, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + ST_Z - 1) / ST_Z;
-    if (tx > 2047 || ty > 2047 || tz > 1023 || L.nz >= (1 << 24)) {
-        S->st_ok = false;
-        return 0;
-    }
+    return false;
+}
+
+// Stencil form of level l (see k_matvec_stencil): the structured values of the
+// level's full-element matrix (Ke at level 0, the Galerkin matrix of the full
+// child pattern on a div-2 coarse level), the blocks that own used nodes and the
+// surface list -- used, non-fixed nodes with an element around them that is not
+// full (absent, or partial on a coarse level), with the row bases and used bits
+// of their 27 neighbours and, on coarse levels, the 8 elements' child patterns.
+// Leaves ok false when the level has no such form (its element-form kernel then
+// serves it).
+int build_level_stencil(vfem_system *S, int l)
+{
+    const Hierarchy &H = S->H;
+    const HostLevel &L = H.L[l];
+    LevelStencil &LS = S->stl[l];
+    LS = LevelStencil{};
+    const bool coarse = l > 0;
+    if (coarse && (L.div != 2 || l >= (int)H.ktab.size() || H.ktab[l].size() != 256 * 576 || L.epat.empty())) return 0;
+    // small coarse levels are latency-bound: one row-segment wave per node row
+    // (k_matvec_coarse) beats a plane-marching block and a second launch, so
+    // VFEM_L0_AUTO keeps their element form
+    LS.auto_on = !coarse || L.nn >= kStencilMinNodes;
+    const double *kfull = coarse ? H.ktab[l].data() + 255 * 576 : H.ke;
+    if (!stencil_coef(kfull, LS)) return 0;
+    LS.zc = L.nz >= 192 ? 32 : (L.nz >= 96 ? 16 : 8);
+    const int tx = (L.nx + ST_X - 1) / ST_X, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + LS.zc - 1) / LS.zc;
+    if (tx > 2047 || ty > 2047 || tz > 1023) return 0;
     std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
     std::vector<int4> surf;
-    auto present = [&](int ex, int ey, int ez) {
-        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return false;
-        return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) != 0;
+    std::vector<uint2> pats;
+    std::vector<int32_t> rb[9];
+    // child pattern of element (ex, ey, ez): 0 absent, 255 full
+    auto pattern = [&](int ex, int ey, int ez) -> unsigned {
+        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return 0;
+        if (coarse) return L.epat[((size_t)ez * L.ey + ey) * L.ex + ex];
+        return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) ? 255u : 0u;
     };
     int64_t id = 0;
     for (int z = 0; z < L.nz; z++)
@@ -2382,12 +2433,36 @@ int build_stencil(vfem_system *S)
             for (int x = 0; x < L.nx; x++) {
                 const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
                 if (!((L.nbits[seg] >> (x & 63)) & 1)) continue;
-                busy[((size_t)(z / ST_Z) * ty + y / ST_Y) * tx + x / ST_X] = 1;
+                busy[((size_t)(z / LS.zc) * ty + y / ST_Y) * tx + x / ST_X] = 1;
                 const bool fixed = (L.fbits[seg] >> (x & 63)) & 1;
-                unsigned absent = 0;
-                for (int e = 0; e < 8; e++)
-                    if (!present(x - 1 + (e & 1), y - 1 + ((e >> 1) & 1), z - 1 + (e >> 2))) absent |= 1u << e;
-                if (absent && !fixed) surf.push_back(make_int4((int)id, x, y, z | (int)(absent << 24)));
+                unsigned present = 0, full = 0;
+                uint32_t pw[2] = {0, 0};
+                for (int e = 0; e < 8; e++) {
+                    const unsigned pe = pattern(x - 1 + (e & 1), y - 1 + ((e >> 1) & 1), z - 1 + (e >> 2));
+                    if (pe) present |= 1u << e;
+                    if (pe == 255) full |= 1u << e;
+                    pw[e >> 2] |= pe << (8 * (e & 3));
+                }
+                if (full != 255u && !fixed) {
+                    unsigned used = 0;
+                    for (int r = 0; r < 9; r++) {
+                        const int yy = y - 1 + r % 3, zz = z - 1 + r / 3;
+                        int32_t b = 0;
+                        if (yy >= 0 && yy < L.ny && zz >= 0 && zz < L.nz) {
+                            const size_t row = ((size_t)zz * L.ny + yy) * L.S;
+                            const int xs = std::max(x - 1, 0);
+                            const uint64_t w = L.nbits[row + (xs >> 6)];
+                            b = L.nbase[row + (xs >> 6)] + __builtin_popcountll(w & ((1ull << (xs & 63)) - 1));
+                            for (int dx = 0; dx < 3; dx++) {
+                                const int xx = x - 1 + dx;
+                                if (xx >= 0 && xx < L.nx && ((L.nbits[row + (xx >> 6)] >> (xx & 63)) & 1)) used |= 1u << (3 * r + dx);
+                            }
+                        }
+                        rb[r].push_back(b);
+                    }
+                    surf.push_back(make_int4((int)id, (int)present, (int)used, 0));
+                    if (coarse) pats.push_back(make_uint2(pw[0], pw[1]));
+                }
                 id++;
             }
     std::vector<uint32_t> tiles;
@@ -2398,13 +2473,35 @@ int build_stencil(vfem_system *S)
     if (tiles.empty()) tiles.push_back(0);
     uint32_t *dt;
     if (int rc = S->upload(&dt, tiles)) return rc;
-    S->st_tiles = {(int)tiles.size(), dt};
-    S->st_surf.n = (int64_t)surf.size();
-    S->st_fix_grid = (int)std::min<int64_t>(512, (S->st_surf.n + kBlock - 1) / kBlock);
-    if (surf.empty()) surf.push_back(make_int4(0, 0, 0, 0));
+    LS.tiles = {(int)tiles.size(), dt};
+    LS.surf.n = (int64_t)surf.size();
+    LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
+    std::vector<int32_t> rbs;
+    rbs.reserve(9 * surf.size() + 9);
+    for (int r = 0; r < 9; r++) rbs.insert(rbs.end(), rb[r].begin(), rb[r].end());
+    if (surf.empty()) {
+        surf.push_back(make_int4(0, 0, 0, 0));
+        rbs.assign(9, 0);
+    }
+    if (pats.empty()) pats.push_back(make_uint2(0, 0));
     int4 *ds;
+    int32_t *drb;
+    uint2 *dp;
     if (int rc = S->upload(&ds, surf)) return rc;
-    S->st_surf.e = ds;
+    if (int rc = S->upload(&drb, rbs)) return rc;
+    if (int rc = S->upload(&dp, pats)) return rc;
+    LS.surf.e = ds;
+    LS.surf.rb = drb;
+    LS.surf.pat = dp;
+    LS.ok = true;
+    return 0;
+}
+
+int build_stencil(vfem_system *S)
+{
+    for (int l = 0; l < S->H.nlev; l++)
+        if (int rc = build_level_stencil(S, l)) return rc;
+    S->st_ok = S->stl[0].ok;
     return 0;
 }
 
@@ -2458,7 +2555,7 @@ int alloc_vectors(vfem_system *S, int nb)
                 const BlockGrid G = block_grid(S->dl[0], c);
                 cap = std::max<int64_t>(cap, (int64_t)G.tx * G.ty * G.tz);
             }
-        if (S->st_ok) cap = std::max<int64_t>(cap, (int64_t)S->st_tiles.n + S->st_fix_grid);
+        if (S->st_ok) cap = std::max<int64_t>(cap, (int64_t)S->stl[0].tiles.n + S->stl[0].fix_grid);
         S->red.cap = (int)cap;
     }
     if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap * nb, true, V)) return rc;
@@ -2589,32 +2686,52 @@ void launch_matvec_block(vfem_system *S, const float *x, float *y, int fin, int 
     else launch_block_passes<6>(S, x, y, fin, honor_done, c);
 }
 
-template <typename T> const StencilCoef<T> &st_coef(vfem_system *S);
-template <> const StencilCoef<double> &st_coef<double>(vfem_system *S) { return S->st64; }
-template <> const StencilCoef<float> &st_coef<float>(vfem_system *S) { return S->st32; }
+template <typename T> const StencilCoef<T> &st_coef(const LevelStencil &LS);
+template <> const StencilCoef<double> &st_coef<double>(const LevelStencil &LS) { return LS.c64; }
+template <> const StencilCoef<float> &st_coef<float>(const LevelStencil &LS) { return LS.c32; }
 
-// stencil matvec of level 0 plus its surface correction; the reduction adds the
-// partials of both kernels
-template <typename T> void launch_matvec_stencil(vfem_system *S, const T *x, T *y, int fin, int honor_done)
+template <typename T, int ZC>
+void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
+                       int honor_done)
 {
-    const DevLevel &L = S->dl[0];
-    const int nt = S->st_tiles.n;
-    if (S->st_perm == 1)
-        hipLaunchKernelGGL((k_matvec_stencil<T, 1>), dim3(nt, S->ncols), dim3(kBlock), 0, S->stream, L, st_coef<T>(S),
-                           S->st_tiles, x, y, S->st, S->red, fin, honor_done);
+    if (LS.perm == 1)
+        hipLaunchKernelGGL((k_matvec_stencil<T, 1, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                           st_coef<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
     else
-        hipLaunchKernelGGL((k_matvec_stencil<T, 0>), dim3(nt, S->ncols), dim3(kBlock), 0, S->stream, L, st_coef<T>(S),
-                           S->st_tiles, x, y, S->st, S->red, fin, honor_done);
-    if (S->st_fix_grid > 0) {
-        Reduction r2 = S->red;
-        r2.partials += nt;
-        hipLaunchKernelGGL(k_stencil_fix<T>, dim3(S->st_fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L, kefine<T>(S),
-                           S->st_surf, x, y, S->st, r2, fin, honor_done);
-    }
-    enqueue_finalize(S, S->stream, nt + S->st_fix_grid, S->ncols, fin, honor_done);
+        hipLaunchKernelGGL((k_matvec_stencil<T, 0, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                           st_coef<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
 }
 
-bool stencil_on(const vfem_system *S) { return S->st_ok && S->l0_kind != VFEM_L0_ELEMENT; }
+// stencil matvec of level l plus its surface recomputation; the reduction (level
+// 0 only) adds the partials of both kernels
+template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T *x, T *y, int fin, int honor_done)
+{
+    const DevLevel &L = S->dl[l];
+    const LevelStencil &LS = S->stl[l];
+    if (LS.zc == 32) launch_stencil_zc<T, 32>(S, L, LS, x, y, fin, honor_done);
+    else if (LS.zc == 16) launch_stencil_zc<T, 16>(S, L, LS, x, y, fin, honor_done);
+    else launch_stencil_zc<T, 8>(S, L, LS, x, y, fin, honor_done);
+    if (LS.fix_grid > 0) {
+        Reduction r2 = S->red;
+        r2.partials += LS.tiles.n;
+        if (l == 0)
+            hipLaunchKernelGGL((k_stencil_fix<T, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done);
+        else
+            hipLaunchKernelGGL((k_stencil_fix<T, true>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done);
+    }
+    enqueue_finalize(S, S->stream, LS.tiles.n + LS.fix_grid, S->ncols, fin, honor_done);
+}
+
+// level l uses its stencil form: wherever available under VFEM_L0_STENCIL,
+// where it is the faster form under VFEM_L0_AUTO
+bool level_stencil_on(const vfem_system *S, int l)
+{
+    const LevelStencil &LS = S->stl[l];
+    return LS.ok && (S->l0_kind == VFEM_L0_STENCIL || (S->l0_kind == VFEM_L0_AUTO && LS.auto_on));
+}
+bool stencil_on(const vfem_system *S) { return level_stencil_on(S, 0); }
 
 // level-0 matvec.  With the stencil form on: FP64, and FP32 at full accuracy
 // (6 products), go to k_matvec_stencil; the BF16-product variants stay on the
@@ -2628,12 +2745,12 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
     if (stencil_on(S)) {
         if constexpr (sizeof(T) == 8) {
             if (exact || !S->l0_mfma) {
-                launch_matvec_stencil<T>(S, x, y, fin, honor_done);
+                launch_matvec_stencil<T>(S, 0, x, y, fin, honor_done);
                 return;
             }
         } else {
             if (passes == 6) {
-                launch_matvec_stencil<T>(S, x, y, fin, honor_done);
+                launch_matvec_stencil<T>(S, 0, x, y, fin, honor_done);
                 return;
             }
         }
@@ -2693,6 +2810,10 @@ template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x
 {
     if (l == 0) {
         launch_matvec_fine<T>(S, x, y, -1, 1, S->cycle_passes);
+        return;
+    }
+    if (level_stencil_on(S, l)) {
+        launch_matvec_stencil<T>(S, l, x, y, -1, 1);
         return;
     }
     const DevLevel &L = S->dl[l];
@@ -3595,6 +3716,12 @@ int32_t vfem_level0_kernel(const vfem_system *sys)
 {
     if (!sys) return -1;
     return stencil_on(sys) ? VFEM_L0_STENCIL : VFEM_L0_ELEMENT;
+}
+
+int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level)
+{
+    if (!sys || level < 0 || level >= sys->H.nlev) return -1;
+    return level_stencil_on(sys, level) ? VFEM_L0_STENCIL : VFEM_L0_ELEMENT;
 }
 
 int vfem_set_block_config(vfem_system *sys, int32_t index)
