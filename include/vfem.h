@@ -174,7 +174,9 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
  * level: the Galerkin matrix of the full child pattern); VFEM_L0_AUTO does so on
  * level 0 and on coarse levels of at least 100000 nodes (smaller ones are
  * latency-bound and faster in element form); VFEM_L0_ELEMENT uses the element
- * form everywhere.  VFEM_L0_STENCIL fails with
+ * form everywhere.  The damped cycle's restriction and prolongation between
+ * div-2 levels follow the same choice: plane-tiled kernels (VFEM_L0_STENCIL, or
+ * VFEM_L0_AUTO on fine levels of at least 500000 nodes) or row-segment kernels.  VFEM_L0_STENCIL fails with
  * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form.  vfem_level0_kernel
  * returns the form in effect on level 0 and vfem_level_operator_form on `level`
  * (VFEM_L0_ELEMENT or VFEM_L0_STENCIL), or -1 for a null system or a level out

@@ -1769,6 +1769,314 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
     if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
+// ---------------------------------------------------------------------------
+// Tiled transfers between a fine level and its div-2 coarse level.  Both march
+// through planes with the coarser data staged in LDS, so every fine value is
+// read from HBM once (the row-segment kernels above gather each fine node for
+// up to 8 coarse nodes, and each coarse node for up to 8 fine nodes).
+//
+// k_restrict_tile: a block owns 16 x 16 coarse nodes in x-y and ZC coarse
+// planes.  Fine plane f (33 x 33 nodes around the tile) is staged as r - t;
+// each thread forms the 2-D full weighting P_f of its coarse node and adds it
+// with weight 1 (f = 2Z) or 1/2 (f = 2Z +- 1): r_c = sum w(f - 2N) (r_f - t_f)
+// over used fine nodes, the same sum as k_restrict (t_f required).
+//
+// k_prolong_tile (damped cycle, z_f += P z_c): a block owns 32 x 32 fine nodes
+// in x-y and ZC fine planes; coarse planes (17 x 17 nodes) are staged in turn
+// and each thread interpolates the 2 x 2 fine nodes between its four coarse
+// nodes, keeping the previous coarse plane's values for the fine plane between
+// the two.
+// ---------------------------------------------------------------------------
+constexpr int RT_X = 16, RT_Y = 16;                 // coarse nodes per restrict block (x, y)
+constexpr int RF_X = 2 * RT_X + 1;                  // staged fine row (33)
+constexpr int RF_N = RF_X * (2 * RT_Y + 1);         // staged fine nodes (1089)
+constexpr int RF_SLOTS = (RF_N + kBlock - 1) / kBlock;
+constexpr int PC_X = 17, PC_N = PC_X * PC_X;        // staged coarse nodes of a prolong block
+constexpr int PC_SLOTS = (PC_N + kBlock - 1) / kBlock;
+static_assert(RT_X * RT_Y == kBlock, "one coarse column per thread");
+
+// compact id, used and fixed flags of node (gx, gy, gz) of L (clamped loads)
+__device__ __forceinline__ int node_info(const DevLevel &L, int gx, int gy, int gz, bool &used, bool &fixed)
+{
+    const bool in = (unsigned)gx < (unsigned)L.nx && (unsigned)gy < (unsigned)L.ny && (unsigned)gz < (unsigned)L.nz;
+    const int seg = in ? (gz * L.ny + gy) * L.S + (gx >> 6) : 0;
+    const int b = in ? gx & 63 : 0;
+    const uint64_t w = L.nbits[seg], fw = L.fbits[seg];  // unconditional: one round trip
+    const int base = L.nbase[seg];
+    used = in && ((w >> b) & 1);
+    fixed = in && ((fw >> b) & 1);
+    return base + __popcll(w & ((1ull << b) - 1));
+}
+
+template <typename Tf, typename Tt, typename Tc, int ZC>
+__global__ __launch_bounds__(kBlock) void k_restrict_tile(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
+                                                           const Tt *__restrict__ tf, Tc *__restrict__ rc,
+                                                           int maskfixed, const CGState *st, StencilTiles tiles)
+{
+    {
+        const int64_t cf = col_offset(3 * F.nn);
+        rf += cf;
+        tf += cf;
+        rc += col_offset(3 * C.nn);
+        st = col_state(st);
+    }
+    if (st->done) return;
+    __shared__ Tc pl[2][3 * RF_N];
+    const uint32_t tl = tiles.tile[blockIdx.x];
+    const int X0 = (int)(tl & 2047) * RT_X, Y0 = (int)((tl >> 11) & 2047) * RT_Y, Z0 = (int)(tl >> 22) * ZC;
+    const int Z1 = min(Z0 + ZC, C.nz);
+    const int fl = 2 * Z1 - 1;  // last fine plane
+    const int t = threadIdx.x, lx = t % RT_X, ly = t / RT_X;
+    // staged fine slots: segment offset within a plane, bit, inside the grid
+    int so[RF_SLOTS], sb[RF_SLOTS];
+    bool sin[RF_SLOTS];
+#pragma unroll
+    for (int k = 0; k < RF_SLOTS; k++) {
+        const int s = t + k * kBlock;
+        const int ss = s < RF_N ? s : 0;
+        const int gx = 2 * X0 - 1 + ss % RF_X, gy = 2 * Y0 - 1 + ss / RF_X;
+        sin[k] = s < RF_N && (unsigned)gx < (unsigned)F.nx && (unsigned)gy < (unsigned)F.ny;
+        so[k] = sin[k] ? gy * F.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    const int pstride = F.ny * F.S;
+    // two-stage load pipeline: row words of plane f+1 (mw, mbase, min) and raw
+    // values of plane f (vr, vt, vu); all loads are unconditional (clamped ids)
+    // and the selection happens when a plane is staged
+    uint64_t mw[RF_SLOTS];
+    int mbase[RF_SLOTS];
+    bool mn[RF_SLOTS], vu[RF_SLOTS];
+    Tf vr[RF_SLOTS][3];
+    Tt vt[RF_SLOTS][3];
+    auto meta = [&](int f) {
+        const bool zin = (unsigned)f < (unsigned)F.nz && f <= fl;
+#pragma unroll
+        for (int k = 0; k < RF_SLOTS; k++) {
+            const int seg = so[k] + (zin ? f : 0) * pstride;
+            mw[k] = F.nbits[seg];
+            mbase[k] = F.nbase[seg];
+            mn[k] = sin[k] && zin;
+        }
+    };
+    auto values = [&]() {
+#pragma unroll
+        for (int k = 0; k < RF_SLOTS; k++) {
+            const bool used = mn[k] && ((mw[k] >> sb[k]) & 1);
+            const int id = used ? mbase[k] + __popcll(mw[k] & ((1ull << sb[k]) - 1)) : 0;
+            const int64_t q = 3 * (int64_t)id;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                vr[k][c] = rf[q + c];
+                vt[k][c] = tf[q + c];
+            }
+            vu[k] = used;
+        }
+    };
+    meta(2 * Z0 - 1);
+    values();
+    meta(2 * Z0);
+    Tc acc[3] = {0, 0, 0};
+    const int base = (2 * ly + 1) * RF_X + 2 * lx + 1;  // the coarse node's fine position in the staged plane
+    for (int f = 2 * Z0 - 1; f <= fl; f++) {
+        Tc *buf = pl[f & 1];
+#pragma unroll
+        for (int k = 0; k < RF_SLOTS; k++) {
+            const int s = t + k * kBlock;
+            if (s < RF_N) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) buf[3 * s + c] = vu[k] ? Tc(vr[k][c]) - Tc(vt[k][c]) : Tc(0);
+            }
+        }
+        if (f < fl) {
+            values();
+            meta(f + 2);
+        }
+        lds_barrier();
+        Tc p[3] = {0, 0, 0};
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++) {
+                const Tc w = (dx ? Tc(0.5) : Tc(1)) * (dy ? Tc(0.5) : Tc(1));
+                const int q = 3 * (base + dy * RF_X + dx);
+#pragma unroll
+                for (int c = 0; c < 3; c++) p[c] = fma(w, buf[q + c], p[c]);
+            }
+        if (!((f - 2 * Z0) & 1)) {  // f = 2Z: weight 1
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[c] += p[c];
+            continue;
+        }
+        // f = 2Z + 1: weight 1/2 to coarse planes Z and Z + 1
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] = fma(Tc(0.5), p[c], acc[c]);
+        if (f > 2 * Z0 - 1) {
+            const int Z = (f - 1) / 2;  // complete
+            bool used, fixed;
+            const int id = node_info(C, X0 + lx, Y0 + ly, Z, used, fixed);
+            if (used) {
+                const bool zero = maskfixed && fixed;
+                rc[3 * (int64_t)id + 0] = zero ? Tc(0) : acc[0];
+                rc[3 * (int64_t)id + 1] = zero ? Tc(0) : acc[1];
+                rc[3 * (int64_t)id + 2] = zero ? Tc(0) : acc[2];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[c] = Tc(0.5) * p[c];
+    }
+}
+
+template <typename Tc, typename Tz, int ZC>
+__global__ __launch_bounds__(kBlock) void k_prolong_tile(DevLevel F, DevLevel C, const Tc *__restrict__ zc,
+                                                          Tz *__restrict__ zf, int maskfixed, const CGState *st,
+                                                          StencilTiles tiles)
+{
+    {
+        zc += col_offset(3 * C.nn);
+        zf += col_offset(3 * F.nn);
+        st = col_state(st);
+    }
+    if (st->done) return;
+    __shared__ Tz pl[2][3 * PC_N];
+    const uint32_t tl = tiles.tile[blockIdx.x];
+    const int fx0 = (int)(tl & 2047) * 2 * RT_X, fy0 = (int)((tl >> 11) & 2047) * 2 * RT_Y, fz0 = (int)(tl >> 22) * ZC;
+    const int fz1 = min(fz0 + ZC, F.nz);
+    // coarse planes c0 .. c1: fine plane 2c uses plane c, 2c + 1 planes c and c + 1
+    const int X0 = fx0 / 2, Y0 = fy0 / 2, c0 = fz0 / 2, c1 = (fz1 + 1) / 2;
+    const int t = threadIdx.x, lx = t % RT_X, ly = t / RT_X;
+    int so[PC_SLOTS], sb[PC_SLOTS];
+    bool sin[PC_SLOTS];
+#pragma unroll
+    for (int k = 0; k < PC_SLOTS; k++) {
+        const int s = t + k * kBlock;
+        const int ss = s < PC_N ? s : 0;
+        const int gx = X0 + ss % PC_X, gy = Y0 + ss / PC_X;
+        sin[k] = s < PC_N && (unsigned)gx < (unsigned)C.nx && (unsigned)gy < (unsigned)C.ny;
+        so[k] = sin[k] ? gy * C.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    const int pstride = C.ny * C.S;
+    // coarse pipeline as in k_restrict_tile: row words of plane c+1, raw values of plane c
+    uint64_t mw[PC_SLOTS];
+    int mbase[PC_SLOTS];
+    bool mn[PC_SLOTS], vu[PC_SLOTS];
+    Tc vz[PC_SLOTS][3];
+    auto meta = [&](int c) {
+        const bool zin = (unsigned)c < (unsigned)C.nz && c <= c1;
+#pragma unroll
+        for (int k = 0; k < PC_SLOTS; k++) {
+            const int seg = so[k] + (zin ? c : 0) * pstride;
+            mw[k] = C.nbits[seg];
+            mbase[k] = C.nbase[seg];
+            mn[k] = sin[k] && zin;
+        }
+    };
+    auto values = [&]() {
+#pragma unroll
+        for (int k = 0; k < PC_SLOTS; k++) {
+            const bool used = mn[k] && ((mw[k] >> sb[k]) & 1);
+            const int id = used ? mbase[k] + __popcll(mw[k] & ((1ull << sb[k]) - 1)) : 0;
+#pragma unroll
+            for (int q = 0; q < 3; q++) vz[k][q] = zc[3 * (int64_t)id + q];
+            vu[k] = used;
+        }
+    };
+    // fine rows of the thread: (fx0 + 2 lx + i, fy0 + 2 ly + j), i, j in {0, 1};
+    // both x lie in one 64-node segment (fx0 is a multiple of 32)
+    const int fx = fx0 + 2 * lx, fb = fx & 63;
+    const bool fxin = fx < F.nx, fx1in = fx + 1 < F.nx;
+    auto emit = [&](int fz, const Tz (&o)[2][2][3]) {
+        int ids[2][2];
+        bool us[2][2], zs[2][2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int fy = fy0 + 2 * ly + j;
+            const bool in = fxin && fy < F.ny && fz < F.nz;
+            const int seg = in ? (fz * F.ny + fy) * F.S + (fx >> 6) : 0;
+            const uint64_t w = F.nbits[seg], fw = F.fbits[seg];
+            const int b0 = F.nbase[seg] + __popcll(w & ((1ull << fb) - 1));
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const bool u = in && (i == 0 || fx1in) && ((w >> (fb + i)) & 1);
+                us[j][i] = u;
+                zs[j][i] = maskfixed && ((fw >> (fb + i)) & 1);
+                ids[j][i] = u ? b0 + (i ? (int)((w >> fb) & 1) : 0) : 0;
+            }
+        }
+        Tz a[2][2][3];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) a[j][i][q] = zf[3 * (int64_t)ids[j][i] + q];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                if (!us[j][i]) continue;
+                Tz *q = zf + 3 * (int64_t)ids[j][i];
+#pragma unroll
+                for (int c = 0; c < 3; c++) q[c] = zs[j][i] ? Tz(0) : a[j][i][c] + o[j][i][c];
+            }
+    };
+    meta(c0);
+    values();
+    meta(c0 + 1);
+    Tz prev[2][2][3];
+    for (int c = c0; c <= c1; c++) {
+        Tz *buf = pl[c & 1];
+#pragma unroll
+        for (int k = 0; k < PC_SLOTS; k++) {
+            const int s = t + k * kBlock;
+            if (s < PC_N) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) buf[3 * s + q] = vu[k] ? Tz(vz[k][q]) : Tz(0);
+            }
+        }
+        if (c < c1) {
+            values();
+            meta(c + 2);
+        }
+        lds_barrier();
+        // 2-D interpolation of the staged coarse plane at the thread's 2 x 2 fine nodes
+        Tz cur[2][2][3];
+        {
+            const int a = 3 * (ly * PC_X + lx);
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const Tz v00 = buf[a + q], v10 = buf[a + 3 + q], v01 = buf[a + 3 * PC_X + q],
+                         v11 = buf[a + 3 * PC_X + 3 + q];
+                cur[0][0][q] = v00;
+                cur[0][1][q] = Tz(0.5) * (v00 + v10);
+                cur[1][0][q] = Tz(0.5) * (v00 + v01);
+                cur[1][1][q] = Tz(0.25) * ((v00 + v10) + (v01 + v11));
+            }
+        }
+        if (c > c0) {
+            // fine planes 2(c-1) (coarse plane c-1) and 2(c-1)+1 (mean of planes c-1 and c)
+            const int f = 2 * (c - 1);
+            if (f >= fz0 && f < fz1) emit(f, prev);
+            if (f + 1 >= fz0 && f + 1 < fz1) {
+                Tz mid[2][2][3];
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int q = 0; q < 3; q++) mid[j][i][q] = Tz(0.5) * (prev[j][i][q] + cur[j][i][q]);
+                emit(f + 1, mid);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) prev[j][i][q] = cur[j][i][q];
+    }
+}
+
 // Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
 //   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
 // fixed rows zeroed (compact-order fixed bitmask); fin >= 0 reduces r.z (and
@@ -2183,6 +2491,9 @@ BlockGrid block_grid(const DevLevel &L, const BlockCfg &c)
 }  // namespace
 
 constexpr int64_t kStencilMinNodes = 100000;  // smallest coarse level given the stencil form
+constexpr int64_t kTiledMinNodes = 500000;     // smallest fine level given the tiled transfers
+constexpr int kRestrictZ = 8;                  // coarse planes per k_restrict_tile block
+constexpr int kProlongZ = 16;                  // fine planes per k_prolong_tile block
 
 // Stencil form of one level's operator (see k_matvec_stencil).
 struct LevelStencil {
@@ -2241,6 +2552,9 @@ struct vfem_system {
     int l0_kind = VFEM_L0_AUTO; // level-0 operator: assembled stencil (when Ke has its structure) or element form
     bool st_ok = false;         // level 0 has the stencil form (build_stencil)
     LevelStencil stl[kMaxLevels];  // stencil form of each level
+    // tiled transfers between level l and l + 1 (div 2, large fine level)
+    bool tiled[kMaxLevels] = {}, tiled_auto[kMaxLevels] = {};
+    StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -2497,11 +2811,53 @@ int build_level_stencil(vfem_system *S, int l)
     return 0;
 }
 
+// blocks (tx | ty << 11 | tz << 22) of a tiling of level L into bx x by x bz
+// nodes that hold a used node
+int upload_tiles(vfem_system *S, const HostLevel &L, int bx, int by, int bz, StencilTiles &out)
+{
+    const int tx = (L.nx + bx - 1) / bx, ty = (L.ny + by - 1) / by, tz = (L.nz + bz - 1) / bz;
+    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
+    for (int z = 0; z < L.nz; z++)
+        for (int y = 0; y < L.ny; y++)
+            for (int s = 0; s < L.S; s++) {
+                uint64_t w = L.nbits[((size_t)z * L.ny + y) * L.S + s];
+                while (w) {
+                    const int x = 64 * s + __builtin_ctzll(w);
+                    w &= w - 1;
+                    busy[((size_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
+                }
+            }
+    std::vector<uint32_t> tiles;
+    for (int k = 0; k < tz; k++)
+        for (int j = 0; j < ty; j++)
+            for (int i = 0; i < tx; i++)
+                if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
+    if (tiles.empty()) tiles.push_back(0);
+    uint32_t *dt;
+    if (int rc = S->upload(&dt, tiles)) return rc;
+    out = {(int)tiles.size(), dt};
+    return 0;
+}
+
 int build_stencil(vfem_system *S)
 {
-    for (int l = 0; l < S->H.nlev; l++)
+    const Hierarchy &H = S->H;
+    for (int l = 0; l < H.nlev; l++)
         if (int rc = build_level_stencil(S, l)) return rc;
     S->st_ok = S->stl[0].ok;
+    // tiled restriction / prolongation between div-2 levels
+    for (int l = 0; l + 1 < H.nlev; l++) {
+        const HostLevel &F = H.L[l], &C = H.L[l + 1];
+        S->tiled[l] = false;
+        if (C.div != 2) continue;
+        if ((C.nx + RT_X - 1) / RT_X > 2047 || (C.ny + RT_Y - 1) / RT_Y > 2047 || (C.nz + kRestrictZ - 1) / kRestrictZ > 1023 ||
+            (F.nz + kProlongZ - 1) / kProlongZ > 1023)
+            continue;
+        if (int rc = upload_tiles(S, C, RT_X, RT_Y, kRestrictZ, S->rtiles[l])) return rc;
+        if (int rc = upload_tiles(S, F, 2 * RT_X, 2 * RT_Y, kProlongZ, S->ptiles[l])) return rc;
+        S->tiled[l] = true;
+        S->tiled_auto[l] = F.nn >= kTiledMinNodes;  // small levels: the row-segment kernels are faster
+    }
     return 0;
 }
 
@@ -2889,13 +3245,22 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
                        (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
-    hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
-                       (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
+    const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
+    if (tiled)
+        hipLaunchKernelGGL((k_restrict_tile<TR, T, T, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
+                           S->stream, L, C, rl, (const T *)tl, level_r<T>(S, l + 1), 1, S->st, S->rtiles[l]);
+    else
+        hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
+                           (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
     enqueue_vcycle_damped_level<T, T>(S, l + 1, level_r<T>(S, l + 1), level_z<T>(S, l + 1), level_t<T>(S, l + 1),
                                       omega, -1, (const T *)nullptr);
-    hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
-                       (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
-                       -1, (const T *)nullptr);
+    if (tiled)
+        hipLaunchKernelGGL((k_prolong_tile<T, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                           C, (const T *)level_z<T>(S, l + 1), zl, 1, S->st, S->ptiles[l]);
+    else
+        hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
+                           (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
+                           -1, (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
