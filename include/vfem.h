@@ -84,6 +84,8 @@ typedef struct {
     double stall_ratio;      /* learned: fall back to the plain cycle for the rest of the solve when the
                                 residual norm falls by less than this factor per iteration over a
                                 polling interval; default 0.9 */
+    int32_t op_fp64;         /* VFEM_FP32 / VFEM_BF16: 1 = the CG operator (A p) in FP64 with FP64 p, the
+                                cycle in reduced precision; default 0 (operator in the cycle's precision) */
 } vfem_options;
 
 typedef struct {
@@ -215,13 +217,17 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
 
 /* Kernel timing hook (no reference counterpart): runs `reps` launches of one
  * kernel on the library's stream between two hipEvents and returns the mean
- * duration.  which: 0 = level-0 matvec of the CG operator (FP64: VALU kernel;
- * FP32: matrix cores, three-way BF16 split, 6 products; BF16: one BF16 product;
- * MIXED: matrix cores on FP64 input), 1 = one damped V(1,1) cycle (omega 0.6),
- * 2 = one CG iteration with that cycle, 3 = level-0 matvec with the two-way
- * BF16 split (3 products), 4 = plain copy of one FP32 level-0 vector (12 bytes
- * read and 12 written per node: the reference for HBM byte counters), 5 = copy
- * of one FP64 level-0 vector, 6 = FP64 dot product of two level-0 vectors. */
+ * duration.  which: 0 = level-0 matvec of the CG operator in the form chosen by
+ * vfem_set_level0_kernel (stencil form: the stencil kernel and the surface pass;
+ * element form: FP64 exact products and FP32 three-way BF16 splits, 6 products,
+ * on the matrix cores; BF16: one BF16 product; MIXED: matrix cores on FP64
+ * input), 1 = one damped V(1,1) cycle (omega 0.6), 2 = one CG iteration with
+ * that cycle, 3 = level-0 element-form matvec with the two-way BF16 split (3
+ * products), 4 = plain copy of one FP32 level-0 vector (12 bytes read and 12
+ * written per node: the reference for HBM byte counters), 5 = copy of one FP64
+ * level-0 vector, 6 = FP64 dot product of two level-0 vectors, 7 = the level-0
+ * stencil kernel alone, without the surface pass (FP32 or FP64; fails with
+ * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form). */
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);
 
 /* Host-only inspection of the hierarchy vfem_assemble builds (no device call; no

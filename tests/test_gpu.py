@@ -221,6 +221,10 @@ def test_solve_parity(systems, name):
             u, st = A.solve(f, tol=1e-10, precision=prec, cycle=cycle)
             assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
             assert rel(u, u_ref) < 1e-6
+    # FP32 cycle with the CG operator and direction in FP64
+    u, st = A.solve(f, tol=1e-10, precision=V.FP32, cycle=V.CYCLE_DAMPED, op_fp64=1)
+    assert st["converged"] == 1 and st["rel_residual"] < 1e-10, st
+    assert rel(u, u_ref) < 1e-6
 
 
 def test_edge_cases(ke0):

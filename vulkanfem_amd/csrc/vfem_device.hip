@@ -475,6 +475,10 @@ struct StencilTiles {
     int n;
     const uint32_t *tile;
 };
+// Tile lists are padded to a multiple of 8 with kNoTile and read through
+// remap_block(), so each XCD works on a contiguous run of neighbouring tiles
+// and finds their shared halo rows in its own L2.
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
 // Used, non-fixed level-0 nodes with at least one absent element around them:
 // e[i] = {compact id, present, used, 0}: present bit a + 2b + 4c for the element
@@ -564,7 +568,11 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
     }
     if (honor_done && st->done) return;
     __shared__ StNode<T> pl[2][ST_HN];
-    const uint32_t tl = tiles.tile[blockIdx.x];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) {  // padding: no work, but the reduction slot is written
+        if (fin >= 0) block_partials(0.0, red);
+        return;
+    }
     const int x0 = (int)(tl & 2047) * ST_X, y0 = (int)((tl >> 11) & 2047) * ST_Y, z0 = (int)(tl >> 22) * ZC;
     const int z1 = min(z0 + ZC, L.nz);
     const int t = threadIdx.x, lx = t & (ST_X - 1), ly = t / ST_X;
@@ -697,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
     }
     if (honor_done && st->done) return;
     double dacc = 0;
-    for (int64_t base = (int64_t)blockIdx.x * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
+    for (int64_t base = (int64_t)remap_block() * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + threadIdx.x;
         const bool act = i < sl.n;
         const int64_t ic = act ? i : 0;
@@ -1822,7 +1830,8 @@ __global__ __launch_bounds__(kBlock) void k_restrict_tile(DevLevel F, DevLevel C
     }
     if (st->done) return;
     __shared__ Tc pl[2][3 * RF_N];
-    const uint32_t tl = tiles.tile[blockIdx.x];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) return;
     const int X0 = (int)(tl & 2047) * RT_X, Y0 = (int)((tl >> 11) & 2047) * RT_Y, Z0 = (int)(tl >> 22) * ZC;
     const int Z1 = min(Z0 + ZC, C.nz);
     const int fl = 2 * Z1 - 1;  // last fine plane
@@ -1938,7 +1947,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_tile(DevLevel F, DevLevel C,
     }
     if (st->done) return;
     __shared__ Tz pl[2][3 * PC_N];
-    const uint32_t tl = tiles.tile[blockIdx.x];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) return;
     const int fx0 = (int)(tl & 2047) * 2 * RT_X, fy0 = (int)((tl >> 11) & 2047) * 2 * RT_Y, fz0 = (int)(tl >> 22) * ZC;
     const int fz1 = min(fz0 + ZC, F.nz);
     // coarse planes c0 .. c1: fine plane 2c uses plane c, 2c + 1 planes c and c + 1
@@ -2785,6 +2795,7 @@ int build_level_stencil(vfem_system *S, int l)
             for (int i = 0; i < tx; i++)
                 if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
     if (tiles.empty()) tiles.push_back(0);
+    while (tiles.size() % 8) tiles.push_back(kNoTile);
     uint32_t *dt;
     if (int rc = S->upload(&dt, tiles)) return rc;
     LS.tiles = {(int)tiles.size(), dt};
@@ -2833,6 +2844,7 @@ int upload_tiles(vfem_system *S, const HostLevel &L, int bx, int by, int bz, Ste
             for (int i = 0; i < tx; i++)
                 if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
     if (tiles.empty()) tiles.push_back(0);
+    while (tiles.size() % 8) tiles.push_back(kNoTile);
     uint32_t *dt;
     if (int rc = S->upload(&dt, tiles)) return rc;
     out = {(int)tiles.size(), dt};
@@ -3177,12 +3189,15 @@ template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x
                        S->st);
 }
 
+// The explicit inverse is always applied in FP64: its entries carry the coarse
+// matrix's condition number, so an FP32 copy would perturb every reduced-precision
+// cycle by ~cond(Kc) * 2^-24 (the vectors may be FP32; products accumulate in FP64).
 template <typename T> void launch_coarse_solve(vfem_system *S, const T *rc, T *zc, bool gal)
 {
     int n = (int)S->H.nfree;
     if (n == 0) return;
-    hipLaunchKernelGGL((k_coarse_solve<T, T>), dim3((n + kWaves - 1) / kWaves, S->ncols), dim3(kBlock), 0, S->stream,
-                       kinv<T>(S, gal), n, S->freedof, rc, zc, S->st, (int64_t)3 * S->dl[S->H.nlev - 1].nn);
+    hipLaunchKernelGGL((k_coarse_solve<double, T>), dim3((n + kWaves - 1) / kWaves, S->ncols), dim3(kBlock), 0, S->stream,
+                       kinv<double>(S, gal), n, S->freedof, rc, zc, S->st, (int64_t)3 * S->dl[S->H.nlev - 1].nn);
 }
 
 // Reference additive V-cycle (MultigridPreconditioner.hpp:158-193) on the
@@ -3277,6 +3292,7 @@ struct Plan {
     bool mfma64;       // VFEM_MIXED: FP64 vectors, level-0 matvecs on the matrix cores
     bool learned;      // damped cycle with the learned coarse correction
     bool force_flex;   // flexible CG although the cycle is fixed (after a learned phase)
+    bool op64;         // reduced-precision cycle with the CG operator (and p) in FP64
 };
 
 // While alive: level-0 FP64 matvecs of the plan go to the matrix cores (VFEM_MIXED).
@@ -3355,6 +3371,12 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
     if (P.precision == VFEM_FP64)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
                            flexible(S, P) ? S->zold64 : (double *)nullptr, S->st);
+    else if (P.op64 && S->H.nlev == 1)
+        hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
+                           (double *)nullptr, S->st);
+    else if (P.op64)  // FP64 direction from the FP32 preconditioned residual
+        hipLaunchKernelGGL((k_cg_pupdate<float, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p64,
+                           S->zold32, S->st);
     else if (S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
                            (double *)nullptr, S->st);
@@ -3370,9 +3392,9 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
 {
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
-    if (P.precision == VFEM_FP64) {
+    if (P.precision == VFEM_FP64 || P.op64) {
         L0Mode mode(S, P);
-        launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes);
+        launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                            S->p64, S->ap64, S->st, S->red);
         enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
@@ -3398,6 +3420,7 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     P.bf16 = o.precision == VFEM_BF16;
     P.learned = o.learned != 0;
     P.force_flex = false;
+    P.op64 = o.op_fp64 != 0 && P.precision == VFEM_FP32;
     if (P.learned && (!S->unet || o.cycle != VFEM_CYCLE_DAMPED || S->unet_level >= S->H.nlev || S->ncols > 1)) {
         g_last_error = "learned correction needs an attached network, the damped cycle and one right-hand side";
         return VFEM_ERR_UNSUPPORTED;
@@ -3431,6 +3454,7 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     key += (int64_t)(P.mfma64 ? 1 : 0) << 50;
     key += (int64_t)(P.learned ? 1 : 0) << 51;
     key += (int64_t)(P.force_flex ? 1 : 0) << 52;
+    key += (int64_t)(P.op64 ? 1 : 0) << 53;
     auto it = S->graphs.find(key);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -3636,6 +3660,7 @@ void vfem_default_options(vfem_options *o)
     o->cycle_passes = 0;
     o->learned = 0;
     o->stall_ratio = 0.9;
+    o->op_fp64 = 0;
 }
 
 const char *vfem_status_string(int status)
@@ -4224,9 +4249,13 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
 
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms)
 {
-    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 6) {
+    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 7) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
+    }
+    if (which == 7 && !sys->stl[0].ok) {
+        g_last_error = "level 0 has no stencil form";
+        return VFEM_ERR_UNSUPPORTED;
     }
     vfem_options o;
     vfem_default_options(&o);
@@ -4250,6 +4279,17 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
         } else if (which == 5) {
             hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0,
                                sys->stream, sys->n3, sys->p64, sys->ap64);
+        } else if (which == 7) {
+            const LevelStencil &LS = sys->stl[0];
+            if (P.precision == VFEM_FP64) {
+                if (LS.zc == 32) launch_stencil_zc<double, 32>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
+                else if (LS.zc == 16) launch_stencil_zc<double, 16>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
+                else launch_stencil_zc<double, 8>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
+            } else {
+                if (LS.zc == 32) launch_stencil_zc<float, 32>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
+                else if (LS.zc == 16) launch_stencil_zc<float, 16>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
+                else launch_stencil_zc<float, 8>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
+            }
         } else if (which == 6) {
             hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0, sys->stream,
                                sys->n3, sys->t64, sys->ap64, sys->st, sys->red, (int)FIN_STORE, 0);
