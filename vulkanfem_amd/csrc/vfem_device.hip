@@ -559,6 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
                                                             Reduction red, int fin, int honor_done)
 {
     constexpr bool CENTRE = sizeof(T) == 4;
+    constexpr int NSLOT = ST_HN;
     {
         const int64_t co = col_offset(3 * L.nn);
         x += co;
@@ -590,11 +591,14 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
         sb[k] = sin[k] ? gx & 63 : 0;
     }
     // pipeline: row words of plane p+1 (mn), values of plane p (xr, with id/flags)
-    StMeta mn[2];
-    T xr[2][3];
-    int inf[2];
+    // the second slot exists for the first waves only: skip its work wave-uniformly
+    const bool slot1 = (int)(threadIdx.x & ~63u) + kBlock < NSLOT;
+    StMeta mn[2] = {};
+    T xr[2][3] = {};
+    int inf[2] = {0, 0};
 #pragma unroll
     for (int k = 0; k < 2; k++) {
+        if (k == 1 && !slot1) break;
         const StMeta m = st_meta(L, so[k], sin[k], z0 - 1);
         const bool used = m.in && ((m.w >> sb[k]) & 1);
         const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
@@ -611,6 +615,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
         StNode<T> *buf = pl[p & 1];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
             const bool used = inf[k] & 2;
             if (sok[k])
                 buf[k * kBlock + t] = StNode<T>{{used ? xr[k][0] : T(0), used ? xr[k][1] : T(0), used ? xr[k][2] : T(0)},
@@ -619,6 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
         // values of plane p+1, row words of plane p+2
 #pragma unroll
         for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
             const StMeta &m = mn[k];
             const bool used = m.in && ((m.w >> sb[k]) & 1);
             const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
@@ -627,7 +633,10 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
             xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
         }
 #pragma unroll
-        for (int k = 0; k < 2; k++) mn[k] = st_meta(L, so[k], sin[k] && p + 2 <= z1, p + 2);
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
+            mn[k] = st_meta(L, so[k], sin[k] && p + 2 <= z1, p + 2);
+        }
         lds_barrier();
         T v[3][3][3];
         int any = 0, info_c = 0;
@@ -670,7 +679,220 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
             q[0] = r[0];
             q[1] = r[1];
             q[2] = r[2];
-            dacc += (double)rm1[0] * (double)r[0] + (double)rm1[1] * (double)r[1] + (double)rm1[2] * (double)r[2];
+            dacc += (double)fma(rm1[2], r[2], fma(rm1[1], r[1], rm1[0] * r[0]));
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            a0[c] = a1[c];
+            a1[c] = a2[c];
+            a2[c] = 0;
+            rm2[c] = rm1[c];
+            rm1[c] = r0[c];
+        }
+        info_m1 = info_c;
+    }
+    if (fin >= 0) block_partials(dacc, red);
+}
+
+// ---------------------------------------------------------------------------
+// Sum-factorised stencil planes.  When a level's stencil is a sum of tensor
+// products of the 1-D node stencils M = [1 4 1], K = [-1 2 -1] and G = [-1 0 1]
+// (the trilinear element of an orthotropic material with its axes on the grid,
+// the reference's Ke0 included):
+//   S^cc  = sum_e k_ce K_e (x) M (x) M       (K along axis e, M along the others)
+//   S^cc2 = g_cc2 G_a (x) G_b (x) M_t         (a, b: the axes of c and c2, t the third)
+// a plane's contribution needs six 2-D products per input component (MM, KM, MK,
+// GG, GM, MG in x-y), and these factor into a y pass over the lane's own column
+// and an x pass over its row neighbours, exchanged by DPP row shifts (16-lane
+// rows = x rows of the tile).  A block owns 14 x 16 output nodes: its rows span
+// columns x0-1 .. x0+14, so lanes 0 and 15 of each row are the halo columns and
+// only run the y pass.  About 110 VALU per node and plane instead of 176.
+// Centring (FP32) as in k_matvec_stencil: the y pass works on v - r, r the lane's
+// own value on the plane; a neighbour's M-weighted column sum is moved to this
+// lane's reference with 6 (r_neighbour - r) (K and G sum to zero), so every
+// product still works on differences to the output node's own value.
+// ---------------------------------------------------------------------------
+constexpr int SF_X = 14;                // output columns per block
+constexpr int SF_W = 16;                // staged columns (= DPP row)
+constexpr int SF_HN = SF_W * (ST_Y + 2);  // staged nodes per plane (288)
+static_assert(SF_HN <= 2 * kBlock, "two staged slots per thread");
+
+template <typename T> struct StencilTensor {
+    T k[3][3];  // k[c][e]: coefficient of K along axis e in S^cc
+    T g[3][3];  // g[c][c2]: coefficient of S^cc2 (c != c2)
+    T t[3];     // diagonal of T = sum_{dx,dy} S_(dx,dy,1)
+};
+
+// value of lane - 1 (SHR) or lane + 1 (SHL) within the 16-lane row
+constexpr int DPP_SHR1 = 0x111, DPP_SHL1 = 0x101;
+template <int CTRL> __device__ __forceinline__ float dpp_row(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_row(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// the 2-D coupling product of the axis pair {a, b}: 0 = GG ({x,y}, z weight M),
+// 1 = GM ({x,z}, z weight G), 2 = MG ({y,z}, z weight G)
+__host__ __device__ constexpr int sf_pair(int a, int b) { return (a + b == 1) ? 0 : (a + b == 2 ? 1 : 2); }
+
+template <typename T, int PERM, int ZC>
+__global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<T> K, StencilTiles tiles,
+                                                       const T *__restrict__ x, T *__restrict__ y, CGState *st,
+                                                       Reduction red, int fin, int honor_done)
+{
+    constexpr bool CENTRE = sizeof(T) == 4;
+    constexpr int NSLOT = SF_HN;
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
+    if (honor_done && st->done) return;
+    __shared__ StNode<T> pl[2][SF_HN];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) {  // padding: no work, but the reduction slot is written
+        if (fin >= 0) block_partials(0.0, red);
+        return;
+    }
+    const int x0 = (int)(tl & 2047) * SF_X, y0 = (int)((tl >> 11) & 2047) * ST_Y, z0 = (int)(tl >> 22) * ZC;
+    const int z1 = min(z0 + ZC, L.nz);
+    const int t = threadIdx.x, lx = t & (SF_W - 1), ly = t / SF_W;
+    const bool out_col = lx >= 1 && lx <= SF_X;
+    // staged slots s = t + 256 k: column s % 16 (x0-1 ..), row s / 16 (y0-1 ..)
+    int so[2], sb[2];
+    bool sok[2], sin[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int s = t + k * kBlock;
+        sok[k] = s < SF_HN;
+        const int ss = sok[k] ? s : 0;
+        const int gx = x0 - 1 + ss % SF_W, gy = y0 - 1 + ss / SF_W;
+        sin[k] = sok[k] && (unsigned)gx < (unsigned)L.nx && (unsigned)gy < (unsigned)L.ny;
+        so[k] = sin[k] ? gy * L.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    // the second slot exists for the first waves only: skip its work wave-uniformly
+    const bool slot1 = (int)(threadIdx.x & ~63u) + kBlock < NSLOT;
+    StMeta mn[2] = {};
+    T xr[2][3] = {};
+    int inf[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        if (k == 1 && !slot1) break;
+        const StMeta m = st_meta(L, so[k], sin[k], z0 - 1);
+        const bool used = m.in && ((m.w >> sb[k]) & 1);
+        const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
+        inf[k] = id << 2 | (used ? 2 : 0) | ((m.in && ((m.f >> sb[k]) & 1)) ? 1 : 0);
+        const T *q = x + 3 * (int64_t)(used ? id : 0);
+        xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
+        mn[k] = st_meta(L, so[k], sin[k], z0);
+    }
+    T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};  // outputs p-1, p, p+1
+    T rm2[3] = {0, 0, 0}, rm1[3] = {0, 0, 0};                      // own values, planes p-2, p-1
+    int info_m1 = 0;
+    double dacc = 0;
+    for (int p = z0 - 1; p <= z1; p++) {
+        StNode<T> *buf = pl[p & 1];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
+            const bool used = inf[k] & 2;
+            if (sok[k])
+                buf[k * kBlock + t] = StNode<T>{{used ? xr[k][0] : T(0), used ? xr[k][1] : T(0), used ? xr[k][2] : T(0)},
+                                                st_pack(inf[k], T(0))};
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
+            const StMeta &m = mn[k];
+            const bool used = m.in && ((m.w >> sb[k]) & 1);
+            const int id = m.base + __popcll(m.w & ((1ull << sb[k]) - 1));
+            inf[k] = id << 2 | (used ? 2 : 0) | ((m.in && ((m.f >> sb[k]) & 1)) ? 1 : 0);
+            const T *q = x + 3 * (int64_t)(used ? id : 0);
+            xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            if (k == 1 && !slot1) break;
+            mn[k] = st_meta(L, so[k], sin[k] && p + 2 <= z1, p + 2);
+        }
+        lds_barrier();
+        // the lane's own column on this plane: rows y-1, y, y+1
+        const StNode<T> nm = buf[ly * SF_W + lx], n0 = buf[(ly + 1) * SF_W + lx], np = buf[(ly + 2) * SF_W + lx];
+        const int info_c = st_unpack(n0.info);
+        const T r0[3] = {n0.v[0], n0.v[1], n0.v[2]};
+        if (__ballot((st_unpack(nm.info) | info_c | st_unpack(np.info)) & 2)) {
+            T MM[3], KM[3], MK[3], GG[3], GM[3], MG[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                T my, ky, gy, myl, myr, kyl, kyr;
+                gy = np.v[c] - nm.v[c];
+                if constexpr (CENTRE) {
+                    const T s = (nm.v[c] - r0[c]) + (np.v[c] - r0[c]);  // M_y and -K_y of v - r (centre term 0)
+                    const T sl = dpp_row<DPP_SHR1>(s), sr = dpp_row<DPP_SHL1>(s);
+                    const T rl = dpp_row<DPP_SHR1>(r0[c]), rr = dpp_row<DPP_SHL1>(r0[c]);
+                    my = s;
+                    ky = -s;
+                    myl = fma(T(6), rl - r0[c], sl);
+                    myr = fma(T(6), rr - r0[c], sr);
+                    kyl = -sl;
+                    kyr = -sr;
+                } else {
+                    const T s = nm.v[c] + np.v[c];
+                    my = fma(T(4), r0[c], s);
+                    ky = fma(T(2), r0[c], -s);
+                    myl = dpp_row<DPP_SHR1>(my), myr = dpp_row<DPP_SHL1>(my);
+                    kyl = dpp_row<DPP_SHR1>(ky), kyr = dpp_row<DPP_SHL1>(ky);
+                }
+                const T gyl = dpp_row<DPP_SHR1>(gy), gyr = dpp_row<DPP_SHL1>(gy);
+                const T ms = myl + myr;
+                MM[c] = fma(T(4), my, ms);
+                KM[c] = fma(T(2), my, -ms);
+                MK[c] = fma(T(4), ky, kyl + kyr);
+                GG[c] = gyr - gyl;
+                GM[c] = myr - myl;
+                MG[c] = fma(T(4), gy, gyl + gyr);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                T em = fma(K.k[c][0], KM[c], K.k[c][1] * MK[c]);  // z weight M
+                const T dk = K.k[c][2] * MM[c];                      // z weight K
+                T cg = T(0);                                         // z weight G
+#pragma unroll
+                for (int c2 = 0; c2 < 3; c2++) {
+                    if (c2 == c) continue;
+                    const int pr = sf_pair(st_axis(PERM, c), st_axis(PERM, c2));
+                    const T q = pr == 0 ? GG[c2] : (pr == 1 ? GM[c2] : MG[c2]);
+                    if (pr == 0) em = fma(K.g[c][c2], q, em);
+                    else cg = fma(K.g[c][c2], q, cg);
+                }
+                const T f = em - dk;
+                a0[c] += f + cg;                        // output plane p-1: dz = +1
+                a1[c] = fma(T(4), em, fma(T(2), dk, a1[c]));  // dz = 0
+                a2[c] += f - cg;                        // dz = -1
+            }
+        }
+        if (p > z0 && out_col && (info_m1 & 2)) {
+            T r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                r[c] = a0[c];
+                if constexpr (CENTRE) r[c] = fma(K.t[c], fma(T(-2), rm1[c], rm2[c] + r0[c]), r[c]);
+                if (info_m1 & 1) r[c] = T(0);
+            }
+            T *q = y + 3 * (int64_t)(info_m1 >> 2);
+            q[0] = r[0];
+            q[1] = r[1];
+            q[2] = r[2];
+            dacc += (double)fma(rm1[2], r[2], fma(rm1[1], r[1], rm1[0] * r[0]));
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
@@ -2513,6 +2735,9 @@ struct LevelStencil {
     int zc = 32;                // planes per block
     StencilCoef<double> c64{};
     StencilCoef<float> c32{};
+    bool sf = false;            // tensor form: the sum-factorised kernel k_matvec_sf serves the level
+    StencilTensor<double> t64{};
+    StencilTensor<float> t32{};
     StencilTiles tiles{};       // blocks of the stencil kernel
     SurfaceList surf{};         // nodes recomputed by k_stencil_fix
     int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
@@ -2716,6 +2941,75 @@ bool stencil_coef(const double *ke, LevelStencil &LS)
     return false;
 }
 
+// Tensor form of the stencil (see k_matvec_sf): recovers k[c][e] from the centre
+// and the axis neighbours of S^cc and g[c][c2] from the (1, 1) diagonal of the
+// coupling block, and accepts them when every entry of the assembled stencil
+// matches to 1e-13 of the largest.
+bool stencil_tensor(const double *ke, int perm, LevelStencil &LS)
+{
+    double sd[3][3][3][3][3] = {};  // [dz][dy][dx][c][c2]
+    for (int e = 0; e < 8; e++) {
+        const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;
+        const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
+        for (int m = 0; m < 8; m++) {
+            const int dx = a + (m & 1) - 1, dy = b + ((m >> 1) & 1) - 1, dz = c + (m >> 2) - 1;
+            for (int r = 0; r < 3; r++)
+                for (int q = 0; q < 3; q++) sd[dz + 1][dy + 1][dx + 1][r][q] += ke[(3 * Ln + r) * 24 + 3 * m + q];
+        }
+    }
+    auto at = [&](const int d[3], int c, int c2) { return sd[d[2] + 1][d[1] + 1][d[0] + 1][c][c2]; };
+    auto wM = [](int d) { return d ? 1.0 : 4.0; };
+    auto wK = [](int d) { return d ? -1.0 : 2.0; };
+    double k[3][3], g[3][3] = {}, smax = 0;
+    for (int i = 0; i < 243; i++) smax = std::max(smax, std::fabs((&sd[0][0][0][0][0])[i]));
+    for (int c = 0; c < 3; c++) {
+        const int z[3] = {0, 0, 0};
+        const double s = at(z, c, c) / 32.0;  // centre: 2 * 4 * 4 * sum_e k_ce
+        for (int e = 0; e < 3; e++) {
+            int d[3] = {0, 0, 0};
+            d[e] = 1;
+            k[c][e] = (8.0 * s - at(d, c, c)) / 24.0;  // S_e = -16 k_ce + 8 (sum - k_ce)
+        }
+        for (int c2 = 0; c2 < 3; c2++) {
+            if (c2 == c) continue;
+            int d[3] = {0, 0, 0};
+            d[st_axis(perm, c)] = 1;
+            d[st_axis(perm, c2)] = 1;
+            g[c][c2] = at(d, c, c2) / 4.0;  // G(1) G(1) M(0)
+        }
+    }
+    double err = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int d[3] = {dx, dy, dz};
+                for (int c = 0; c < 3; c++)
+                    for (int c2 = 0; c2 < 3; c2++) {
+                        double model = 0;
+                        if (c == c2) {
+                            for (int e = 0; e < 3; e++)
+                                model += k[c][e] * wK(d[e]) * wM(d[(e + 1) % 3]) * wM(d[(e + 2) % 3]);
+                        } else {
+                            const int a = st_axis(perm, c), b = st_axis(perm, c2), t3 = 3 - a - b;
+                            model = g[c][c2] * d[a] * d[b] * wM(d[t3]);
+                        }
+                        err = std::max(err, std::fabs(model - at(d, c, c2)));
+                    }
+            }
+    if (!(smax > 0) || err > 1e-13 * smax) return false;
+    for (int c = 0; c < 3; c++) {
+        for (int e = 0; e < 3; e++) {
+            LS.t64.k[c][e] = k[c][e];
+            LS.t64.g[c][e] = g[c][e];
+            LS.t32.k[c][e] = (float)k[c][e];
+            LS.t32.g[c][e] = (float)g[c][e];
+        }
+        LS.t64.t[c] = LS.c64.t[c];
+        LS.t32.t[c] = LS.c32.t[c];
+    }
+    return true;
+}
+
 // Stencil form of level l (see k_matvec_stencil): the structured values of the
 // level's full-element matrix (Ke at level 0, the Galerkin matrix of the full
 // child pattern on a div-2 coarse level), the blocks that own used nodes and the
@@ -2738,8 +3032,10 @@ int build_level_stencil(vfem_system *S, int l)
     LS.auto_on = !coarse || L.nn >= kStencilMinNodes;
     const double *kfull = coarse ? H.ktab[l].data() + 255 * 576 : H.ke;
     if (!stencil_coef(kfull, LS)) return 0;
+    LS.sf = stencil_tensor(kfull, LS.perm, LS);
+    const int tw = LS.sf ? SF_X : ST_X;  // output columns per block
     LS.zc = L.nz >= 192 ? 32 : (L.nz >= 96 ? 16 : 8);
-    const int tx = (L.nx + ST_X - 1) / ST_X, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + LS.zc - 1) / LS.zc;
+    const int tx = (L.nx + tw - 1) / tw, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + LS.zc - 1) / LS.zc;
     if (tx > 2047 || ty > 2047 || tz > 1023) return 0;
     std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
     std::vector<int4> surf;
@@ -2757,7 +3053,7 @@ int build_level_stencil(vfem_system *S, int l)
             for (int x = 0; x < L.nx; x++) {
                 const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
                 if (!((L.nbits[seg] >> (x & 63)) & 1)) continue;
-                busy[((size_t)(z / LS.zc) * ty + y / ST_Y) * tx + x / ST_X] = 1;
+                busy[((size_t)(z / LS.zc) * ty + y / ST_Y) * tx + x / tw] = 1;
                 const bool fixed = (L.fbits[seg] >> (x & 63)) & 1;
                 unsigned present = 0, full = 0;
                 uint32_t pw[2] = {0, 0};
@@ -3058,10 +3354,23 @@ template <typename T> const StencilCoef<T> &st_coef(const LevelStencil &LS);
 template <> const StencilCoef<double> &st_coef<double>(const LevelStencil &LS) { return LS.c64; }
 template <> const StencilCoef<float> &st_coef<float>(const LevelStencil &LS) { return LS.c32; }
 
+template <typename T> const StencilTensor<T> &st_tensor(const LevelStencil &LS);
+template <> const StencilTensor<double> &st_tensor<double>(const LevelStencil &LS) { return LS.t64; }
+template <> const StencilTensor<float> &st_tensor<float>(const LevelStencil &LS) { return LS.t32; }
+
 template <typename T, int ZC>
 void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
                        int honor_done)
 {
+    if (LS.sf) {
+        if (LS.perm == 1)
+            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               st_tensor<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+        else
+            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               st_tensor<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+        return;
+    }
     if (LS.perm == 1)
         hipLaunchKernelGGL((k_matvec_stencil<T, 1, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
                            st_coef<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
