@@ -147,6 +147,7 @@ def main():
     if args.profile_kernels:
         for which, prec_k, form, name in (
                 (7, V.FP32, V.L0_AUTO, "stencil kernel fp32"), (0, V.FP32, V.L0_AUTO, "operator fp32 (stencil + surface)"),
+                (7, V.FP32, V.L0_STENCIL_DIRECT, "structured (not sum-factorised) stencil kernel fp32"),
                 (7, V.FP64, V.L0_AUTO, "stencil kernel fp64"), (0, V.FP64, V.L0_AUTO, "operator fp64 (stencil + surface)"),
                 (0, V.BF16, V.L0_AUTO, "element form bf16 1-pass (matrix cores)"),
                 (0, V.FP32, V.L0_ELEMENT, "element form fp32 6-pass (matrix cores)"),
@@ -225,7 +226,7 @@ def main():
         out_extra.update(learned_solve(A, p.loads[0], nlev))
     if not args.no_batched:
         out_extra.update(batched_solves())
-    kernel = "k_matvec_stencil<float,1,32>"
+    kernel = "k_matvec_sf<float,1,32>"
     traffic = measured_traffic(kernel, p.name)
     cpu = None
     if not args.no_cpu and "ref_cycle_fp64_iterations" in out_extra:
@@ -248,8 +249,8 @@ def main():
                    "cycle": "damped V(1,1), omega 0.6", "tol": 1e-8, "true_res_every": 20},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": kernel + " (level-0 stencil matvec of the CG operator and the cycle; its surface "
-                               "pass k_stencil_fix runs after it)",
+                     "kernel": kernel + " (level-0 sum-factorised stencil matvec of the CG operator and the cycle; "
+                               "its surface pass k_stencil_fix runs after it)",
                      "algorithmic_bytes_per_launch": b32},
         "cpu_baseline": cpu,
     }

@@ -58,7 +58,10 @@ enum vfem_precision {
 enum vfem_level0 {
     VFEM_L0_AUTO = 0,     /* the stencil form where it is available and faster (default) */
     VFEM_L0_ELEMENT = 1,  /* element form: Ke times each element's 24 values, on the matrix cores */
-    VFEM_L0_STENCIL = 2   /* assembled 27-point stencil plus a correction of the surface nodes */
+    VFEM_L0_STENCIL = 2,  /* assembled 27-point stencil plus a recomputation of the surface nodes;
+                             sum-factorised planes when the stencil is a tensor-product form */
+    VFEM_L0_STENCIL_DIRECT = 3  /* as VFEM_L0_STENCIL with the structured 27-point kernel (no sum
+                                   factorisation), for comparison */
 };
 
 /* multigrid cycle */
@@ -171,18 +174,25 @@ int vfem_get_solution(vfem_system *sys, double *u);
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision);
 
 /* Operator form (enum vfem_level0; no reference counterpart, whose MatxVec.comp
- * is the element form): VFEM_L0_STENCIL uses the assembled stencil on every level
- * whose full-element matrix has its structure (level 0: Ke; a damped-cycle coarse
- * level: the Galerkin matrix of the full child pattern); VFEM_L0_AUTO does so on
- * level 0 and on coarse levels of at least 100000 nodes (smaller ones are
- * latency-bound and faster in element form); VFEM_L0_ELEMENT uses the element
- * form everywhere.  The damped cycle's restriction and prolongation between
- * div-2 levels follow the same choice: plane-tiled kernels (VFEM_L0_STENCIL, or
- * VFEM_L0_AUTO on fine levels of at least 500000 nodes) or row-segment kernels.  VFEM_L0_STENCIL fails with
- * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form.  vfem_level0_kernel
- * returns the form in effect on level 0 and vfem_level_operator_form on `level`
- * (VFEM_L0_ELEMENT or VFEM_L0_STENCIL), or -1 for a null system or a level out
- * of range. */
+ * is the element form).
+ *   VFEM_L0_STENCIL: the assembled stencil on every level whose full-element
+ *     matrix has its structure (level 0: Ke; a damped-cycle coarse level: the
+ *     Galerkin matrix of the full child pattern), sum-factorised where it is a
+ *     tensor-product form.
+ *   VFEM_L0_STENCIL_DIRECT: as VFEM_L0_STENCIL with the structured 27-point
+ *     kernel everywhere (no sum factorisation), for comparison.
+ *   VFEM_L0_AUTO (default): the stencil form on level 0 and on coarse levels of
+ *     at least 100000 nodes; smaller levels are latency-bound and faster in
+ *     element form.
+ *   VFEM_L0_ELEMENT: the element form everywhere.
+ * The damped cycle's restriction and prolongation between div-2 levels follow
+ * the same choice: plane-tiled kernels (both stencil kinds, or VFEM_L0_AUTO on
+ * fine levels of at least 500000 nodes) or row-segment kernels.  The stencil
+ * kinds fail with VFEM_ERR_UNSUPPORTED when level 0 has no stencil form.
+ * vfem_level0_kernel returns the form in effect on level 0 and
+ * vfem_level_operator_form on `level` (VFEM_L0_ELEMENT, VFEM_L0_STENCIL for the
+ * sum-factorised kernel or VFEM_L0_STENCIL_DIRECT for the structured one), or -1
+ * for a null system or a level out of range. */
 int vfem_set_level0_kernel(vfem_system *sys, int32_t kind);
 int32_t vfem_level0_kernel(const vfem_system *sys);
 int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level);

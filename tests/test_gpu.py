@@ -68,24 +68,25 @@ def test_stencil_form_matches_element_form(systems, name):
     """The assembled-stencil level-0 operator plus its surface correction against
     the element form and the oracle (FP64 to rounding, FP32 to single precision)."""
     A, ora, _, _ = systems[name]
-    assert A.level0_kernel == V.L0_STENCIL
+    assert A.level0_kernel == V.L0_STENCIL  # Ke0's stencil has the tensor form: sum-factorised kernel
     x = np.random.default_rng(5).standard_normal(A.n)
     ref = ora.matvec(x)
     out = {}
     try:
-        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT):
             A.set_level0_kernel(kind)
             assert A.level0_kernel == kind
             out[kind] = (A.matvec(x, V.FP64), A.matvec(x, V.FP32))
     finally:
         A.set_level0_kernel(V.L0_AUTO)
-    s64, s32 = out[V.L0_STENCIL]
     e64, e32 = out[V.L0_ELEMENT]
-    say("stencil %s: fp64 %.2e (element %.2e), fp32 %.2e (element %.2e)"
-        % (name, rel(s64, ref), rel(e64, ref), rel(s32, ref), rel(e32, ref)))
-    assert rel(s64, ref) < 1e-14
-    assert rel(s64, e64) < 1e-14
-    assert rel(s32, ref) < 2e-6
+    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+        s64, s32 = out[kind]
+        say("stencil kind %d %s: fp64 %.2e (element %.2e), fp32 %.2e (element %.2e)"
+            % (kind, name, rel(s64, ref), rel(e64, ref), rel(s32, ref), rel(e32, ref)))
+        assert rel(s64, ref) < 1e-14
+        assert rel(s64, e64) < 1e-14
+        assert rel(s32, ref) < 2e-6
 
 
 @pytest.mark.parametrize("name", ["ell24", "ell33"])
@@ -102,19 +103,20 @@ def test_stencil_cycle_matches_element_cycle(systems, name):
     z_d = ora.vcycle(r, O.DAMPED, 0.6)
     out = {}
     try:
-        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT):
             A.set_level0_kernel(kind)
-            if kind == V.L0_STENCIL:  # every div-2 level with full elements
-                assert all(A.level_operator_form(l) == V.L0_STENCIL for l in range(A.num_levels - 1))
+            if kind != V.L0_ELEMENT:  # every div-2 level with full elements
+                assert all(A.level_operator_form(l) == kind for l in range(A.num_levels - 1))
             out[kind] = (V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r),
                          V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.FP32).solve(r))
     finally:
         A.set_level0_kernel(V.L0_AUTO)
     say("cycle %s: fp64 %.2e fp32 %.2e (element fp32 %.2e)" % (name, rel(out[2][0], z_d), rel(out[2][1], z_d),
                                                                rel(out[1][1], z_d)))
-    assert rel(out[V.L0_STENCIL][0], z_d) < 1e-12
-    assert rel(out[V.L0_STENCIL][0], out[V.L0_ELEMENT][0]) < 1e-12
-    assert rel(out[V.L0_STENCIL][1], z_d) < 1e-5
+    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+        assert rel(out[kind][0], z_d) < 1e-12
+        assert rel(out[kind][0], out[V.L0_ELEMENT][0]) < 1e-12
+        assert rel(out[kind][1], z_d) < 1e-5
 
 
 def test_unstructured_ke_uses_element_form(ke0):

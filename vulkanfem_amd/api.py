@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "vfem.h")
 
 FP64, FP32, BF16, MIXED = 0, 1, 2, 3
 CYCLE_REFERENCE, CYCLE_DAMPED = 0, 1
-L0_AUTO, L0_ELEMENT, L0_STENCIL = 0, 1, 2
+L0_AUTO, L0_ELEMENT, L0_STENCIL, L0_STENCIL_DIRECT = 0, 1, 2, 3
 
 OK, ERR_ARGUMENT, ERR_EMPTY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_UNSUPPORTED = range(8)
 
@@ -359,16 +359,16 @@ class MatrixFreeSparse:
         return U
 
     def set_level0_kernel(self, kind):
-        """Level-0 operator form: L0_AUTO, L0_ELEMENT or L0_STENCIL (vfem_set_level0_kernel)."""
+        """Operator form: L0_AUTO, L0_ELEMENT, L0_STENCIL or L0_STENCIL_DIRECT (vfem_set_level0_kernel)."""
         _check(self._lib.vfem_set_level0_kernel(self._h, int(kind)))
 
     @property
     def level0_kernel(self):
-        """Level-0 form in effect: L0_ELEMENT or L0_STENCIL."""
+        """Level-0 form in effect: L0_ELEMENT, L0_STENCIL (sum-factorised) or L0_STENCIL_DIRECT."""
         return int(self._lib.vfem_level0_kernel(self._h))
 
     def level_operator_form(self, level):
-        """Operator form in effect on `level`: L0_ELEMENT or L0_STENCIL."""
+        """Operator form in effect on `level`: L0_ELEMENT, L0_STENCIL or L0_STENCIL_DIRECT."""
         return int(self._lib.vfem_level_operator_form(self._h, int(level)))
 
     def matvec_block(self, X, precision=FP64):

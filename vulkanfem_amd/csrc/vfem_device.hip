@@ -2738,7 +2738,8 @@ struct LevelStencil {
     bool sf = false;            // tensor form: the sum-factorised kernel k_matvec_sf serves the level
     StencilTensor<double> t64{};
     StencilTensor<float> t32{};
-    StencilTiles tiles{};       // blocks of the stencil kernel
+    StencilTiles tiles{};       // blocks of k_matvec_stencil (16 columns)
+    StencilTiles tiles_sf{};    // blocks of k_matvec_sf (14 columns)
     SurfaceList surf{};         // nodes recomputed by k_stencil_fix
     int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
 };
@@ -2871,6 +2872,35 @@ int build_schedule(vfem_system *S)
     T.off = doff;
     T.list = dlist;
     S->tiles0 = T;
+    return 0;
+}
+
+// blocks (tx | ty << 11 | tz << 22) of a tiling of level L into bx x by x bz
+// nodes that hold a used node
+int upload_tiles(vfem_system *S, const HostLevel &L, int bx, int by, int bz, StencilTiles &out)
+{
+    const int tx = (L.nx + bx - 1) / bx, ty = (L.ny + by - 1) / by, tz = (L.nz + bz - 1) / bz;
+    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
+    for (int z = 0; z < L.nz; z++)
+        for (int y = 0; y < L.ny; y++)
+            for (int s = 0; s < L.S; s++) {
+                uint64_t w = L.nbits[((size_t)z * L.ny + y) * L.S + s];
+                while (w) {
+                    const int x = 64 * s + __builtin_ctzll(w);
+                    w &= w - 1;
+                    busy[((size_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
+                }
+            }
+    std::vector<uint32_t> tiles;
+    for (int k = 0; k < tz; k++)
+        for (int j = 0; j < ty; j++)
+            for (int i = 0; i < tx; i++)
+                if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
+    if (tiles.empty()) tiles.push_back(0);
+    while (tiles.size() % 8) tiles.push_back(kNoTile);
+    uint32_t *dt;
+    if (int rc = S->upload(&dt, tiles)) return rc;
+    out = {(int)tiles.size(), dt};
     return 0;
 }
 
@@ -3033,11 +3063,12 @@ int build_level_stencil(vfem_system *S, int l)
     const double *kfull = coarse ? H.ktab[l].data() + 255 * 576 : H.ke;
     if (!stencil_coef(kfull, LS)) return 0;
     LS.sf = stencil_tensor(kfull, LS.perm, LS);
-    const int tw = LS.sf ? SF_X : ST_X;  // output columns per block
     LS.zc = L.nz >= 192 ? 32 : (L.nz >= 96 ? 16 : 8);
-    const int tx = (L.nx + tw - 1) / tw, ty = (L.ny + ST_Y - 1) / ST_Y, tz = (L.nz + LS.zc - 1) / LS.zc;
-    if (tx > 2047 || ty > 2047 || tz > 1023) return 0;
-    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
+    // tile indices are packed in 11 + 11 + 10 bits
+    if ((L.nx + SF_X - 1) / SF_X > 2047 || (L.ny + ST_Y - 1) / ST_Y > 2047 || (L.nz + LS.zc - 1) / LS.zc > 1023) return 0;
+    if (int rc = upload_tiles(S, L, ST_X, ST_Y, LS.zc, LS.tiles)) return rc;
+    if (LS.sf)
+        if (int rc = upload_tiles(S, L, SF_X, ST_Y, LS.zc, LS.tiles_sf)) return rc;
     std::vector<int4> surf;
     std::vector<uint2> pats;
     std::vector<int32_t> rb[9];
@@ -3053,7 +3084,6 @@ int build_level_stencil(vfem_system *S, int l)
             for (int x = 0; x < L.nx; x++) {
                 const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
                 if (!((L.nbits[seg] >> (x & 63)) & 1)) continue;
-                busy[((size_t)(z / LS.zc) * ty + y / ST_Y) * tx + x / tw] = 1;
                 const bool fixed = (L.fbits[seg] >> (x & 63)) & 1;
                 unsigned present = 0, full = 0;
                 uint32_t pw[2] = {0, 0};
@@ -3085,16 +3115,6 @@ int build_level_stencil(vfem_system *S, int l)
                 }
                 id++;
             }
-    std::vector<uint32_t> tiles;
-    for (int k = 0; k < tz; k++)
-        for (int j = 0; j < ty; j++)
-            for (int i = 0; i < tx; i++)
-                if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
-    if (tiles.empty()) tiles.push_back(0);
-    while (tiles.size() % 8) tiles.push_back(kNoTile);
-    uint32_t *dt;
-    if (int rc = S->upload(&dt, tiles)) return rc;
-    LS.tiles = {(int)tiles.size(), dt};
     LS.surf.n = (int64_t)surf.size();
     LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
     std::vector<int32_t> rbs;
@@ -3115,35 +3135,6 @@ int build_level_stencil(vfem_system *S, int l)
     LS.surf.rb = drb;
     LS.surf.pat = dp;
     LS.ok = true;
-    return 0;
-}
-
-// blocks (tx | ty << 11 | tz << 22) of a tiling of level L into bx x by x bz
-// nodes that hold a used node
-int upload_tiles(vfem_system *S, const HostLevel &L, int bx, int by, int bz, StencilTiles &out)
-{
-    const int tx = (L.nx + bx - 1) / bx, ty = (L.ny + by - 1) / by, tz = (L.nz + bz - 1) / bz;
-    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
-    for (int z = 0; z < L.nz; z++)
-        for (int y = 0; y < L.ny; y++)
-            for (int s = 0; s < L.S; s++) {
-                uint64_t w = L.nbits[((size_t)z * L.ny + y) * L.S + s];
-                while (w) {
-                    const int x = 64 * s + __builtin_ctzll(w);
-                    w &= w - 1;
-                    busy[((size_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
-                }
-            }
-    std::vector<uint32_t> tiles;
-    for (int k = 0; k < tz; k++)
-        for (int j = 0; j < ty; j++)
-            for (int i = 0; i < tx; i++)
-                if (busy[((size_t)k * ty + j) * tx + i]) tiles.push_back((uint32_t)i | (uint32_t)j << 11 | (uint32_t)k << 22);
-    if (tiles.empty()) tiles.push_back(0);
-    while (tiles.size() % 8) tiles.push_back(kNoTile);
-    uint32_t *dt;
-    if (int rc = S->upload(&dt, tiles)) return rc;
-    out = {(int)tiles.size(), dt};
     return 0;
 }
 
@@ -3219,7 +3210,8 @@ int alloc_vectors(vfem_system *S, int nb)
                 const BlockGrid G = block_grid(S->dl[0], c);
                 cap = std::max<int64_t>(cap, (int64_t)G.tx * G.ty * G.tz);
             }
-        if (S->st_ok) cap = std::max<int64_t>(cap, (int64_t)S->stl[0].tiles.n + S->stl[0].fix_grid);
+        if (S->st_ok)
+            cap = std::max<int64_t>(cap, (int64_t)std::max(S->stl[0].tiles.n, S->stl[0].tiles_sf.n) + S->stl[0].fix_grid);
         S->red.cap = (int)cap;
     }
     if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap * nb, true, V)) return rc;
@@ -3358,17 +3350,25 @@ template <typename T> const StencilTensor<T> &st_tensor(const LevelStencil &LS);
 template <> const StencilTensor<double> &st_tensor<double>(const LevelStencil &LS) { return LS.t64; }
 template <> const StencilTensor<float> &st_tensor<float>(const LevelStencil &LS) { return LS.t32; }
 
+// the sum-factorised kernel serves a level with the tensor form unless the
+// structured-stencil kernel is asked for
+bool use_sf(const vfem_system *S, const LevelStencil &LS) { return LS.sf && S->l0_kind != VFEM_L0_STENCIL_DIRECT; }
+const StencilTiles &stencil_tiles(const vfem_system *S, const LevelStencil &LS)
+{
+    return use_sf(S, LS) ? LS.tiles_sf : LS.tiles;
+}
+
 template <typename T, int ZC>
 void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
                        int honor_done)
 {
-    if (LS.sf) {
+    if (use_sf(S, LS)) {
         if (LS.perm == 1)
-            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
-                               st_tensor<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC>), dim3(LS.tiles_sf.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               st_tensor<T>(LS), LS.tiles_sf, x, y, S->st, S->red, fin, honor_done);
         else
-            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
-                               st_tensor<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC>), dim3(LS.tiles_sf.n, S->ncols), dim3(kBlock), 0, S->stream, L,
+                               st_tensor<T>(LS), LS.tiles_sf, x, y, S->st, S->red, fin, honor_done);
         return;
     }
     if (LS.perm == 1)
@@ -3390,7 +3390,7 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
     else launch_stencil_zc<T, 8>(S, L, LS, x, y, fin, honor_done);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
-        r2.partials += LS.tiles.n;
+        r2.partials += stencil_tiles(S, LS).n;
         if (l == 0)
             hipLaunchKernelGGL((k_stencil_fix<T, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
                                kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done);
@@ -3398,7 +3398,7 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
             hipLaunchKernelGGL((k_stencil_fix<T, true>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
                                ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done);
     }
-    enqueue_finalize(S, S->stream, LS.tiles.n + LS.fix_grid, S->ncols, fin, honor_done);
+    enqueue_finalize(S, S->stream, stencil_tiles(S, LS).n + LS.fix_grid, S->ncols, fin, honor_done);
 }
 
 // level l uses its stencil form: wherever available under VFEM_L0_STENCIL,
@@ -3406,7 +3406,8 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
 bool level_stencil_on(const vfem_system *S, int l)
 {
     const LevelStencil &LS = S->stl[l];
-    return LS.ok && (S->l0_kind == VFEM_L0_STENCIL || (S->l0_kind == VFEM_L0_AUTO && LS.auto_on));
+    return LS.ok && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
+                     (S->l0_kind == VFEM_L0_AUTO && LS.auto_on));
 }
 bool stencil_on(const vfem_system *S) { return level_stencil_on(S, 0); }
 
@@ -3569,7 +3570,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                        (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
                        (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
-    const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
+    const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
+                                       (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
     if (tiled)
         hipLaunchKernelGGL((k_restrict_tile<TR, T, T, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
                            S->stream, L, C, rl, (const T *)tl, level_r<T>(S, l + 1), 1, S->st, S->rtiles[l]);
@@ -4394,11 +4396,11 @@ int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz)
 
 int vfem_set_level0_kernel(vfem_system *sys, int32_t kind)
 {
-    if (!sys || kind < VFEM_L0_AUTO || kind > VFEM_L0_STENCIL) {
+    if (!sys || kind < VFEM_L0_AUTO || kind > VFEM_L0_STENCIL_DIRECT) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
-    if (kind == VFEM_L0_STENCIL && !sys->st_ok) {
+    if ((kind == VFEM_L0_STENCIL || kind == VFEM_L0_STENCIL_DIRECT) && !sys->st_ok) {
         g_last_error = "Ke does not assemble to the structured level-0 stencil";
         return VFEM_ERR_UNSUPPORTED;
     }
@@ -4411,17 +4413,19 @@ int vfem_set_level0_kernel(vfem_system *sys, int32_t kind)
     return VFEM_OK;
 }
 
-int32_t vfem_level0_kernel(const vfem_system *sys)
-{
-    if (!sys) return -1;
-    return stencil_on(sys) ? VFEM_L0_STENCIL : VFEM_L0_ELEMENT;
-}
-
 int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level)
 {
     if (!sys || level < 0 || level >= sys->H.nlev) return -1;
-    return level_stencil_on(sys, level) ? VFEM_L0_STENCIL : VFEM_L0_ELEMENT;
+    if (!level_stencil_on(sys, level)) return VFEM_L0_ELEMENT;
+    return use_sf(sys, sys->stl[level]) ? VFEM_L0_STENCIL : VFEM_L0_STENCIL_DIRECT;
 }
+
+int32_t vfem_level0_kernel(const vfem_system *sys)
+{
+    if (!sys) return -1;
+    return vfem_level_operator_form(sys, 0);
+}
+
 
 int vfem_set_block_config(vfem_system *sys, int32_t index)
 {
