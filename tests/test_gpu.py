@@ -315,6 +315,25 @@ def test_batched_solve_matches_single_solves(systems, prec):
             assert rel(U[k], u) < 1e-6
 
 
+def test_batched_fp64_fused_smoothing_matches_single_solves(systems):
+    """Regression: the FP64 damped cycle with the post-smoothing fused into the
+    level-0 stencil (forced on this small case by L0_STENCIL) over several columns
+    -- every column bitwise equal to its single solve, D^-1 shared by all columns."""
+    A, ora, vox, fixed = systems["ell24"]
+    F = problems.patch_load_cases(vox, 3, seed=13, patch=4)
+    F[:, np.repeat(ora.fixed(0), 3)] = 0
+    try:
+        A.set_level0_kernel(V.L0_STENCIL)
+        U, stats = A.solve_batched(F, tol=1e-9, precision=V.FP64, cycle=V.CYCLE_DAMPED)
+        for k in range(3):
+            u, st = A.solve(F[k], tol=1e-9, precision=V.FP64, cycle=V.CYCLE_DAMPED)
+            say("fused column", k, stats[k]["iterations"], st["iterations"])
+            assert stats[k]["converged"] == 1 and stats[k]["iterations"] == st["iterations"]
+            assert np.array_equal(U[k], u)
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
+
+
 @pytest.mark.parametrize("cfg", range(len(V.api.block_configs()) if V.api.os.path.exists(V.api.LIB_PATH) else 0))
 def test_block_matvec_parity(systems, cfg):
     """Every tile shape of the multi-RHS matrix-core matvec against the oracle,

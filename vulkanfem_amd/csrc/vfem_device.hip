@@ -763,9 +763,8 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
         const int64_t co = col_offset(3 * L.nn);
         x += co;
         y += co;
-        if constexpr (EPI) {
+        if constexpr (EPI) {  // D^-1 is one array shared by all columns
             ep.r += co;
-            ep.dinv += co;
             if (ep.zold) ep.zold += co;
             ep.z += co;
         }
@@ -970,9 +969,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
         const int64_t co = col_offset(3 * L.nn);
         x += co;
         y += co;
-        if constexpr (EPI) {
+        if constexpr (EPI) {  // D^-1 is one array shared by all columns
             ep.r += co;
-            ep.dinv += co;
             ep.z += co;
         }
         st = col_state(st);
