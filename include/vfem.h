@@ -89,6 +89,8 @@ typedef struct {
                                 polling interval; default 0.9 */
     int32_t op_fp64;         /* VFEM_FP32 / VFEM_BF16: 1 = the CG operator (A p) in FP64 with FP64 p, the
                                 cycle in reduced precision; default 0 (operator in the cycle's precision) */
+    int32_t coarse_fp64;     /* VFEM_FP32 / VFEM_BF16 with the damped cycle: 1 = only level 0 of the cycle
+                                in reduced precision, the coarser levels in FP64; default 0 */
 } vfem_options;
 
 typedef struct {

@@ -40,7 +40,8 @@ class Options(ctypes.Structure):
                 ("cycle", ctypes.c_int32), ("omega", ctypes.c_double), ("true_res_every", ctypes.c_int32),
                 ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32),
                 ("op_passes", ctypes.c_int32), ("cycle_passes", ctypes.c_int32), ("learned", ctypes.c_int32),
-                ("stall_ratio", ctypes.c_double), ("op_fp64", ctypes.c_int32)]
+                ("stall_ratio", ctypes.c_double), ("op_fp64", ctypes.c_int32),
+                ("coarse_fp64", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

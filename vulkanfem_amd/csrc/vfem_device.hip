@@ -2506,6 +2506,61 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
     block_partials(acc, red);
 }
 
+// k_cg_update fused with the damped cycle's level-0 pre-smoothing, which reads
+// the residual it has just written: z = omega D^-1 r (fixed rows 0).
+template <typename Tp, typename Ta, typename Tz>
+__global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ x, double *__restrict__ r,
+                                                              const Tp *__restrict__ p, const Ta *__restrict__ ap,
+                                                              const uint32_t *__restrict__ fixc,
+                                                              const Tz *__restrict__ dinv, Tz *__restrict__ z,
+                                                              double omega, CGState *st, Reduction red)
+{
+    {
+        const int64_t co = col_offset(n);
+        x += co;
+        r += co;
+        p += co;
+        ap += co;
+        z += co;
+        st = col_state(st);
+        red = col_red(red);
+    }
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const Tz om = Tz(omega);
+    const Range R = flat_range(n);
+    double acc = 0;
+    for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
+        double xv[FLAT_U], rv[FLAT_U];
+        Tp pv[FLAT_U];
+        Ta av[FLAT_U];
+        Tz dv[FLAT_U];
+        uint32_t fw[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = min(i0 + u * kBlock, R.end - 1);
+            xv[u] = x[i];
+            rv[u] = r[i];
+            pv[u] = p[i];
+            av[u] = ap[i];
+            dv[u] = dinv[i];
+            fw[u] = fixc[((uint32_t)i / 3u) >> 5];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t i = i0 + u * kBlock;
+            if (i >= R.end) break;
+            x[i] = xv[u] + alpha * (double)pv[u];
+            const double rn = rv[u] - alpha * (double)av[u];
+            r[i] = rn;
+            acc += rn * rn;
+            const bool fx = (fw[u] >> (((uint32_t)i / 3u) & 31)) & 1;
+            z[i] = fx ? Tz(0) : om * dv[u] * Tz(rn);
+        }
+    }
+    block_partials(acc, red);
+}
+
 // ConjugateGradient_2.comp: p = z + beta p
 // (flexible CG: also keeps z for the next Polak-Ribiere beta)
 template <typename Tz, typename Tp>
@@ -2856,6 +2911,7 @@ struct vfem_system {
     // tiled transfers between level l and l + 1 (div 2, large fine level)
     bool tiled[kMaxLevels] = {}, tiled_auto[kMaxLevels] = {};
     StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
+    bool presmoothed = false;   // the level-0 pre-smoothing was fused into the CG update just enqueued
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -3627,7 +3683,7 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
 
 // Damped multiplicative V(1,1) cycle, level l >= 0.  rl: level residual
 // (level 0: FP64 CG residual), zl: output, tl: operator scratch.
-template <typename T, typename TR>
+template <typename T, typename TR, typename TN = T>
 void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin,
                                  const T *zold)
 {
@@ -3651,20 +3707,25 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
     const int64_t n3l = 3 * L.nn;
-    hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
-                       (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
-                       (const T *)nullptr);
+    if (l == 0 && S->presmoothed)
+        S->presmoothed = false;  // z = omega D^-1 r was written by k_cg_update_smooth
+    else
+        hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
+                           S->fixc[l], rl, (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
+                           (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
                                        (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
+    // the coarser levels run in TN (the level-0 cycle of a reduced-precision plan may
+    // keep them in FP64)
     if (tiled)
-        hipLaunchKernelGGL((k_restrict_tile<TR, T, T, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, rl, (const T *)tl, level_r<T>(S, l + 1), 1, S->st, S->rtiles[l]);
+        hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
+                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l]);
     else
-        hipLaunchKernelGGL((k_restrict<TR, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
-                           (const T *)tl, level_r<T>(S, l + 1), 1, S->st);
-    enqueue_vcycle_damped_level<T, T>(S, l + 1, level_r<T>(S, l + 1), level_z<T>(S, l + 1), level_t<T>(S, l + 1),
-                                      omega, -1, (const T *)nullptr);
+        hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
+                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st);
+    enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
+                                        omega, -1, (const TN *)nullptr);
     // FP64 cycle, level 0 with the sum-factorised stencil: the prolongation writes
     // z + P z_c to t and the operator applied to it finishes the cycle with the
     // post-smoothing fused in (z = t + omega D^-1 (r - A t)), saving the separate
@@ -3674,11 +3735,11 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     if constexpr (std::is_same<TR, double>::value && std::is_same<T, double>::value)
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_sf(S, S->stl[0]);
     if (tiled)
-        hipLaunchKernelGGL((k_prolong_tile<T, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream, L,
-                           C, (const T *)level_z<T>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st, S->ptiles[l]);
+        hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream,
+                           L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st, S->ptiles[l]);
     else
-        hipLaunchKernelGGL((k_prolong<T, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
-                           (const T *)level_z<T>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
+        hipLaunchKernelGGL((k_prolong<TN, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
+                           (const TN *)level_z<TN>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
                            -1, (const T *)nullptr);
     if constexpr (std::is_same<TR, double>::value) {
         if (fuse) {
@@ -3703,6 +3764,7 @@ struct Plan {
     bool learned;      // damped cycle with the learned coarse correction
     bool force_flex;   // flexible CG although the cycle is fixed (after a learned phase)
     bool op64;         // reduced-precision cycle with the CG operator (and p) in FP64
+    bool coarse64;     // reduced-precision cycle on level 0 only, the coarser levels in FP64
 };
 
 // While alive: level-0 FP64 matvecs of the plan go to the matrix cores (VFEM_MIXED).
@@ -3759,6 +3821,7 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
         }
         S->cycle_passes = P.cycle_passes;
         if (P.cycle == VFEM_CYCLE_REFERENCE) enqueue_vcycle_ref<float>(S, r, S->z32, fin, zold);
+        else if (P.coarse64) enqueue_vcycle_damped_level<float, double, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin, zold);
         else enqueue_vcycle_damped_level<float, double>(S, 0, r, S->z32, S->t0_32, P.omega, fin, zold);
         S->cycle_passes = 6;
     }
@@ -3802,20 +3865,38 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
 {
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
+    // the damped cycle's level-0 pre-smoothing reads the residual the update writes:
+    // fuse it unless the residual is replaced (true residual) or a learned phase runs
+    const bool fuse = !with_true && P.cycle == VFEM_CYCLE_DAMPED && S->H.nlev > 1 && !P.learned &&
+                      !(S->unet_on && S->unet_level == 0);
+    const double om = P.omega;
     if (P.precision == VFEM_FP64 || P.op64) {
         L0Mode mode(S, P);
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
-        hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
-                           S->p64, S->ap64, S->st, S->red);
+        if (fuse && P.precision == VFEM_FP64)
+            hipLaunchKernelGGL((k_cg_update_smooth<double, double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], S->dinv64[0], S->z64, om, S->st, S->red);
+        else if (fuse)
+            hipLaunchKernelGGL((k_cg_update_smooth<double, double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], S->dinv32[0], S->z32, om, S->st, S->red);
+        else
+            hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
+                               S->r, S->p64, S->ap64, S->st, S->red);
         enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
-        hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
-                           S->p32, S->ap32, S->st, S->red);
+        if (fuse)
+            hipLaunchKernelGGL((k_cg_update_smooth<float, float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
+                               S->x, S->r, S->p32, S->ap32, S->fixc[0], S->dinv32[0], S->z32, om, S->st, S->red);
+        else
+            hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
+                               S->p32, S->ap32, S->st, S->red);
         enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
     }
     if (with_true) enqueue_true_residual(S);
+    S->presmoothed = fuse;
     enqueue_precond(S, P, S->r, FIN_RZ);
+    S->presmoothed = false;
     enqueue_pupdate(S, P);
 }
 
@@ -3831,6 +3912,7 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     P.learned = o.learned != 0;
     P.force_flex = false;
     P.op64 = o.op_fp64 != 0 && P.precision == VFEM_FP32;
+    P.coarse64 = o.coarse_fp64 != 0 && P.precision == VFEM_FP32 && o.cycle == VFEM_CYCLE_DAMPED;
     if (P.learned && (!S->unet || o.cycle != VFEM_CYCLE_DAMPED || S->unet_level >= S->H.nlev || S->ncols > 1)) {
         g_last_error = "learned correction needs an attached network, the damped cycle and one right-hand side";
         return VFEM_ERR_UNSUPPORTED;
@@ -3865,6 +3947,7 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     key += (int64_t)(P.learned ? 1 : 0) << 51;
     key += (int64_t)(P.force_flex ? 1 : 0) << 52;
     key += (int64_t)(P.op64 ? 1 : 0) << 53;
+    key += (int64_t)(P.coarse64 ? 1 : 0) << 54;
     auto it = S->graphs.find(key);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -4071,6 +4154,7 @@ void vfem_default_options(vfem_options *o)
     o->learned = 0;
     o->stall_ratio = 0.9;
     o->op_fp64 = 0;
+    o->coarse_fp64 = 0;
 }
 
 const char *vfem_status_string(int status)
