@@ -75,7 +75,10 @@ typedef struct {
     int32_t max_iter;        /* default 30000 (FEMGPU.cpp:407) */
     int32_t precision;       /* enum vfem_precision */
     int32_t cycle;           /* enum vfem_cycle */
-    double omega;            /* damped Jacobi weight (VFEM_CYCLE_DAMPED), default 0.6 */
+    double omega;            /* damped Jacobi weight (VFEM_CYCLE_DAMPED), default 0.5: the V(1,1)
+                                cycle is a positive definite preconditioner only while
+                                omega * lambda_max(D^-1 A) < 2 on every level, and the trilinear
+                                elasticity element has lambda_max ~ 3.8 around surface nodes */
     int32_t true_res_every;  /* mixed precision: FP64 true residual every k iterations, default 20 */
     int32_t check_every;     /* iterations between host convergence polls, default 8 */
     int32_t use_graph;       /* capture one CG iteration in a hipGraph (default 1) */
@@ -233,7 +236,7 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
  * vfem_set_level0_kernel (stencil form: the stencil kernel and the surface pass;
  * element form: FP64 exact products and FP32 three-way BF16 splits, 6 products,
  * on the matrix cores; BF16: one BF16 product; MIXED: matrix cores on FP64
- * input), 1 = one damped V(1,1) cycle (omega 0.6), 2 = one CG iteration with
+ * input), 1 = one damped V(1,1) cycle (default omega), 2 = one CG iteration with
  * that cycle, 3 = level-0 element-form matvec with the two-way BF16 split (3
  * products), 4 = plain copy of one FP32 level-0 vector (12 bytes read and 12
  * written per node: the reference for HBM byte counters), 5 = copy of one FP64

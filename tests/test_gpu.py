@@ -107,8 +107,8 @@ def test_stencil_cycle_matches_element_cycle(systems, name):
             A.set_level0_kernel(kind)
             if kind != V.L0_ELEMENT:  # every div-2 level with full elements
                 assert all(A.level_operator_form(l) == kind for l in range(A.num_levels - 1))
-            out[kind] = (V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r),
-                         V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.FP32).solve(r))
+            out[kind] = (V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.6).solve(r),
+                         V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.6, precision=V.FP32).solve(r))
     finally:
         A.set_level0_kernel(V.L0_AUTO)
     say("cycle %s: fp64 %.2e fp32 %.2e (element fp32 %.2e)" % (name, rel(out[2][0], z_d), rel(out[2][1], z_d),
@@ -176,9 +176,9 @@ def test_precondition_parity(systems, name):
     mask = np.repeat(ora.fixed(0), 3)
     r[mask] = 0
     z_d = ora.vcycle(r, O.DAMPED, 0.6)
-    assert rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r), z_d) < 1e-12
-    e32 = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.FP32).solve(r), z_d)
-    e16 = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, precision=V.BF16).solve(r), z_d)
+    assert rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.6).solve(r), z_d) < 1e-12
+    e32 = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.6, precision=V.FP32).solve(r), z_d)
+    e16 = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.6, precision=V.BF16).solve(r), z_d)
     say("damped cycle rel err fp32 %.2e bf16 %.2e" % (e32, e16))
     assert e32 < 1e-4 and e16 < 2e-2
 
@@ -203,9 +203,11 @@ def test_box_solve_matches_direct_solution(systems, goldens):
     _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.REF_ADDITIVE)
     _, st = A.solve(f, tol=1e-8)
     assert st["iterations"] == it_ora
-    _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.DAMPED, omega=0.6)
-    _, st = A.solve(f, tol=1e-8, cycle=V.CYCLE_DAMPED)
-    assert st["iterations"] == it_ora
+    # the damped cycle at the configuration's omega = 0.6 and at the default 0.5
+    for om in (0.6, 0.5):
+        _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.DAMPED, omega=om)
+        _, st = A.solve(f, tol=1e-8, cycle=V.CYCLE_DAMPED, omega=om)
+        assert st["iterations"] == it_ora, (om, st["iterations"], it_ora)
 
 
 @pytest.mark.parametrize("name", ["ell24", "ell33"])

@@ -24,7 +24,7 @@ def test_status_strings_and_defaults():
     assert L.vfem_status_string(0) == b"ok"
     assert L.vfem_status_string(5) == b"not converged"
     o = V.options()
-    assert o.tol == 1e-8 and o.max_iter == 30000 and o.omega == 0.6 and o.true_res_every == 20
+    assert o.tol == 1e-8 and o.max_iter == 30000 and o.omega == 0.5 and o.true_res_every == 20
 
 
 def _cases():

@@ -4144,7 +4144,10 @@ void vfem_default_options(vfem_options *o)
     o->max_iter = 30000;
     o->precision = VFEM_FP64;
     o->cycle = VFEM_CYCLE_REFERENCE;
-    o->omega = 0.6;
+    // damped Jacobi needs omega * lambda_max(D^-1 A) < 2 on every level for the V(1,1)
+    // cycle to be a positive definite preconditioner; the trilinear elasticity element
+    // (nu = 0.3) has lambda_max ~ 3.8 around surface nodes, so 0.5 (0.6 exceeds it)
+    o->omega = 0.5;
     o->true_res_every = 20;
     o->check_every = 8;
     o->use_graph = 1;
