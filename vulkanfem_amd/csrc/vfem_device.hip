@@ -734,17 +734,20 @@ template <typename T> struct SmoothEpi {
     double omega;
 };
 
-// value of lane - 1 (SHR) or lane + 1 (SHL) within the 16-lane row
+// value of lane - 1 (SHR) or lane + 1 (SHL) within the 16-lane row; lanes whose
+// source is outside the row (0 for SHR, 15 for SHL: the tile's halo columns, which
+// produce no output) read 0.  Zero fill leaves the destination untied from an old
+// value, so no copy precedes the DPP move and the move can fold into its user.
 constexpr int DPP_SHR1 = 0x111, DPP_SHL1 = 0x101;
 template <int CTRL> __device__ __forceinline__ float dpp_row(float v)
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 template <int CTRL> __device__ __forceinline__ double dpp_row(double v)
 {
     const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
@@ -811,8 +814,12 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
         xr[k][0] = q[0], xr[k][1] = q[1], xr[k][2] = q[2];
         mn[k] = st_meta(L, so[k], sin[k], z0);
     }
-    T a0[3] = {0, 0, 0}, a1[3] = {0, 0, 0}, a2[3] = {0, 0, 0};  // outputs p-1, p, p+1
-    T rm2[3] = {0, 0, 0}, rm1[3] = {0, 0, 0};                      // own values, planes p-2, p-1
+    // z direction without rotating accumulators: plane p adds U (dz = +1) to output
+    // p-1, V (dz = 0) to output p and W (dz = -1) to output p+1.  sa holds output
+    // p-1 but for U_p (W_{p-2} + V_{p-1}), wb holds W_{p-1}, rp the own values of
+    // plane p-1.  FP32 centring adds T (r_{q-1} - 2 r_q + r_{q+1}) to output q as
+    // T (r_{q-1} - r_q) + T (r_{q+1} - r_q), each a difference of neighbouring planes.
+    T sa[3] = {0, 0, 0}, wb[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
     int info_m1 = 0;
     double dacc = 0, dacc2 = 0;
     // epilogue operands of the own node, loaded one plane ahead of its output
@@ -859,6 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
                 if (ep.zold) no[c] = ep.zold[q3 + c];
             }
         }
+        T U[3] = {0, 0, 0}, V[3] = {0, 0, 0}, W[3] = {0, 0, 0};
         if (__ballot((st_unpack(nm.info) | info_c | st_unpack(np.info)) & 2)) {
             T MM[3], KM[3], MK[3], GG[3], GM[3], MG[3];
 #pragma unroll
@@ -905,25 +913,28 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
                     else cg = fma(K.g[c][c2], q, cg);
                 }
                 const T f = em - dk;
-                a0[c] += f + cg;                        // output plane p-1: dz = +1
-                a1[c] = fma(T(4), em, fma(T(2), dk, a1[c]));  // dz = 0
-                a2[c] += f - cg;                        // dz = -1
+                U[c] = f + cg;
+                V[c] = fma(T(4), em, T(2) * dk);
+                W[c] = f - cg;
             }
+        }
+        T out[3], d[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            d[c] = r0[c] - rp[c];
+            out[c] = sa[c] + U[c];
+            if constexpr (CENTRE) out[c] = fma(K.t[c], d[c], out[c]);
         }
         if (p > z0 && out_col && (info_m1 & 2)) {
             T r[3];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                r[c] = a0[c];
-                if constexpr (CENTRE) r[c] = fma(K.t[c], fma(T(-2), rm1[c], rm2[c] + r0[c]), r[c]);
-                if (info_m1 & 1) r[c] = T(0);
-            }
+            for (int c = 0; c < 3; c++) r[c] = (info_m1 & 1) ? T(0) : out[c];
             const int64_t q3 = 3 * (int64_t)(info_m1 >> 2);
             if constexpr (EPI) {
                 const T om = T(ep.omega);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    const T v = (info_m1 & 1) ? T(0) : rm1[c] + om * ed[c] * (T(er[c]) - r[c]);
+                    const T v = (info_m1 & 1) ? T(0) : rp[c] + om * ed[c] * (T(er[c]) - r[c]);
                     ep.z[q3 + c] = v;
                     dacc += er[c] * (double)v;
                     dacc2 += er[c] * (double)eo[c];
@@ -933,16 +944,15 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
                 q[0] = r[0];
                 q[1] = r[1];
                 q[2] = r[2];
-                dacc += (double)fma(rm1[2], r[2], fma(rm1[1], r[1], rm1[0] * r[0]));
+                dacc += (double)fma(rp[2], r[2], fma(rp[1], r[1], rp[0] * r[0]));
             }
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            a0[c] = a1[c];
-            a1[c] = a2[c];
-            a2[c] = 0;
-            rm2[c] = rm1[c];
-            rm1[c] = r0[c];
+            sa[c] = wb[c] + V[c];
+            if constexpr (CENTRE) sa[c] = fma(-K.t[c], d[c], sa[c]);
+            wb[c] = W[c];
+            rp[c] = r0[c];
             if constexpr (EPI) {
                 er[c] = nr[c];
                 ed[c] = nd[c];
