@@ -737,6 +737,8 @@ template <typename T> struct SmoothEpi {
     const T *zold;
     T *z;
     double omega;
+    T dc[3];   // D^-1 of every node that is not a surface node (when dcon)
+    int dcon;  // 1: the operator kernel takes D^-1 = dc (surface nodes are redone by the surface pass)
 };
 
 // value of lane - 1 (SHR) or lane + 1 (SHL) within the 16-lane row; lanes whose
@@ -867,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 nr[c] = ep.r[q3 + c];
-                nd[c] = ep.dinv[q3 + c];
+                nd[c] = ep.dcon ? ep.dc[c] : ep.dinv[q3 + c];
                 if (ep.zold) no[c] = ep.zold[q3 + c];
             }
         }
@@ -2389,6 +2391,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong_tile(DevLevel F, DevLevel C,
     }
 }
 
+// D^-1 of a level as the flat kernels read it: the stored array, except at the
+// nodes flagged in `flat`, which hold the level's common value c (the interior:
+// no load there, saving 4 or 8 bytes per DOF on most of the grid).
+template <typename Td> struct DinvRef {
+    const Td *d;
+    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
+    Td c[3];
+};
+
 // Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
 //   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
 // fixed rows zeroed (compact-order fixed bitmask); fin >= 0 reduces r.z (and
@@ -2398,7 +2409,7 @@ enum { SMOOTH_INIT = 0, SMOOTH_POST = 1 };
 template <typename Tr, typename Td, typename Tz>
 __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *__restrict__ fixc,
                                                     const Tr *__restrict__ r, const Tz *__restrict__ t,
-                                                    const Td *__restrict__ dinv, Tz *__restrict__ z, double omega,
+                                                    DinvRef<Td> D, Tz *__restrict__ z, double omega,
                                                     int mode, CGState *st, Reduction red, int fin,
                                                     const Tz *__restrict__ zold)
 {
@@ -2419,18 +2430,27 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
         Tr rv[FLAT_U];
         Td dv[FLAT_U];
         Tz zv[FLAT_U], tv[FLAT_U], ov[FLAT_U];
-        uint32_t fw[FLAT_U];
+        uint32_t fw[FLAT_U], cw[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {  // mask words first: the D^-1 loads wait on them only
+            const uint32_t nd = (uint32_t)min(k0 + u * kBlock, R.end - 1) / 3u;  // n3 < 2^32 (grids up to 1022^3)
+            fw[u] = fixc[nd >> 5];
+            cw[u] = D.flat ? D.flat[nd >> 5] : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t k = min(k0 + u * kBlock, R.end - 1);
             rv[u] = r[k];
-            dv[u] = dinv[k];
-            fw[u] = fixc[((uint32_t)k / 3u) >> 5];  // n3 < 2^32 (grids up to 1022^3)
             if (mode != SMOOTH_INIT) {
                 zv[u] = z[k];
                 tv[u] = t[k];
             }
             if (fin >= 0 && zold) ov[u] = zold[k];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const uint32_t k = (uint32_t)min(k0 + u * kBlock, R.end - 1), nd = k / 3u;
+            dv[u] = ((cw[u] >> (nd & 31)) & 1) ? D.c[k - 3u * nd] : D.d[k];
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -2527,7 +2547,7 @@ template <typename Tp, typename Ta, typename Tz>
 __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ x, double *__restrict__ r,
                                                               const Tp *__restrict__ p, const Ta *__restrict__ ap,
                                                               const uint32_t *__restrict__ fixc,
-                                                              const Tz *__restrict__ dinv, Tz *__restrict__ z,
+                                                              DinvRef<Tz> D, Tz *__restrict__ z,
                                                               double omega, CGState *st, Reduction red)
 {
     {
@@ -2550,7 +2570,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
         Tp pv[FLAT_U];
         Ta av[FLAT_U];
         Tz dv[FLAT_U];
-        uint32_t fw[FLAT_U];
+        uint32_t fw[FLAT_U], cw[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {  // mask words first: the D^-1 loads wait on them only
+            const uint32_t nd = (uint32_t)min(i0 + u * kBlock, R.end - 1) / 3u;
+            fw[u] = fixc[nd >> 5];
+            cw[u] = D.flat ? D.flat[nd >> 5] : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
@@ -2558,8 +2584,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
             rv[u] = r[i];
             pv[u] = p[i];
             av[u] = ap[i];
-            dv[u] = dinv[i];
-            fw[u] = fixc[((uint32_t)i / 3u) >> 5];
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const uint32_t i = (uint32_t)min(i0 + u * kBlock, R.end - 1), nd = i / 3u;
+            dv[u] = ((cw[u] >> (nd & 31)) & 1) ? D.c[i - 3u * nd] : D.d[i];
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -2877,6 +2906,7 @@ struct LevelStencil {
     StencilTiles tiles_sf{};    // blocks of k_matvec_sf (14 columns)
     SurfaceList surf{};         // nodes recomputed by k_stencil_fix
     int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
+    bool dcon_interior = false; // every used, non-fixed node outside the surface list has D^-1 = dcon
 };
 
 struct vfem_system {
@@ -2890,6 +2920,11 @@ struct vfem_system {
     uint32_t *fixc[kMaxLevels] = {};   // compact-order fixed-node bitmask per level
     double *dinv64[kMaxLevels] = {};
     float *dinv32[kMaxLevels] = {};
+    // D^-1 shared by the interior: every node whose 8 elements are present sums
+    // the same element diagonals in the same order, so its D^-1 triple is one
+    // value, dcon; dflat flags (compact-order node bitmask) the nodes that hold it
+    uint32_t *dflat[kMaxLevels] = {};
+    double dcon[kMaxLevels][3] = {};
     double *ktab64[kMaxLevels] = {};
     float *ktab32[kMaxLevels] = {};
     double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
@@ -3215,6 +3250,7 @@ int build_level_stencil(vfem_system *S, int l)
         return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) ? 255u : 0u;
     };
     int64_t id = 0;
+    bool dcon_ok = true;
     for (int z = 0; z < L.nz; z++)
         for (int y = 0; y < L.ny; y++)
             for (int x = 0; x < L.nx; x++) {
@@ -3248,10 +3284,13 @@ int build_level_stencil(vfem_system *S, int l)
                     }
                     surf.push_back(make_int4((int)id, (int)present, (int)used, 0));
                     if (coarse) pats.push_back(make_uint2(pw[0], pw[1]));
+                } else if (!fixed && std::memcmp(L.invdiag.data() + 3 * id, S->dcon[l], 3 * sizeof(double)) != 0) {
+                    dcon_ok = false;
                 }
                 id++;
             }
     LS.surf.n = (int64_t)surf.size();
+    LS.dcon_interior = dcon_ok;
     LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
     std::vector<int32_t> rbs;
     rbs.reserve(9 * surf.size() + 9);
@@ -3393,6 +3432,26 @@ int upload_system(vfem_system *S)
         }
         if (int rc = S->upload(&S->dinv64[l], L.invdiag)) return rc;
         if (int rc = S->upload(&S->dinv32[l], convert<float>(L.invdiag))) return rc;
+        {
+            // the common D^-1 triple: majority vote over the nodes (interior nodes
+            // dominate every level), then flag the nodes that hold it bit for bit
+            const double *d = L.invdiag.data();
+            auto same = [&](int64_t a, int64_t b) { return std::memcmp(d + 3 * a, d + 3 * b, 3 * sizeof(double)) == 0; };
+            int64_t cand = 0, votes = 0;
+            for (int64_t i = 0; i < L.nn; i++) {
+                if (votes == 0) {
+                    cand = i;
+                    votes = 1;
+                } else {
+                    votes += same(i, cand) ? 1 : -1;
+                }
+            }
+            std::vector<uint32_t> fl((L.nn + 31) / 32 + 1, 0u);
+            for (int64_t i = 0; i < L.nn; i++)
+                if (same(i, cand)) fl[i >> 5] |= 1u << (i & 31);
+            for (int c = 0; c < 3; c++) S->dcon[l][c] = L.nn > 0 ? d[3 * cand + c] : 0.0;
+            if (int rc = S->upload(&S->dflat[l], fl)) return rc;
+        }
         if (l >= 1 && l < (int)H.ktab.size() && !H.ktab[l].empty()) {
             if (int rc = S->upload(&S->ktab64[l], H.ktab[l])) return rc;
             if (int rc = S->upload(&S->ktab32[l], convert<float>(H.ktab[l]))) return rc;
@@ -3437,6 +3496,10 @@ template <> float *level_t<float>(vfem_system *S, int l) { return S->tl32[l]; }
 template <typename T> const T *dinv(vfem_system *S, int l);
 template <> const double *dinv<double>(vfem_system *S, int l) { return S->dinv64[l]; }
 template <> const float *dinv<float>(vfem_system *S, int l) { return S->dinv32[l]; }
+template <typename T> DinvRef<T> dinv_ref(vfem_system *S, int l)
+{
+    return DinvRef<T>{dinv<T>(S, l), S->dflat[l], {T(S->dcon[l][0]), T(S->dcon[l][1]), T(S->dcon[l][2])}};
+}
 template <typename T> const T *kefine(vfem_system *S);
 template <> const double *kefine<double>(vfem_system *S) { return S->ke64; }
 template <> const float *kefine<float>(vfem_system *S) { return S->ke32; }
@@ -3542,7 +3605,8 @@ void launch_stencil_smooth(vfem_system *S, const T *x, T *z, const double *r, co
 {
     const DevLevel &L = S->dl[0];
     const LevelStencil &LS = S->stl[0];
-    const SmoothEpi<T> ep{r, dinv<T>(S, 0), zold, z, omega};
+    const SmoothEpi<T> ep{r, dinv<T>(S, 0), zold, z, omega, {T(S->dcon[0][0]), T(S->dcon[0][1]), T(S->dcon[0][2])},
+                          LS.dcon_interior ? 1 : 0};
     launch_stencil_zc<T, kStencilZ, true>(S, L, LS, x, (T *)nullptr, fin, 1, ep);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
@@ -3722,7 +3786,7 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         S->presmoothed = false;  // z = omega D^-1 r was written by k_cg_update_smooth
     else
         hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
-                           S->fixc[l], rl, (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
+                           S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
                            (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
@@ -3760,7 +3824,7 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     }
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
-                       (const T *)tl, dinv<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
+                       (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
     enqueue_finalize(S, S->stream, grid_for_flat(n3l), S->ncols, fin, 1);
 }
 
@@ -3886,10 +3950,10 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         if (fuse && P.precision == VFEM_FP64)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], S->dinv64[0], S->z64, om, S->st, S->red);
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red);
         else if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], S->dinv32[0], S->z32, om, S->st, S->red);
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red);
         else
             hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
                                S->r, S->p64, S->ap64, S->st, S->red);
@@ -3898,7 +3962,7 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<float, float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->x, S->r, S->p32, S->ap32, S->fixc[0], S->dinv32[0], S->z32, om, S->st, S->red);
+                               S->x, S->r, S->p32, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red);
         else
             hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                                S->p32, S->ap32, S->st, S->red);
