@@ -132,6 +132,7 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         break;
     case FIN_RR:
         st->rr = v;
+        st->rzx = v2;  // r_{k+1}.z_k when the update reduced it (k_cg_update_smooth), else 0
         st->last_true = 0;
         if (!isfinite(v)) st->done = 3;
         else if (v < st->threshold) st->done = 1;
@@ -163,6 +164,17 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         st->absOld = st->absNew;
         st->absNew = v;
         st->beta = (v - v2) / st->absOld;
+        st->iter += 1;
+        if (!isfinite(st->beta)) st->done = 3;
+        else if (st->iter >= st->maxit) st->done = 2;
+        break;
+    case FIN_RZX:
+        st->rzx = v;
+        break;
+    case FIN_RZ_XFLEX:
+        st->absOld = st->absNew;
+        st->absNew = v;
+        st->beta = (v - st->rzx) / st->absOld;
         st->iter += 1;
         if (!isfinite(st->beta)) st->done = 3;
         else if (st->iter >= st->maxit) st->done = 2;
@@ -2542,13 +2554,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 }
 
 // k_cg_update fused with the damped cycle's level-0 pre-smoothing, which reads
-// the residual it has just written: z = omega D^-1 r (fixed rows 0).
+// the residual it has just written: z = omega D^-1 r (fixed rows 0).  With
+// `rzx`, the z it overwrites (z_k, the last preconditioned residual) is first
+// dotted with the new residual: the r_{k+1}.z_k of the flexible beta, so no copy
+// of z_k has to be kept.
 template <typename Tp, typename Ta, typename Tz>
 __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ x, double *__restrict__ r,
                                                               const Tp *__restrict__ p, const Ta *__restrict__ ap,
                                                               const uint32_t *__restrict__ fixc,
                                                               DinvRef<Tz> D, Tz *__restrict__ z,
-                                                              double omega, CGState *st, Reduction red)
+                                                              double omega, CGState *st, Reduction red, int rzx)
 {
     {
         const int64_t co = col_offset(n);
@@ -2564,12 +2579,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
     const double alpha = st->alpha;
     const Tz om = Tz(omega);
     const Range R = flat_range(n);
-    double acc = 0;
+    double acc = 0, acc2 = 0;
     for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
         double xv[FLAT_U], rv[FLAT_U];
         Tp pv[FLAT_U];
         Ta av[FLAT_U];
-        Tz dv[FLAT_U];
+        Tz dv[FLAT_U], zo[FLAT_U];
         uint32_t fw[FLAT_U], cw[FLAT_U];
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {  // mask words first: the D^-1 loads wait on them only
@@ -2584,6 +2599,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
             rv[u] = r[i];
             pv[u] = p[i];
             av[u] = ap[i];
+            zo[u] = rzx ? z[i] : Tz(0);
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -2600,9 +2616,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
             acc += rn * rn;
             const bool fx = (fw[u] >> (((uint32_t)i / 3u) & 31)) & 1;
             z[i] = fx ? Tz(0) : om * dv[u] * Tz(rn);
+            acc2 += rn * (double)zo[u];
         }
     }
-    block_partials(acc, red);
+    block_partials2(acc, acc2, red);
 }
 
 // ConjugateGradient_2.comp: p = z + beta p
@@ -3866,6 +3883,28 @@ bool flexible(const vfem_system *S, const Plan &P)
     return (P.precision != VFEM_FP64 || P.mfma64 || P.learned || P.force_flex) && S->H.nlev > 1;
 }
 
+// Flexible CG without a stored z_old: r_{k+1}.z_k is reduced by the fused residual
+// update (k_cg_update_smooth) while it overwrites z_k, or, after the residual is
+// replaced by the true one, by a dot of the new residual with z_k (enqueue_rzx).
+// Plans with learned phases keep z_old.
+bool xflex(const vfem_system *S, const Plan &P)
+{
+    return flexible(S, P) && P.cycle == VFEM_CYCLE_DAMPED && !P.learned && !P.force_flex;
+}
+
+// rzx = r.z_k for a residual that was just replaced (FIN_RZX)
+void enqueue_rzx(vfem_system *S, const Plan &P)
+{
+    const int gf = grid_for_flat(S->n3);
+    if (P.precision == VFEM_FP64)
+        hipLaunchKernelGGL((k_dot<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->r, S->z64,
+                           S->st, S->red, (int)FIN_RZX, 1);
+    else
+        hipLaunchKernelGGL((k_dot<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->r, S->z32,
+                           S->st, S->red, (int)FIN_RZX, 1);
+    enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RZX, 1);
+}
+
 void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 {
     LearnedMode learned(S, P);
@@ -3879,7 +3918,9 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
     }
     if (P.precision == VFEM_FP64) {
         const double *zold = nullptr;
-        if (fin == FIN_RZ && flexible(S, P)) {
+        if (fin == FIN_RZ && xflex(S, P)) {
+            fin = FIN_RZ_XFLEX;
+        } else if (fin == FIN_RZ && flexible(S, P)) {
             fin = FIN_RZ_FLEX;
             zold = S->zold64;
         }
@@ -3890,7 +3931,9 @@ void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
         S->cycle_passes = 6;
     } else {
         const float *zold = nullptr;
-        if (fin == FIN_RZ && flexible(S, P)) {
+        if (fin == FIN_RZ && xflex(S, P)) {
+            fin = FIN_RZ_XFLEX;
+        } else if (fin == FIN_RZ && flexible(S, P)) {
             fin = FIN_RZ_FLEX;
             zold = S->zold32;
         }
@@ -3916,21 +3959,22 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
 {
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
+    const bool keep = flexible(S, P) && !xflex(S, P);  // z_old for the next flexible beta
     if (P.precision == VFEM_FP64)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           flexible(S, P) ? S->zold64 : (double *)nullptr, S->st);
+                           keep ? S->zold64 : (double *)nullptr, S->st);
     else if (P.op64 && S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
                            (double *)nullptr, S->st);
     else if (P.op64)  // FP64 direction from the FP32 preconditioned residual
         hipLaunchKernelGGL((k_cg_pupdate<float, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p64,
-                           S->zold32, S->st);
+                           keep ? S->zold32 : (float *)nullptr, S->st);
     else if (S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
                            (double *)nullptr, S->st);
     else
         hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
-                           S->zold32, S->st);
+                           keep ? S->zold32 : (float *)nullptr, S->st);
 }
 
 // one CG iteration (FEMGPU.cpp:471-541 / Eigen ConjugateGradient loop body).
@@ -3945,15 +3989,18 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
     const bool fuse = !with_true && P.cycle == VFEM_CYCLE_DAMPED && S->H.nlev > 1 && !P.learned &&
                       !(S->unet_on && S->unet_level == 0);
     const double om = P.omega;
+    const int rzx = xflex(S, P) ? 1 : 0;  // the fused update reduces r_{k+1}.z_k
     if (P.precision == VFEM_FP64 || P.op64) {
         L0Mode mode(S, P);
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         if (fuse && P.precision == VFEM_FP64)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red);
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red,
+                               rzx);
         else if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red);
+                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
+                               rzx);
         else
             hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
                                S->r, S->p64, S->ap64, S->st, S->red);
@@ -3962,13 +4009,17 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<float, float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->x, S->r, S->p32, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red);
+                               S->x, S->r, S->p32, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
+                               rzx);
         else
             hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                                S->p32, S->ap32, S->st, S->red);
         enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
     }
-    if (with_true) enqueue_true_residual(S);
+    if (with_true) {
+        enqueue_true_residual(S);
+        if (rzx) enqueue_rzx(S, P);
+    }
     S->presmoothed = fuse;
     enqueue_precond(S, P, S->r, FIN_RZ);
     S->presmoothed = false;
@@ -4121,6 +4172,8 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             if (int rc = read_states(S, h)) return rc;
             if (!all_done()) {
                 // finish the interrupted iteration of the reopened columns: z = M r, beta, p
+                // (a stored-z_old-free flexible beta keeps the r_{k+1}.z_k its last
+                // residual update reduced: z_k itself was overwritten there)
                 enqueue_precond(S, P, S->r, FIN_RZ);
                 enqueue_pupdate(S, P);
                 if (int rc = read_states(S, h)) return rc;

@@ -38,6 +38,7 @@ struct CGState {
     double beta;
     double rr;         // r.r
     double result;     // scratch reduction result (vfem_dot)
+    double rzx;        // r_{k+1}.z_k, reduced by the residual update (flexible beta without z_old)
     double tol;
     int iter;          // completed iterations
     int maxit;
@@ -55,7 +56,9 @@ enum Finalize {
     FIN_RZ = 4,       // absOld = absNew; absNew = v; beta; iter++
     FIN_RHS = 5,      // rhs2 = v; threshold; zero right-hand side -> done
     FIN_RR_TRUE = 6,  // rr = v (true residual); converged test, else continue
-    FIN_RZ_FLEX = 7   // v = r.z, v2 = r.z_old: Polak-Ribiere beta (flexible CG); iter++
+    FIN_RZ_FLEX = 7,  // v = r.z, v2 = r.z_old: Polak-Ribiere beta (flexible CG); iter++
+    FIN_RZX = 8,      // rzx = v (r.z_k of a replaced residual)
+    FIN_RZ_XFLEX = 9  // v = r.z: Polak-Ribiere beta with r.z_old = rzx; iter++
 };
 
 constexpr int kMaxPartials = 4096;
