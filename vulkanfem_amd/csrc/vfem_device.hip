@@ -131,8 +131,10 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         else st->alpha = st->absNew / v;
         break;
     case FIN_RR:
+    case FIN_RR_LAZY:
         st->rr = v;
         st->rzx = v2;  // r_{k+1}.z_k when the update reduced it (k_cg_update_smooth), else 0
+        st->x_lag = fin == FIN_RR_LAZY;
         st->last_true = 0;
         if (!isfinite(v)) st->done = 3;
         else if (v < st->threshold) st->done = 1;
@@ -169,7 +171,9 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         else if (st->iter >= st->maxit) st->done = 2;
         break;
     case FIN_RZX:
-        st->rzx = v;
+        // a column whose x lagged was stopped before this true residual reopened
+        // it: its z_k is gone, so it keeps the r_{k+1}.z_k its last update reduced
+        if (!st->x_lag) st->rzx = v;
         break;
     case FIN_RZ_XFLEX:
         st->absOld = st->absNew;
@@ -2554,7 +2558,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 }
 
 // k_cg_update fused with the damped cycle's level-0 pre-smoothing, which reads
-// the residual it has just written: z = omega D^-1 r (fixed rows 0).  With
+// the residual it has just written: z = omega D^-1 r (fixed rows 0).  `lazy`
+// leaves x += alpha p to the p update, which reads p anyway (k_cg_pupdate).  With
 // `rzx`, the z it overwrites (z_k, the last preconditioned residual) is first
 // dotted with the new residual: the r_{k+1}.z_k of the flexible beta, so no copy
 // of z_k has to be kept.
@@ -2563,7 +2568,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
                                                               const Tp *__restrict__ p, const Ta *__restrict__ ap,
                                                               const uint32_t *__restrict__ fixc,
                                                               DinvRef<Tz> D, Tz *__restrict__ z,
-                                                              double omega, CGState *st, Reduction red, int rzx)
+                                                              double omega, CGState *st, Reduction red, int rzx,
+                                                              int lazy)
 {
     {
         const int64_t co = col_offset(n);
@@ -2595,9 +2601,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
-            xv[u] = x[i];
+            if (!lazy) {
+                xv[u] = x[i];
+                pv[u] = p[i];
+            }
             rv[u] = r[i];
-            pv[u] = p[i];
             av[u] = ap[i];
             zo[u] = rzx ? z[i] : Tz(0);
         }
@@ -2610,7 +2618,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = i0 + u * kBlock;
             if (i >= R.end) break;
-            x[i] = xv[u] + alpha * (double)pv[u];
+            if (!lazy) x[i] = xv[u] + alpha * (double)pv[u];
             const double rn = rv[u] - alpha * (double)av[u];
             r[i] = rn;
             acc += rn * rn;
@@ -2623,29 +2631,33 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 }
 
 // ConjugateGradient_2.comp: p = z + beta p
-// (flexible CG: also keeps z for the next Polak-Ribiere beta)
+// (flexible CG: also keeps z for the next Polak-Ribiere beta; x non-null: first
+// x += alpha p with the outgoing p, the update a lazy k_cg_update_smooth left)
 template <typename Tz, typename Tp>
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
-                                                        Tz *__restrict__ zold, const CGState *st)
+                                                        Tz *__restrict__ zold, CGState *st, double *__restrict__ x)
 {
     {
         const int64_t co = col_offset(n);
         z += co;
         p += co;
         if (zold) zold += co;
+        if (x) x += co;
         st = col_state(st);
     }
     if (st->done) return;
-    const double beta = st->beta;
+    const double beta = st->beta, alpha = st->alpha;
     const Range R = flat_range(n);
     for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
         Tz zv[FLAT_U];
         Tp pv[FLAT_U];
+        double xv[FLAT_U];
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
             zv[u] = z[i];
             pv[u] = p[i];
+            if (x) xv[u] = x[i];
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -2653,8 +2665,33 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__re
             if (i >= R.end) break;
             p[i] = Tp((double)zv[u] + beta * (double)pv[u]);
             if (zold) zold[i] = zv[u];
+            if (x) x[i] = xv[u] + alpha * (double)pv[u];
         }
     }
+    if (x && blockIdx.x == 0 && threadIdx.x == 0) st->x_lag = 0;  // nothing else reads it in this launch
+}
+
+// x += alpha p for columns whose last update left it to a p update that did not
+// run (the column stopped); k_x_lag_clear then resets the flags
+template <typename Tp>
+__global__ __launch_bounds__(kBlock) void k_x_flush(int64_t n, double *__restrict__ x, const Tp *__restrict__ p,
+                                                     const CGState *st)
+{
+    {
+        const int64_t co = col_offset(n);
+        x += co;
+        p += co;
+        st = col_state(st);
+    }
+    if (!st->x_lag) return;
+    const double alpha = st->alpha;
+    const Range R = flat_range(n);
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) x[i] += alpha * (double)p[i];
+}
+
+__global__ void k_x_lag_clear(CGState *st, int ncols)
+{
+    if ((int)threadIdx.x < ncols) st[threadIdx.x].x_lag = 0;
 }
 
 // x = 0, r = f, |f|^2 (x0 = 0)   or   r = f - t, |r|^2 (true residual / warm start)
@@ -2835,6 +2872,8 @@ __global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int
         st->maxit = maxit;
         st->tol = tol;
         st->absNew = st->absOld = st->pAp = st->alpha = st->beta = st->rr = 0;
+        st->rzx = 0;
+        st->x_lag = 0;
     }
 }
 
@@ -2979,6 +3018,7 @@ struct vfem_system {
     bool tiled[kMaxLevels] = {}, tiled_auto[kMaxLevels] = {};
     StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
     bool presmoothed = false;   // the level-0 pre-smoothing was fused into the CG update just enqueued
+    bool lazy_x = false;        // that update left x += alpha p to the p update
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -3960,21 +4000,41 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
     const int64_t n3 = S->n3;
     const int gf = grid_for_flat(n3);
     const bool keep = flexible(S, P) && !xflex(S, P);  // z_old for the next flexible beta
+    double *xl = S->lazy_x ? S->x : nullptr;            // x += alpha p left by the residual update
     if (P.precision == VFEM_FP64)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           keep ? S->zold64 : (double *)nullptr, S->st);
+                           keep ? S->zold64 : (double *)nullptr, S->st, xl);
     else if (P.op64 && S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           (double *)nullptr, S->st);
+                           (double *)nullptr, S->st, xl);
     else if (P.op64)  // FP64 direction from the FP32 preconditioned residual
         hipLaunchKernelGGL((k_cg_pupdate<float, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p64,
-                           keep ? S->zold32 : (float *)nullptr, S->st);
+                           keep ? S->zold32 : (float *)nullptr, S->st, xl);
     else if (S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
-                           (double *)nullptr, S->st);
+                           (double *)nullptr, S->st, xl);
     else
         hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
-                           keep ? S->zold32 : (float *)nullptr, S->st);
+                           keep ? S->zold32 : (float *)nullptr, S->st, xl);
+}
+
+void enqueue_x_lag_clear(vfem_system *S)
+{
+    hipLaunchKernelGGL(k_x_lag_clear, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, S->ncols);
+}
+
+// x += alpha p for the columns whose last iteration stopped before its p update
+// (then the flags are cleared, unless the caller clears them later)
+void enqueue_x_flush(vfem_system *S, const Plan &P, bool clear = true)
+{
+    const int gf = grid_for_flat(S->n3);
+    if (P.precision == VFEM_FP64 || P.op64)
+        hipLaunchKernelGGL((k_x_flush<double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->x, S->p64,
+                           S->st);
+    else
+        hipLaunchKernelGGL((k_x_flush<float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->x, S->p32,
+                           S->st);
+    if (clear) enqueue_x_lag_clear(S);
 }
 
 // one CG iteration (FEMGPU.cpp:471-541 / Eigen ConjugateGradient loop body).
@@ -3990,41 +4050,49 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
                       !(S->unet_on && S->unet_level == 0);
     const double om = P.omega;
     const int rzx = xflex(S, P) ? 1 : 0;  // the fused update reduces r_{k+1}.z_k
+    const int lazy = fuse ? 1 : 0;        // and leaves x += alpha p to the p update, which reads p anyway
     if (P.precision == VFEM_FP64 || P.op64) {
         L0Mode mode(S, P);
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         if (fuse && P.precision == VFEM_FP64)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
                                n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red,
-                               rzx);
+                               rzx, lazy);
         else if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<double, double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
                                n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
-                               rzx);
+                               rzx, lazy);
         else
             hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
                                S->r, S->p64, S->ap64, S->st, S->red);
-        enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
+        enqueue_finalize(S, S->stream, gf, S->ncols, lazy ? FIN_RR_LAZY : FIN_RR, 1);
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         if (fuse)
             hipLaunchKernelGGL((k_cg_update_smooth<float, float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
                                S->x, S->r, S->p32, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
-                               rzx);
+                               rzx, lazy);
         else
             hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                                S->p32, S->ap32, S->st, S->red);
-        enqueue_finalize(S, S->stream, gf, S->ncols, FIN_RR, 1);
+        enqueue_finalize(S, S->stream, gf, S->ncols, lazy ? FIN_RR_LAZY : FIN_RR, 1);
     }
     if (with_true) {
+        // columns that stopped earlier in the polling interval may still lack their
+        // last alpha p: add it before the true residual reads x
+        enqueue_x_flush(S, P, false);
         enqueue_true_residual(S);
         if (rzx) enqueue_rzx(S, P);
+        enqueue_x_lag_clear(S);
     }
     S->presmoothed = fuse;
     enqueue_precond(S, P, S->r, FIN_RZ);
     S->presmoothed = false;
+    S->lazy_x = lazy;
     enqueue_pupdate(S, P);
+    S->lazy_x = false;
 }
+
 
 int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
 {
@@ -4160,6 +4228,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     };
     while (true) {
         if (all_done()) {
+            enqueue_x_flush(S, P);
             // mixed precision stops on the FP64 true residual only
             bool verify = false;
             for (int c = 0; c < nc; c++)
@@ -4183,6 +4252,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
         if (launched >= cap) {
             for (int c = 0; c < nc; c++)
                 if (!h[c].done) capped[c] = 1;
+            enqueue_x_flush(S, P);
             break;
         }
         const int iter_before = h[0].iter;

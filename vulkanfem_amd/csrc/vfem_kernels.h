@@ -45,7 +45,7 @@ struct CGState {
     int done;          // 0 running, 1 converged, 2 iteration cap, 3 breakdown
     int last_true;     // 1 if the last convergence test used the FP64 true residual
     int no_stop;       // kernel timing: finalizers never end the iteration
-    int pad;
+    int x_lag;         // x still lacks alpha p: the residual update left it to the p update
 };
 
 enum Finalize {
@@ -58,7 +58,8 @@ enum Finalize {
     FIN_RR_TRUE = 6,  // rr = v (true residual); converged test, else continue
     FIN_RZ_FLEX = 7,  // v = r.z, v2 = r.z_old: Polak-Ribiere beta (flexible CG); iter++
     FIN_RZX = 8,      // rzx = v (r.z_k of a replaced residual)
-    FIN_RZ_XFLEX = 9  // v = r.z: Polak-Ribiere beta with r.z_old = rzx; iter++
+    FIN_RZ_XFLEX = 9, // v = r.z: Polak-Ribiere beta with r.z_old = rzx; iter++
+    FIN_RR_LAZY = 10  // FIN_RR after an update that left x += alpha p to k_cg_pupdate
 };
 
 constexpr int kMaxPartials = 4096;
