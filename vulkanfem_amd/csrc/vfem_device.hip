@@ -1879,6 +1879,35 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
     }
 }
 
+// D^-1 of a level as the flat kernels read it: the stored array, except at the
+// nodes flagged in `flat`, which hold the level's common value c (the interior:
+// no load there, saving 4 or 8 bytes per DOF on most of the grid).
+template <typename Td> struct DinvRef {
+    const Td *d;
+    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
+    Td c[3];
+};
+
+// The damped cycle's pre-smoothing of the next coarser level, z_c = omega D_c^-1 r_c,
+// written by the restriction that produces r_c (fixed rows: r_c is 0 there, so
+// z_c is too).  z null: none.
+template <typename Tc> struct PreSmooth {
+    Tc *z;
+    DinvRef<Tc> D;
+    double omega;
+};
+template <typename Tc>
+__device__ __forceinline__ void pre_smooth_out(const PreSmooth<Tc> &ps, int64_t id, Tc a0, Tc a1, Tc a2)
+{
+    const Tc om = Tc(ps.omega);
+    const bool flat = ps.D.flat && ((ps.D.flat[id >> 5] >> (id & 31)) & 1);
+    const Tc d0 = flat ? ps.D.c[0] : ps.D.d[3 * id + 0], d1 = flat ? ps.D.c[1] : ps.D.d[3 * id + 1],
+             d2 = flat ? ps.D.c[2] : ps.D.d[3 * id + 2];
+    ps.z[3 * id + 0] = om * d0 * a0;
+    ps.z[3 * id + 1] = om * d1 * a1;
+    ps.z[3 * id + 2] = om * d2 * a2;
+}
+
 // ---------------------------------------------------------------------------
 // transfers
 // ---------------------------------------------------------------------------
@@ -1888,13 +1917,14 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
 template <typename Tf, typename Tt, typename Tc>
 __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
                                                       const Tt *__restrict__ tf, Tc *__restrict__ rc, int maskfixed,
-                                                      const CGState *st)
+                                                      const CGState *st, PreSmooth<Tc> ps)
 {
     {
         const int64_t cf = col_offset(3 * F.nn);
         rf += cf;
         if (tf) tf += cf;
         rc += col_offset(3 * C.nn);
+        if (ps.z) ps.z += col_offset(3 * C.nn);
         st = col_state(st);
     }
     if (st->done) return;
@@ -1988,6 +2018,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
         rc[3 * id + 0] = a0;
         rc[3 * id + 1] = a1;
         rc[3 * id + 2] = a2;
+        if (ps.z) pre_smooth_out(ps, id, a0, a1, a2);
     }
 }
 
@@ -2138,13 +2169,15 @@ __device__ __forceinline__ int node_info(const DevLevel &L, int gx, int gy, int 
 template <typename Tf, typename Tt, typename Tc, int ZC>
 __global__ __launch_bounds__(kBlock) void k_restrict_tile(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
                                                            const Tt *__restrict__ tf, Tc *__restrict__ rc,
-                                                           int maskfixed, const CGState *st, StencilTiles tiles)
+                                                           int maskfixed, const CGState *st, StencilTiles tiles,
+                                                           PreSmooth<Tc> ps)
 {
     {
         const int64_t cf = col_offset(3 * F.nn);
         rf += cf;
         tf += cf;
         rc += col_offset(3 * C.nn);
+        if (ps.z) ps.z += col_offset(3 * C.nn);
         st = col_state(st);
     }
     if (st->done) return;
@@ -2244,9 +2277,11 @@ __global__ __launch_bounds__(kBlock) void k_restrict_tile(DevLevel F, DevLevel C
             const int id = node_info(C, X0 + lx, Y0 + ly, Z, used, fixed);
             if (used) {
                 const bool zero = maskfixed && fixed;
-                rc[3 * (int64_t)id + 0] = zero ? Tc(0) : acc[0];
-                rc[3 * (int64_t)id + 1] = zero ? Tc(0) : acc[1];
-                rc[3 * (int64_t)id + 2] = zero ? Tc(0) : acc[2];
+                const Tc o0 = zero ? Tc(0) : acc[0], o1 = zero ? Tc(0) : acc[1], o2 = zero ? Tc(0) : acc[2];
+                rc[3 * (int64_t)id + 0] = o0;
+                rc[3 * (int64_t)id + 1] = o1;
+                rc[3 * (int64_t)id + 2] = o2;
+                if (ps.z) pre_smooth_out(ps, id, o0, o1, o2);
             }
         }
 #pragma unroll
@@ -2406,15 +2441,6 @@ __global__ __launch_bounds__(kBlock) void k_prolong_tile(DevLevel F, DevLevel C,
                 for (int q = 0; q < 3; q++) prev[j][i][q] = cur[j][i][q];
     }
 }
-
-// D^-1 of a level as the flat kernels read it: the stored array, except at the
-// nodes flagged in `flat`, which hold the level's common value c (the interior:
-// no load there, saving 4 or 8 bytes per DOF on most of the grid).
-template <typename Td> struct DinvRef {
-    const Td *d;
-    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
-    Td c[3];
-};
 
 // Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
 //   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
@@ -3019,6 +3045,7 @@ struct vfem_system {
     StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
     bool presmoothed = false;   // the level-0 pre-smoothing was fused into the CG update just enqueued
     bool lazy_x = false;        // that update left x += alpha p to the p update
+    int pre_level = -1;         // coarse level whose pre-smoothing the restriction just enqueued wrote
     int nb = 0;                 // right-hand-side columns the vectors are allocated for
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
@@ -3794,10 +3821,11 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
         const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
         if (l == 0)
             hipLaunchKernelGGL((k_restrict<double, double, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, F,
-                               C, r, (const double *)nullptr, level_r<T>(S, 1), 0, S->st);
+                               C, r, (const double *)nullptr, level_r<T>(S, 1), 0, S->st, PreSmooth<T>{});
         else
             hipLaunchKernelGGL((k_restrict<T, T, T>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, F, C,
-                               (const T *)level_r<T>(S, l), (const T *)nullptr, level_r<T>(S, l + 1), 0, S->st);
+                               (const T *)level_r<T>(S, l), (const T *)nullptr, level_r<T>(S, l + 1), 0, S->st,
+                               PreSmooth<T>{});
     }
     launch_coarse_solve<T>(S, level_r<T>(S, nl - 1), level_z<T>(S, nl - 1), false);
     for (int l = nl - 2; l >= 1; l--) {
@@ -3841,6 +3869,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     const int64_t n3l = 3 * L.nn;
     if (l == 0 && S->presmoothed)
         S->presmoothed = false;  // z = omega D^-1 r was written by k_cg_update_smooth
+    else if (l > 0 && S->pre_level == l)
+        S->pre_level = -1;  // written by the restriction that produced r_l
     else
         hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
                            S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
@@ -3849,13 +3879,18 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
                                        (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
     // the coarser levels run in TN (the level-0 cycle of a reduced-precision plan may
-    // keep them in FP64)
+    // keep them in FP64).  The restriction also pre-smooths level l + 1 (z = omega
+    // D^-1 r) when that level runs a damped cycle of its own (not the coarse solve,
+    // not the learned correction), saving the separate smoothing launch.
+    const bool pre = l + 2 < nl && !(S->unet_on && l + 1 == S->unet_level);
+    const PreSmooth<TN> ps{pre ? level_z<TN>(S, l + 1) : (TN *)nullptr, dinv_ref<TN>(S, l + 1), omega};
     if (tiled)
         hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l]);
+                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     else
         hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
-                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st);
+                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
+    if (pre) S->pre_level = l + 1;
     enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
                                         omega, -1, (const TN *)nullptr);
     // FP64 cycle, level 0 with the sum-factorised stencil: the prolongation writes
