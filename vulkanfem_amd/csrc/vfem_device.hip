@@ -2972,6 +2972,10 @@ constexpr int64_t kStencilMinNodes = 100000;  // smallest coarse level given the
 constexpr int64_t kTiledMinNodes = 500000;     // smallest fine level given the tiled transfers
 constexpr int kRestrictZ = 8;                  // coarse planes per k_restrict_tile block
 constexpr int kProlongZ = 16;                  // fine planes per k_prolong_tile block
+// transfers from fine levels below kShortZNodes nodes use 4x shorter chunks: their
+// grids are otherwise too small to fill the GPU (level 1 at 256^3: 176 blocks)
+constexpr int kRestrictZShort = 2, kProlongZShort = 4;
+constexpr int64_t kShortZNodes = 4000000;
 
 // Stencil form of one level's operator (see k_matvec_stencil).
 struct LevelStencil {
@@ -3043,6 +3047,7 @@ struct vfem_system {
     // tiled transfers between level l and l + 1 (div 2, large fine level)
     bool tiled[kMaxLevels] = {}, tiled_auto[kMaxLevels] = {};
     StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
+    bool short_z[kMaxLevels] = {};  // the tiles of level l use kRestrictZShort / kProlongZShort
     bool presmoothed = false;   // the level-0 pre-smoothing was fused into the CG update just enqueued
     bool lazy_x = false;        // that update left x += alpha p to the p update
     int pre_level = -1;         // coarse level whose pre-smoothing the restriction just enqueued wrote
@@ -3408,11 +3413,14 @@ int build_stencil(vfem_system *S)
         const HostLevel &F = H.L[l], &C = H.L[l + 1];
         S->tiled[l] = false;
         if (C.div != 2) continue;
-        if ((C.nx + RT_X - 1) / RT_X > 2047 || (C.ny + RT_Y - 1) / RT_Y > 2047 || (C.nz + kRestrictZ - 1) / kRestrictZ > 1023 ||
-            (F.nz + kProlongZ - 1) / kProlongZ > 1023)
+        const bool sz = F.nn < kShortZNodes;
+        const int rz = sz ? kRestrictZShort : kRestrictZ, pz = sz ? kProlongZShort : kProlongZ;
+        if ((C.nx + RT_X - 1) / RT_X > 2047 || (C.ny + RT_Y - 1) / RT_Y > 2047 || (C.nz + rz - 1) / rz > 1023 ||
+            (F.nz + pz - 1) / pz > 1023)
             continue;
-        if (int rc = upload_tiles(S, C, RT_X, RT_Y, kRestrictZ, S->rtiles[l])) return rc;
-        if (int rc = upload_tiles(S, F, 2 * RT_X, 2 * RT_Y, kProlongZ, S->ptiles[l])) return rc;
+        if (int rc = upload_tiles(S, C, RT_X, RT_Y, rz, S->rtiles[l])) return rc;
+        if (int rc = upload_tiles(S, F, 2 * RT_X, 2 * RT_Y, pz, S->ptiles[l])) return rc;
+        S->short_z[l] = sz;
         S->tiled[l] = true;
         S->tiled_auto[l] = F.nn >= kTiledMinNodes;  // small levels: the row-segment kernels are faster
     }
@@ -3884,12 +3892,17 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     // not the learned correction), saving the separate smoothing launch.
     const bool pre = l + 2 < nl && !(S->unet_on && l + 1 == S->unet_level);
     const PreSmooth<TN> ps{pre ? level_z<TN>(S, l + 1) : (TN *)nullptr, dinv_ref<TN>(S, l + 1), omega};
-    if (tiled)
-        hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
-    else
+    if (tiled) {
+        if (S->short_z[l])
+            hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZShort>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock),
+                               0, S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+        else
+            hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
+                               S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+    } else {
         hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
                            (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
+    }
     if (pre) S->pre_level = l + 1;
     enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
                                         omega, -1, (const TN *)nullptr);
@@ -3901,13 +3914,20 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     bool fuse = false;
     if constexpr (std::is_same<TR, double>::value && std::is_same<T, double>::value)
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_sf(S, S->stl[0]);
-    if (tiled)
-        hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream,
-                           L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st, S->ptiles[l]);
-    else
+    if (tiled) {
+        if (S->short_z[l])
+            hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZShort>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
+                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
+                               S->ptiles[l]);
+        else
+            hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
+                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
+                               S->ptiles[l]);
+    } else {
         hipLaunchKernelGGL((k_prolong<TN, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
                            (const TN *)level_z<TN>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
                            -1, (const T *)nullptr);
+    }
     if constexpr (std::is_same<TR, double>::value) {
         if (fuse) {
             launch_stencil_smooth<T>(S, tl, zl, (const double *)rl, zold, omega, fin);
