@@ -729,11 +729,14 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
 // product still works on differences to the output node's own value.
 // ---------------------------------------------------------------------------
 constexpr int SF_X = 14;                // output columns per block
-// planes per block of the stencil kernels on every level.  Short chunks balance the
-// grid: at 256^3, 32-plane chunks (2160 blocks, 8.4 waves per SIMD) left a tail of
-// lightly occupied SIMDs; 8 planes (2 halo planes per 8) took the FP32 sum-factorised
-// kernel from 111 to 91 us and the FP64 one from 190 to 175 us
+// planes per block of the stencil kernels.  Short chunks balance the grid: at
+// 256^3, 32-plane chunks (2160 blocks, 8.4 waves per SIMD) left a tail of lightly
+// occupied SIMDs; 8 planes (2 halo planes per 8) took the FP32 sum-factorised
+// kernel from 111 to 91 us and the FP64 one from 190 to 175 us.  Levels under
+// kStencilShortNodes nodes use 4 planes, for enough blocks to fill the GPU.
 constexpr int kStencilZ = 8;
+constexpr int kStencilZShort = 4;
+constexpr int64_t kStencilShortNodes = 4000000;
 constexpr int SF_W = 16;                // staged columns (= DPP row)
 constexpr int SF_HN = SF_W * (ST_Y + 2);  // staged nodes per plane (288)
 static_assert(SF_HN <= 2 * kBlock, "two staged slots per thread");
@@ -3323,7 +3326,7 @@ int build_level_stencil(vfem_system *S, int l)
     const double *kfull = coarse ? H.ktab[l].data() + 255 * 576 : H.ke;
     if (!stencil_coef(kfull, LS)) return 0;
     LS.sf = stencil_tensor(kfull, LS.perm, LS);
-    LS.zc = kStencilZ;
+    LS.zc = L.nn < kStencilShortNodes ? kStencilZShort : kStencilZ;
     // tile indices are packed in 11 + 11 + 10 bits
     if ((L.nx + SF_X - 1) / SF_X > 2047 || (L.ny + ST_Y - 1) / ST_Y > 2047 || (L.nz + LS.zc - 1) / LS.zc > 1023) return 0;
     if (int rc = upload_tiles(S, L, ST_X, ST_Y, LS.zc, LS.tiles)) return rc;
@@ -3676,7 +3679,8 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
 {
     const DevLevel &L = S->dl[l];
     const LevelStencil &LS = S->stl[l];
-    launch_stencil_zc<T, kStencilZ>(S, L, LS, x, y, fin, honor_done);
+    if (LS.zc == kStencilZShort) launch_stencil_zc<T, kStencilZShort>(S, L, LS, x, y, fin, honor_done);
+    else launch_stencil_zc<T, kStencilZ>(S, L, LS, x, y, fin, honor_done);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
         r2.partials += stencil_tiles(S, LS).n;
@@ -3699,7 +3703,8 @@ void launch_stencil_smooth(vfem_system *S, const T *x, T *z, const double *r, co
     const LevelStencil &LS = S->stl[0];
     const SmoothEpi<T> ep{r, dinv<T>(S, 0), zold, z, omega, {T(S->dcon[0][0]), T(S->dcon[0][1]), T(S->dcon[0][2])},
                           LS.dcon_interior ? 1 : 0};
-    launch_stencil_zc<T, kStencilZ, true>(S, L, LS, x, (T *)nullptr, fin, 1, ep);
+    if (LS.zc == kStencilZShort) launch_stencil_zc<T, kStencilZShort, true>(S, L, LS, x, (T *)nullptr, fin, 1, ep);
+    else launch_stencil_zc<T, kStencilZ, true>(S, L, LS, x, (T *)nullptr, fin, 1, ep);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
         r2.partials += LS.tiles_sf.n;
@@ -5034,9 +5039,15 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
         } else if (which == 7) {
             const LevelStencil &LS = sys->stl[0];
             if (P.precision == VFEM_FP64) {
-                launch_stencil_zc<double, kStencilZ>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
+                if (LS.zc == kStencilZShort)
+                    launch_stencil_zc<double, kStencilZShort>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
+                else
+                    launch_stencil_zc<double, kStencilZ>(sys, sys->dl[0], LS, sys->p64, sys->ap64, -1, 0);
             } else {
-                launch_stencil_zc<float, kStencilZ>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
+                if (LS.zc == kStencilZShort)
+                    launch_stencil_zc<float, kStencilZShort>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
+                else
+                    launch_stencil_zc<float, kStencilZ>(sys, sys->dl[0], LS, sys->p32, sys->ap32, -1, 0);
             }
         } else if (which == 6) {
             hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(sys->n3)), dim3(kBlock), 0, sys->stream,
