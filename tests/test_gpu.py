@@ -89,6 +89,35 @@ def test_stencil_form_matches_element_form(systems, name):
         assert rel(s32, ref) < 2e-6
 
 
+def test_large_level_kernels_match_element_form(ke0):
+    """A level 0 of at least 4M nodes, as at 256^3, runs the 8-plane stencil
+    kernels and the long transfer chunks (smaller levels run 4 planes and short
+    chunks): its operator and damped cycle against the element form on the same
+    system, which the smaller cases pin to the oracle."""
+    p = problems.cantilever_box(160, 160, 160)
+    A = V.assemble_system_matrix(p.voxels, ke0, p.fixed, 6)
+    assert A.n // 3 >= 4_000_000
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(A.n)
+    r = rng.standard_normal(A.n)
+    r[3 * np.repeat(np.asarray(p.fixed), 3) + np.tile([0, 1, 2], len(p.fixed))] = 0
+    out = {}
+    try:
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+            A.set_level0_kernel(kind)
+            say("large system, operator form", kind)
+            out[kind] = (A.matvec(x, V.FP64), A.matvec(x, V.FP32),
+                         V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED).solve(r))
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
+    e64, _, ze = out[V.L0_ELEMENT]
+    s64, s32, zs = out[V.L0_STENCIL]
+    say("large system: fp64 %.2e, fp32 %.2e, cycle %.2e" % (rel(s64, e64), rel(s32, s64), rel(zs, ze)))
+    assert rel(s64, e64) < 1e-14
+    assert rel(s32, s64) < 2e-6
+    assert rel(zs, ze) < 1e-12
+
+
 @pytest.mark.parametrize("name", ["ell24", "ell33"])
 def test_stencil_cycle_matches_element_cycle(systems, name):
     """The damped cycle with the stencil form on every level (the coarse levels'
