@@ -2532,8 +2532,19 @@ __global__ __launch_bounds__(kBlock) void k_coarse_solve(const Tk *__restrict__ 
     const int rowi = blockIdx.x * kWaves + wave;
     if (rowi >= n) return;
     const Tk *Krow = kinv + (int64_t)rowi * n;
-    double acc = 0;
-    for (int k = lane; k < n; k += 64) acc += (double)Krow[k] * (double)rc[freedof[k]];
+    // four independent row chunks in flight per step (the row streams from HBM/MALL)
+    double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+    int k = lane;
+    for (; k + 192 < n; k += 256) {
+        const Tk a0 = Krow[k], a1 = Krow[k + 64], a2 = Krow[k + 128], a3 = Krow[k + 192];
+        const int f0 = freedof[k], f1 = freedof[k + 64], f2 = freedof[k + 128], f3 = freedof[k + 192];
+        acc0 += (double)a0 * (double)rc[f0];
+        acc1 += (double)a1 * (double)rc[f1];
+        acc2 += (double)a2 * (double)rc[f2];
+        acc3 += (double)a3 * (double)rc[f3];
+    }
+    for (; k < n; k += 64) acc0 += (double)Krow[k] * (double)rc[freedof[k]];
+    double acc = (acc0 + acc1) + (acc2 + acc3);
     acc = wave_sum(acc);
     if (lane == 0) zc[freedof[rowi]] = Tv(acc);
 }
