@@ -1824,6 +1824,77 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
 
 // Coarse-level operator of the damped cycle: element matrices from a table
 // indexed by the element's child pattern (tab[p*576 ...], pattern 0 = absent).
+// One 64-node row segment per block: wave w takes the two elements (x-1, x) of
+// layer (y-1+B, z-1+C), (B, C) = (w & 1, w >> 1), gathering only the 12 nodes
+// they touch, and the four partial products meet in LDS in a fixed order.  The
+// levels this kernel serves are small and latency-bound: four short dependency
+// chains per segment instead of one long one.
+template <typename T, int B, int C>
+__device__ __forceinline__ void coarse_pair(const DevLevel &L, const T *__restrict__ tab, const T *__restrict__ x,
+                                            int s, int yy, int zz, int lane, T (&acc)[3])
+{
+    const uint64_t lt = (1ull << lane) - 1;
+    T xv[2][2][3][3];  // [dz - C][dy - B][dx][component], offsets from (x-1, y-1, z-1)
+#pragma unroll
+    for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+        for (int iy = 0; iy < 2; iy++) {
+            const int ry = yy + B + iy - 1, rz = zz + C + iz - 1;
+            const bool rowok = ry >= 0 && ry < L.ny && rz >= 0 && rz < L.nz;  // wave-uniform
+            const int64_t seg = rowok ? ((int64_t)rz * L.ny + ry) * L.S + s : 0;
+            const uint64_t w = rowok ? L.nbits[seg] : 0;
+            const int base = L.nbase[seg];
+            const uint64_t wp = (rowok && s > 0) ? L.nbits[seg - 1] : 0;
+            const uint64_t wn = (rowok && s + 1 < L.S) ? L.nbits[seg + 1] : 0;
+            const int idc = base + __popcll(w & lt);
+            const bool vc = (w >> lane) & 1;
+            const bool vm = lane ? ((w >> (lane - 1)) & 1) : (wp >> 63);
+            const bool vp = (lane < 63) ? ((w >> (lane + 1)) & 1) : (wn & 1);
+            const int im = vm ? idc - 1 : 0, ic = vc ? idc : 0, ip = vp ? idc + (vc ? 1 : 0) : 0;
+            const T *pm = x + 3 * im, *pc = x + 3 * ic, *pp = x + 3 * ip;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const T am = pm[c], ac = pc[c], ap = pp[c];
+                xv[iz][iy][0][c] = vm ? am : T(0);
+                xv[iz][iy][1][c] = vc ? ac : T(0);
+                xv[iz][iy][2][c] = vp ? ap : T(0);
+            }
+        }
+    // differences to the node's own value (the element matrices annihilate rigid
+    // translations): the centre (dz, dy, dx) = (1, 1, 1) is in every element
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const T ctr = xv[1 - C][1 - B][1][c];
+#pragma unroll
+        for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+            for (int iy = 0; iy < 2; iy++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) xv[iz][iy][dx][c] -= ctr;
+    }
+    const int xx = 64 * s + lane;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+        const int ex = xx - 1 + a, ey = yy - 1 + B, ez = zz - 1 + C;
+        const bool inside = ex >= 0 && ex < L.ex && ey >= 0 && ey < L.ey && ez >= 0 && ez < L.ez;
+        const int pat = inside ? L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex] : 0;
+        if (pat == 0) continue;
+        const int Ln = (1 - a) + 2 * (1 - B) + 4 * (1 - C);
+        const T *K = tab + (int64_t)pat * 576 + 3 * Ln * 24;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int iz = m >> 2, iy = (m >> 1) & 1, dx = a + (m & 1);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const T v = xv[iz][iy][dx][j];
+                acc[0] += K[0 * 24 + 3 * m + j] * v;
+                acc[1] += K[1 * 24 + 3 * m + j] * v;
+                acc[2] += K[2 * 24 + 3 * m + j] * v;
+            }
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *__restrict__ tab,
                                                            const T *__restrict__ x, T *__restrict__ y,
@@ -1836,49 +1907,38 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
         st = col_state(st);
     }
     if (st->done) return;
+    static_assert(kWaves == 4, "one element layer (B, C) per wave");
+    __shared__ T part[kWaves][3][64];
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t W = (int64_t)L.nz * L.ny * L.S;
-    const Range R = block_range(W);
-    for (int64_t it = R.begin + wave; it < R.end; it += kWaves) {
+    for (int64_t it = blockIdx.x; it < W; it += gridDim.x) {  // block-uniform
         const uint64_t wself = L.nbits[it];
         if (wself == 0) continue;
         const int s = (int)(it % L.S);
         const int64_t row = it / L.S;
         const int yy = (int)(row % L.ny), zz = (int)(row / L.ny);
-        T xv[27][3];
-        gather27(L, x, s, yy, zz, lane, xv);
-        center_neighbours(xv);
-        const int xx = 64 * s + lane;
-        T acc0 = 0, acc1 = 0, acc2 = 0;
+        T acc[3] = {0, 0, 0};
         if ((wself >> lane) & 1) {
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int a = e & 1, b = (e >> 1) & 1, c = e >> 2;
-                const int ex = xx - 1 + a, ey = yy - 1 + b, ez = zz - 1 + c;
-                const bool inside = ex >= 0 && ex < L.ex && ey >= 0 && ey < L.ey && ez >= 0 && ez < L.ez;
-                const int p = inside ? L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex] : 0;
-                if (p != 0) {
-                    const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
-                    const T *K = tab + (int64_t)p * 576 + 3 * Ln * 24;
-#pragma unroll
-                    for (int m = 0; m < 8; m++) {
-                        const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
-#pragma unroll
-                        for (int j = 0; j < 3; j++) {
-                            const T v = xv[k][j];
-                            acc0 += K[0 * 24 + 3 * m + j] * v;
-                            acc1 += K[1 * 24 + 3 * m + j] * v;
-                            acc2 += K[2 * 24 + 3 * m + j] * v;
-                        }
-                    }
-                }
-            }
-            const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
-            if ((L.fbits[it] >> lane) & 1) acc0 = acc1 = acc2 = 0;
-            y[3 * id + 0] = acc0;
-            y[3 * id + 1] = acc1;
-            y[3 * id + 2] = acc2;
+            if (wave == 0) coarse_pair<T, 0, 0>(L, tab, x, s, yy, zz, lane, acc);
+            else if (wave == 1) coarse_pair<T, 1, 0>(L, tab, x, s, yy, zz, lane, acc);
+            else if (wave == 2) coarse_pair<T, 0, 1>(L, tab, x, s, yy, zz, lane, acc);
+            else coarse_pair<T, 1, 1>(L, tab, x, s, yy, zz, lane, acc);
         }
+        part[wave][0][lane] = acc[0];
+        part[wave][1][lane] = acc[1];
+        part[wave][2][lane] = acc[2];
+        __syncthreads();
+        if (wave == 0 && ((wself >> lane) & 1)) {
+            T r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) r[c] = ((part[0][c][lane] + part[1][c][lane]) + part[2][c][lane]) + part[3][c][lane];
+            const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+            if ((L.fbits[it] >> lane) & 1) r[0] = r[1] = r[2] = 0;
+            y[3 * id + 0] = r[0];
+            y[3 * id + 1] = r[1];
+            y[3 * id + 2] = r[2];
+        }
+        __syncthreads();
     }
 }
 
@@ -3819,7 +3879,8 @@ template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x
         return;
     }
     const DevLevel &L = S->dl[l];
-    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
+    const int segs = (int)std::min<int64_t>((int64_t)L.nz * L.ny * L.S, 8192);  // one row segment per block
+    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
                        S->st);
 }
 
