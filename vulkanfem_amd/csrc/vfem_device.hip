@@ -1977,6 +1977,53 @@ __device__ __forceinline__ void pre_smooth_out(const PreSmooth<Tc> &ps, int64_t 
 
 // r_c[N] = sum_{|f - div N| < div} w(f - div N) (r_f[f] - t_f[f]); t_f may be null.
 // Coarse fixed rows zeroed when maskfixed.
+// One fine row (2Y + dy, 2Z + dz) of the div-2 full-weighting restriction at the
+// lane's coarse node X = 64 s + lane: fine x = 2X + {-1, 0, +1}, where 2X lies in
+// fine word 2s (lanes < 32) or 2s + 1.  Clamped, unconditional loads (rows outside
+// the grid get weight 0), so the loads of several rows can be in flight together.
+template <typename Tf, typename Tt, typename Tc>
+__device__ __forceinline__ void restrict_fine_row(const DevLevel &F, const Tf *__restrict__ rf,
+                                                  const Tt *__restrict__ tf, int s, int Y, int Z, int dz, int dy,
+                                                  int lane, Tc &a0, Tc &a1, Tc &a2)
+{
+    const int b = (2 * lane) & 63;
+    const uint64_t below = (1ull << b) - 1;
+    const int fs0 = 2 * s, fs1 = 2 * s + 1;
+    const int cs0 = min(fs0, F.S - 1), cs1 = min(fs1, F.S - 1), csp = max(fs0 - 1, 0);
+    const int fz = 2 * Z + dz, fy = 2 * Y + dy;
+    const bool rowok = fz >= 0 && fz < F.nz && fy >= 0 && fy < F.ny;
+    const Tc wzy = rowok ? (dz ? Tc(0.5) : Tc(1)) * (dy ? Tc(0.5) : Tc(1)) : Tc(0);
+    const int64_t sb = rowok ? ((int64_t)fz * F.ny + fy) * F.S : 0;
+    const uint64_t wA0 = F.nbits[sb + cs0], wB0 = F.nbits[sb + cs1], wP0 = F.nbits[sb + csp];
+    const int bA0 = F.nbase[sb + cs0], bB0 = F.nbase[sb + cs1];
+    const uint64_t wA = rowok && fs0 < F.S ? wA0 : 0, wB = rowok && fs1 < F.S ? wB0 : 0;
+    const uint64_t wP = rowok && fs0 > 0 ? wP0 : 0;
+    const uint64_t w = lane < 32 ? wA : wB;
+    const int id0 = (lane < 32 ? bA0 : bB0) + __popcll(w & below);
+    const bool v0 = (w >> b) & 1, vp = (w >> (b + 1)) & 1;
+    const bool vm = b ? ((w >> (b - 1)) & 1) : (lane < 32 ? (wP >> 63) : (wA >> 63));
+    // clamped: lanes past the coarse row end load valid (discarded) entries
+    const int hi = (int)F.nn - 1;
+    const int i0 = v0 ? min(id0, hi) : 0, ip = vp ? min(id0 + (v0 ? 1 : 0), hi) : 0;
+    const int im = vm ? min(max(id0 - 1, 0), hi) : 0;
+    Tc q[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        q[0][c] = Tc(rf[3 * im + c]);
+        q[1][c] = Tc(rf[3 * i0 + c]);
+        q[2][c] = Tc(rf[3 * ip + c]);
+        if (tf) {
+            q[0][c] -= Tc(tf[3 * im + c]);
+            q[1][c] -= Tc(tf[3 * i0 + c]);
+            q[2][c] -= Tc(tf[3 * ip + c]);
+        }
+    }
+    const Tc wm = vm ? wzy * Tc(0.5) : Tc(0), w0 = v0 ? wzy : Tc(0), wp = vp ? wzy * Tc(0.5) : Tc(0);
+    a0 += wm * q[0][0] + w0 * q[1][0] + wp * q[2][0];
+    a1 += wm * q[0][1] + w0 * q[1][1] + wp * q[2][1];
+    a2 += wm * q[0][2] + w0 * q[1][2] + wp * q[2][2];
+}
+
 template <typename Tf, typename Tt, typename Tc>
 __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
                                                       const Tt *__restrict__ tf, Tc *__restrict__ rc, int maskfixed,
@@ -2004,51 +2051,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
         const int Y = (int)(row % C.ny), Z = (int)(row / C.ny), X = 64 * s + lane;
         Tc a0 = 0, a1 = 0, a2 = 0;
         if (div == 2) {
-            // fine x = 2X + {-1, 0, +1}; 2X lies in fine word 2s (lanes < 32) or 2s+1.
-            // All nine fine rows are unrolled with clamped, unconditional loads (rows
-            // outside the grid get weight 0), so their loads are in flight together.
-            const int b = (2 * lane) & 63;
-            const uint64_t below = (1ull << b) - 1;
-            const int fs0 = 2 * s, fs1 = 2 * s + 1;
-            const int cs0 = min(fs0, F.S - 1), cs1 = min(fs1, F.S - 1), csp = max(fs0 - 1, 0);
 #pragma unroll
-            for (int dz = -1; dz <= 1; dz++) {
+            for (int dz = -1; dz <= 1; dz++)
 #pragma unroll
-                for (int dy = -1; dy <= 1; dy++) {
-                    const int fz = 2 * Z + dz, fy = 2 * Y + dy;
-                    const bool rowok = fz >= 0 && fz < F.nz && fy >= 0 && fy < F.ny;
-                    const Tc wzy = rowok ? (dz ? Tc(0.5) : Tc(1)) * (dy ? Tc(0.5) : Tc(1)) : Tc(0);
-                    const int64_t sb = rowok ? ((int64_t)fz * F.ny + fy) * F.S : 0;
-                    const uint64_t wA0 = F.nbits[sb + cs0], wB0 = F.nbits[sb + cs1], wP0 = F.nbits[sb + csp];
-                    const int bA0 = F.nbase[sb + cs0], bB0 = F.nbase[sb + cs1];
-                    const uint64_t wA = rowok && fs0 < F.S ? wA0 : 0, wB = rowok && fs1 < F.S ? wB0 : 0;
-                    const uint64_t wP = rowok && fs0 > 0 ? wP0 : 0;
-                    const uint64_t w = lane < 32 ? wA : wB;
-                    const int id0 = (lane < 32 ? bA0 : bB0) + __popcll(w & below);
-                    const bool v0 = (w >> b) & 1, vp = (w >> (b + 1)) & 1;
-                    const bool vm = b ? ((w >> (b - 1)) & 1) : (lane < 32 ? (wP >> 63) : (wA >> 63));
-                    // clamped: lanes past the coarse row end load valid (discarded) entries
-                    const int hi = (int)F.nn - 1;
-                    const int i0 = v0 ? min(id0, hi) : 0, ip = vp ? min(id0 + (v0 ? 1 : 0), hi) : 0;
-                    const int im = vm ? min(max(id0 - 1, 0), hi) : 0;
-                    Tc q[3][3];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        q[0][c] = Tc(rf[3 * im + c]);
-                        q[1][c] = Tc(rf[3 * i0 + c]);
-                        q[2][c] = Tc(rf[3 * ip + c]);
-                        if (tf) {
-                            q[0][c] -= Tc(tf[3 * im + c]);
-                            q[1][c] -= Tc(tf[3 * i0 + c]);
-                            q[2][c] -= Tc(tf[3 * ip + c]);
-                        }
-                    }
-                    const Tc wm = vm ? wzy * Tc(0.5) : Tc(0), w0 = v0 ? wzy : Tc(0), wp = vp ? wzy * Tc(0.5) : Tc(0);
-                    a0 += wm * q[0][0] + w0 * q[1][0] + wp * q[2][0];
-                    a1 += wm * q[0][1] + w0 * q[1][1] + wp * q[2][1];
-                    a2 += wm * q[0][2] + w0 * q[1][2] + wp * q[2][2];
-                }
-            }
+                for (int dy = -1; dy <= 1; dy++) restrict_fine_row(F, rf, tf, s, Y, Z, dz, dy, lane, a0, a1, a2);
         } else {
             for (int dz = 1 - div; dz < div; dz++) {
                 const int fz = div * Z + dz;
@@ -2082,6 +2088,54 @@ __global__ __launch_bounds__(kBlock) void k_restrict(DevLevel F, DevLevel C, con
         rc[3 * id + 1] = a1;
         rc[3 * id + 2] = a2;
         if (ps.z) pre_smooth_out(ps, id, a0, a1, a2);
+    }
+}
+
+// k_restrict for div-2 levels with one coarse row segment per block: the nine
+// fine rows are dealt to the four waves (rows w, w + 4, w + 8) and the partial
+// sums meet in LDS in a fixed order.  The levels that use the row-segment kernels
+// are small: four short dependency chains per segment instead of one long one.
+template <typename Tf, typename Tt, typename Tc>
+__global__ __launch_bounds__(kBlock) void k_restrict2(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
+                                                       const Tt *__restrict__ tf, Tc *__restrict__ rc, int maskfixed,
+                                                       const CGState *st, PreSmooth<Tc> ps)
+{
+    {
+        const int64_t cf = col_offset(3 * F.nn);
+        rf += cf;
+        if (tf) tf += cf;
+        rc += col_offset(3 * C.nn);
+        if (ps.z) ps.z += col_offset(3 * C.nn);
+        st = col_state(st);
+    }
+    if (st->done) return;
+    __shared__ Tc part[kWaves][3][64];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t W = (int64_t)C.nz * C.ny * C.S;
+    for (int64_t it = blockIdx.x; it < W; it += gridDim.x) {  // block-uniform
+        const uint64_t wself = C.nbits[it];
+        if (wself == 0) continue;
+        const int s = (int)(it % C.S);
+        const int64_t row = it / C.S;
+        const int Y = (int)(row % C.ny), Z = (int)(row / C.ny);
+        Tc a0 = 0, a1 = 0, a2 = 0;
+        for (int q = wave; q < 9; q += kWaves) restrict_fine_row(F, rf, tf, s, Y, Z, q / 3 - 1, q % 3 - 1, lane, a0, a1, a2);
+        part[wave][0][lane] = a0;
+        part[wave][1][lane] = a1;
+        part[wave][2][lane] = a2;
+        __syncthreads();
+        if (wave == 0 && ((wself >> lane) & 1)) {
+            Tc r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) r[c] = ((part[0][c][lane] + part[1][c][lane]) + part[2][c][lane]) + part[3][c][lane];
+            const int id = C.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
+            if (maskfixed && ((C.fbits[it] >> lane) & 1)) r[0] = r[1] = r[2] = 0;
+            rc[3 * id + 0] = r[0];
+            rc[3 * id + 1] = r[1];
+            rc[3 * id + 2] = r[2];
+            if (ps.z) pre_smooth_out(ps, id, r[0], r[1], r[2]);
+        }
+        __syncthreads();
     }
 }
 
@@ -3977,7 +4031,9 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
             hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
                                S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     } else {
-        hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
+        // div-2 levels only (the damped cycle needs skip_levels = 0): one row segment per block
+        const int segs = (int)std::min<int64_t>((int64_t)C.nz * C.ny * C.S, 8192);
+        hipLaunchKernelGGL((k_restrict2<TR, T, TN>), dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
                            (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
     }
     if (pre) S->pre_level = l + 1;
