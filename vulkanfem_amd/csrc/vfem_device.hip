@@ -232,11 +232,22 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Reduction red, int nblocks,
     if (honor_done && st->done) return;
     __shared__ double sh[2][kWaves];
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    double s = 0, s2 = 0;
-    for (int i = threadIdx.x; i < nblocks; i += kBlock) {
-        s += red.partials[i];
-        s2 += red.partials[red.cap + i];
+    // four independent partial sums per thread keep the loads in flight (a fixed
+    // order: the result is deterministic)
+    double a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    int i = threadIdx.x;
+    for (; i + 3 * kBlock < nblocks; i += 4 * kBlock) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a[u] += red.partials[i + u * kBlock];
+            b[u] += red.partials[red.cap + i + u * kBlock];
+        }
     }
+    for (; i < nblocks; i += kBlock) {
+        a[0] += red.partials[i];
+        b[0] += red.partials[red.cap + i];
+    }
+    double s = (a[0] + a[1]) + (a[2] + a[3]), s2 = (b[0] + b[1]) + (b[2] + b[3]);
     s = wave_sum(s);
     s2 = wave_sum(s2);
     if (lane == 0) {
