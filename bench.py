@@ -98,7 +98,9 @@ def batched_solves():
 def cpu_baseline(p, nlev, iterations, sample_iters, threads):
     """The oracle (C restatement of the reference CPU path, additive cycle, FP64)
     on the host cores: setup plus `sample_iters` PCG iterations, scaled to the
-    iteration count of the same algorithm's full solve."""
+    iteration count of the same algorithm's full solve.  Also one iteration on a
+    single thread (the reference's Eigen path is single-threaded), returned
+    separately as the scaled solve time."""
     from oracle import oracle as O
 
     O.set_threads(threads)
@@ -109,12 +111,17 @@ def cpu_baseline(p, nlev, iterations, sample_iters, threads):
     t0 = time.perf_counter()
     s.pcg(p.loads[0], tol=1e-8, max_iter=sample_iters, precond=O.REF_ADDITIVE)
     t_it = (time.perf_counter() - t0) / (sample_iters + 1)  # + the initial cycle
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    s.pcg(p.loads[0], tol=1e-8, max_iter=1, precond=O.REF_ADDITIVE)
+    t_it1 = (time.perf_counter() - t0) / 2
+    O.set_threads(threads)
     s.close()
     return {"value": t_it * (iterations + 1), "unit": "s", "cores": threads, "kind": "port",
             "sample": f"oracle setup ({t_setup:.1f} s, not counted) + {sample_iters} FP64 PCG iterations with the "
                       f"reference additive V-cycle on the {p.name} workload, {t_it:.3f} s/iteration x "
                       f"{iterations + 1} (initial cycle + the iteration count of the full GPU solve of the same "
-                      f"algorithm)"}
+                      f"algorithm)"}, t_it1 * (iterations + 1)
 
 
 def main():
@@ -236,7 +243,9 @@ def main():
     cpu = None
     if not args.no_cpu and "ref_cycle_fp64_iterations" in out_extra:
         threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(p, nlev, out_extra["ref_cycle_fp64_iterations"], args.cpu_iters, threads)
+        cpu, t1 = cpu_baseline(p, nlev, out_extra["ref_cycle_fp64_iterations"], args.cpu_iters, threads)
+        # the same algorithm on one host thread (1 iteration + the initial cycle, scaled likewise)
+        out_extra["cpu_baseline_1thread_s"] = round(t1, 1)
     rec = {
         "metric": "time to rel-res 1e-8 (s) + matvec GDOF/s, 256^3 voxels, mixed vs plain FP64",
         "value": round(t_mixed, 5),
@@ -251,6 +260,8 @@ def main():
         "dtype": "f64+f32" if prec == V.FP32 else "f64+bf16",
         "data": "synthetic",
         "config": {"workload": p.name, "voxels": list(p.voxels.shape[::-1]), "ndof": A.n, "levels": A.num_levels,
+                   "fill": round(p.info.get("fill", 1.0), 4), "components": p.info.get("components", 1),
+                   "kept_components": p.info.get("kept_components", 1),
                    "cycle": f"damped V(1,1), omega {OMEGA}", "tol": 1e-8, "true_res_every": 20},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
