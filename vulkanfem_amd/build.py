@@ -1,8 +1,9 @@
-"""Builds the HIP library vulkanfem_amd/libvfem.so for gfx950 with hipcc.
+"""Builds the HIP library vulkanfem_amd/libvfem.so for gfx950 with hipcc, and the
+command-line solver vulkanfem_amd/vfem_cli (C++, over include/vfem.hpp) against it.
 
     python -m vulkanfem_amd.build
 
-The library is compiled in-tree (it travels with the source tree); nothing is
+Both are built in-tree (they travel with the source tree); nothing is
 installed.  Every HIP source is compiled with packed FP32 instructions disabled.
 """
 import os
@@ -15,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvfem.so")
+CLI = os.path.join(HERE, "vfem_cli")
 
 SOURCES = ["vfem_setup.cpp", "vfem_device.hip", "vfem_unet.hip"]
 HIP_FLAGS = [
@@ -64,7 +66,22 @@ def build(verbose=False, jobs=2):
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         os.replace(tmp_lib, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose=False):
+    """The command-line drop-in for the reference's solver program: host C++ only,
+    linked against libvfem.so, found next to it at run time."""
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(CSRC, "vfem_cli.cpp"),
+           "-L", HERE, "-lvfem", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":
