@@ -15,6 +15,8 @@
 //   - the coarsest-level matrix Kc on free DOFs (FEM.cpp:473-513) and its
 //     inverse for the device GEMV.
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -97,60 +99,109 @@ std::vector<uint8_t> used_from_elements(const HostLevel &L, const std::vector<ui
     return used;
 }
 
+// Runs f(i) for every i in [0, count) on up to 16 host threads.  Indices are
+// handed out in ascending order from a shared counter, so callers put the
+// costliest rows first; every entry is computed by one fixed sequence of
+// operations, so the result does not depend on the thread count.
+template <class F> void parallel_rows(int64_t count, const F &f)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (count < 32) nt = 1;
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < count;) f(i);
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
+
+// a[0..m) . b[0..m) with four partial sums
+inline double dot4(const double *a, const double *b, int64_t m)
+{
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int64_t k = 0;
+    for (; k + 4 <= m; k += 4) {
+        s0 += a[k] * b[k];
+        s1 += a[k + 1] * b[k + 1];
+        s2 += a[k + 2] * b[k + 2];
+        s3 += a[k + 3] * b[k + 3];
+    }
+    for (; k < m; k++) s0 += a[k] * b[k];
+    return (s0 + s1) + (s2 + s3);
+}
+
 // Cholesky (lower, row-major) in place; returns false if not positive definite.
+// Blocked right-looking form: each 64-column block is factored, the rows below
+// it are solved against it, and the trailing lower triangle is updated with
+// dot products of contiguous row pieces; the last two steps run on threads.
+// Only the lower triangle is read or written.
 bool cholesky(std::vector<double> &A, int64_t n)
 {
-    for (int64_t i = 0; i < n; i++) {
-        double *Ai = &A[i * n];
-        for (int64_t j = 0; j <= i; j++) {
-            const double *Aj = &A[j * n];
-            double s = Ai[j];
-            for (int64_t k = 0; k < j; k++) s -= Ai[k] * Aj[k];
-            if (j == i) {
-                if (!(s > 0)) return false;
-                Ai[i] = std::sqrt(s);
-            } else {
-                Ai[j] = s / Aj[j];
+    constexpr int64_t NB = 64;
+    for (int64_t kb = 0; kb < n; kb += NB) {
+        const int64_t ke = std::min(n, kb + NB);
+        // the diagonal block holds A minus the products of the earlier blocks
+        for (int64_t i = kb; i < ke; i++) {
+            double *Ai = &A[i * n];
+            for (int64_t j = kb; j <= i; j++) {
+                const double *Aj = &A[j * n];
+                const double s = Ai[j] - dot4(Ai + kb, Aj + kb, j - kb);
+                if (j == i) {
+                    if (!(s > 0)) return false;
+                    Ai[i] = std::sqrt(s);
+                } else {
+                    Ai[j] = s / Aj[j];
+                }
             }
         }
+        if (ke == n) break;
+        parallel_rows(n - ke, [&](int64_t r) {
+            double *Ai = &A[(ke + r) * n];
+            for (int64_t j = kb; j < ke; j++) {
+                const double *Aj = &A[j * n];
+                Ai[j] = (Ai[j] - dot4(Ai + kb, Aj + kb, j - kb)) / Aj[j];
+            }
+        });
+        parallel_rows(n - ke, [&](int64_t r) {
+            const int64_t i = n - 1 - r;  // longest rows first
+            double *Ai = &A[i * n];
+            for (int64_t j = ke; j <= i; j++) Ai[j] -= dot4(Ai + kb, &A[j * n] + kb, ke - kb);
+        });
     }
     return true;
 }
 
-// inverse of A = L L^T from the factor; columns solved in parallel
+// Inverse of A = L L^T from the factor, as W^T W with W = L^-1.  Column j of W
+// (nonzero from row j on) is stored as row j of WT, so both the triangular
+// solves and the product are dot products of contiguous row pieces:
+// (A^-1)_ij = sum over k >= max(i, j) of W_ki W_kj.
 std::vector<double> spd_inverse(std::vector<double> Lf, int64_t n, bool &ok)
 {
     std::vector<double> inv;
     ok = cholesky(Lf, n);
     if (!ok) return inv;
+    std::vector<double> WT((size_t)n * n, 0.0);
+    parallel_rows(n, [&](int64_t j) {
+        double *w = &WT[j * n];
+        w[j] = 1.0 / Lf[j * n + j];
+        for (int64_t i = j + 1; i < n; i++) {
+            const double *Li = &Lf[i * n];
+            w[i] = -dot4(Li + j, w + j, i - j) / Li[i];
+        }
+    });
     inv.assign((size_t)n * n, 0.0);
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; t++) {
-        th.emplace_back([&, t]() {
-            std::vector<double> b(n);
-            for (int64_t col = t; col < n; col += nt) {
-                std::fill(b.begin(), b.end(), 0.0);
-                b[col] = 1.0;
-                for (int64_t i = col; i < n; i++) {
-                    double v = b[i];
-                    const double *Li = &Lf[i * n];
-                    for (int64_t k = col; k < i; k++) v -= Li[k] * b[k];
-                    b[i] = v / Li[i];
-                }
-                for (int64_t i = n - 1; i >= 0; i--) {
-                    double v = b[i];
-                    for (int64_t k = i + 1; k < n; k++) v -= Lf[k * n + i] * b[k];
-                    b[i] = v / Lf[i * n + i];
-                }
-                for (int64_t i = 0; i < n; i++) inv[i * n + col] = b[i];
-            }
-        });
-    }
-    for (auto &x : th) x.join();
+    parallel_rows(n, [&](int64_t i) {
+        const double *wi = &WT[i * n];
+        for (int64_t j = i; j < n; j++) {
+            const double v = dot4(wi + j, &WT[j * n] + j, n - j);
+            inv[i * n + j] = v;
+            inv[j * n + i] = v;
+        }
+    });
     return inv;
 }
-
 }  // namespace
 
 bool cholesky_factor(std::vector<double> &A, int64_t n) { return cholesky(A, n); }
@@ -316,42 +367,53 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
     for (int64_t r = 0; r < (int64_t)F->ny * F->nz; r++)
         for (int x = 0; x < F->nx; x++) cur_fixed[r * F->nx + x] = (F->fbits[r * F->S + x / 64] >> (x % 64)) & 1;
 
-    // inverse diagonal with per-element diagonals
+    // inverse diagonal with per-element diagonals, node by node: a used node sums
+    // the diagonals of its present elements in ascending element order (the
+    // order an element-by-element scatter adds them), so every node whose eight
+    // elements are full gets the same value bit for bit
     auto inverse_diagonal = [&](HostLevel &L, const std::vector<uint8_t> &pres, const std::vector<uint64_t> *masks,
                                 const std::vector<int32_t> &nid) {
-        std::vector<double> diag(3 * L.nn, 0.0);
-        std::unordered_map<uint64_t, std::vector<double>> cache;
-        uint64_t fullmask = (L.div == 4) ? ~0ull : 0xFFull;
-        for (int z = 0; z < L.ez; z++)
-            for (int y = 0; y < L.ey; y++)
-                for (int x = 0; x < L.ex; x++) {
-                    int64_t e = lin(x, y, z, L.ex, L.ey);
-                    if (!pres[e]) continue;
-                    const double *dv = nullptr;
-                    double sc = L.fullscale;
-                    std::vector<double> fulld(24);
-                    if (masks && (*masks)[e] != fullmask) {
-                        auto it = cache.find((*masks)[e]);
-                        if (it == cache.end()) {
-                            std::vector<double> K(576);
-                            galerkin_element(H.ke, L.div, (*masks)[e], K.data());
-                            std::vector<double> d(24);
-                            for (int q = 0; q < 24; q++) d[q] = K[q * 24 + q];
-                            it = cache.emplace((*masks)[e], d).first;
-                        }
-                        dv = it->second.data();
-                        sc = L.partscale;
-                    } else {
-                        for (int q = 0; q < 24; q++) fulld[q] = H.ke[q * 24 + q];
-                        dv = fulld.data();
-                    }
-                    for (int Ln = 0; Ln < 8; Ln++) {
-                        int32_t id = nid[lin(x + (Ln & 1), y + ((Ln >> 1) & 1), z + ((Ln >> 2) & 1), L.nx, L.ny)];
-                        for (int c = 0; c < 3; c++) diag[3 * (int64_t)id + c] += sc * dv[3 * Ln + c];
-                    }
+        const uint64_t fullmask = (L.div == 4) ? ~0ull : 0xFFull;
+        std::array<double, 24> fulld;
+        for (int q = 0; q < 24; q++) fulld[q] = L.fullscale * H.ke[q * 24 + q];
+        // diagonals of the partial elements' Galerkin matrices, one per child mask
+        std::unordered_map<uint64_t, std::array<double, 24>> partial;
+        if (masks) {
+            std::vector<uint64_t> todo;
+            for (size_t e = 0; e < pres.size(); e++) {
+                const uint64_t m = (*masks)[e];
+                if (pres[e] && m != fullmask && partial.emplace(m, std::array<double, 24>{}).second) todo.push_back(m);
+            }
+            // the map is not modified below: threads look up and fill distinct entries
+            parallel_rows((int64_t)todo.size(), [&](int64_t t) {
+                std::vector<double> K(576);
+                galerkin_element(H.ke, L.div, todo[t], K.data());
+                std::array<double, 24> &d = partial.find(todo[t])->second;
+                for (int q = 0; q < 24; q++) d[q] = L.partscale * K[q * 24 + q];
+            });
+        }
+        L.invdiag.assign(3 * L.nn, 0.0);
+        parallel_rows(L.nz, [&](int64_t z) {
+            for (int y = 0; y < L.ny; y++)
+                for (int x = 0; x < L.nx; x++) {
+                    const int32_t id = nid[lin(x, y, (int)z, L.nx, L.ny)];
+                    if (id < 0) continue;
+                    double s[3] = {0.0, 0.0, 0.0};
+                    for (int k = 0; k < 2; k++)
+                        for (int j = 0; j < 2; j++)
+                            for (int i = 0; i < 2; i++) {
+                                const int ex = x - 1 + i, ey = y - 1 + j, ez = (int)z - 1 + k;
+                                if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) continue;
+                                const int64_t e = lin(ex, ey, ez, L.ex, L.ey);
+                                if (!pres[e]) continue;
+                                const double *dv = fulld.data();
+                                if (masks && (*masks)[e] != fullmask) dv = partial.find((*masks)[e])->second.data();
+                                const int ln = (1 - i) + 2 * (1 - j) + 4 * (1 - k);
+                                for (int c = 0; c < 3; c++) s[c] += dv[3 * ln + c];
+                            }
+                    for (int c = 0; c < 3; c++) L.invdiag[3 * (int64_t)id + c] = 1.0 / s[c];
                 }
-        L.invdiag.resize(3 * L.nn);
-        for (int64_t i = 0; i < 3 * L.nn; i++) L.invdiag[i] = 1.0 / diag[i];
+        });
     };
     inverse_diagonal(*F, present, nullptr, ids);
 
