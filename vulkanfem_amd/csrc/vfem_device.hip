@@ -2723,24 +2723,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 }
 
 // k_cg_update fused with the damped cycle's level-0 pre-smoothing, which reads
-// the residual it has just written: z = omega D^-1 r (fixed rows 0).  `lazy`
-// leaves x += alpha p to the p update, which reads p anyway (k_cg_pupdate).  With
-// `rzx`, the z it overwrites (z_k, the last preconditioned residual) is first
-// dotted with the new residual: the r_{k+1}.z_k of the flexible beta, so no copy
-// of z_k has to be kept.
-template <typename Tp, typename Ta, typename Tz>
-__global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ x, double *__restrict__ r,
-                                                              const Tp *__restrict__ p, const Ta *__restrict__ ap,
+// the residual it has just written: z = omega D^-1 r (fixed rows 0).  It leaves
+// x += alpha p to the p update, which reads p anyway (k_cg_pupdate), so x and p
+// are not touched here.  With `rzx`, the z it overwrites (z_k, the last
+// preconditioned residual) is first dotted with the new residual: the
+// r_{k+1}.z_k of the flexible beta, so no copy of z_k has to be kept.
+template <typename Ta, typename Tz>
+__global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ r,
+                                                              const Ta *__restrict__ ap,
                                                               const uint32_t *__restrict__ fixc,
                                                               DinvRef<Tz> D, Tz *__restrict__ z,
-                                                              double omega, CGState *st, Reduction red, int rzx,
-                                                              int lazy)
+                                                              double omega, CGState *st, Reduction red, int rzx)
 {
     {
         const int64_t co = col_offset(n);
-        x += co;
         r += co;
-        p += co;
         ap += co;
         z += co;
         st = col_state(st);
@@ -2752,8 +2749,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
     const Range R = flat_range(n);
     double acc = 0, acc2 = 0;
     for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
-        double xv[FLAT_U], rv[FLAT_U];
-        Tp pv[FLAT_U];
+        double rv[FLAT_U];
         Ta av[FLAT_U];
         Tz dv[FLAT_U], zo[FLAT_U];
         uint32_t fw[FLAT_U], cw[FLAT_U];
@@ -2766,10 +2762,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
-            if (!lazy) {
-                xv[u] = x[i];
-                pv[u] = p[i];
-            }
             rv[u] = r[i];
             av[u] = ap[i];
             zo[u] = rzx ? z[i] : Tz(0);
@@ -2783,7 +2775,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = i0 + u * kBlock;
             if (i >= R.end) break;
-            if (!lazy) x[i] = xv[u] + alpha * (double)pv[u];
             const double rn = rv[u] - alpha * (double)av[u];
             r[i] = rn;
             acc += rn * rn;
@@ -2797,7 +2788,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 
 // ConjugateGradient_2.comp: p = z + beta p
 // (flexible CG: also keeps z for the next Polak-Ribiere beta; x non-null: first
-// x += alpha p with the outgoing p, the update a lazy k_cg_update_smooth left)
+// x += alpha p with the outgoing p, the update k_cg_update_smooth left)
 template <typename Tz, typename Tp>
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
                                                         Tz *__restrict__ zold, CGState *st, double *__restrict__ x)
@@ -4249,18 +4240,16 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
                       !(S->unet_on && S->unet_level == 0);
     const double om = P.omega;
     const int rzx = xflex(S, P) ? 1 : 0;  // the fused update reduces r_{k+1}.z_k
-    const int lazy = fuse ? 1 : 0;        // and leaves x += alpha p to the p update, which reads p anyway
+    const int lazy = fuse ? 1 : 0;        // the fused update leaves x += alpha p to the p update
     if (P.precision == VFEM_FP64 || P.op64) {
         L0Mode mode(S, P);
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         if (fuse && P.precision == VFEM_FP64)
-            hipLaunchKernelGGL((k_cg_update_smooth<double, double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red,
-                               rzx, lazy);
+            hipLaunchKernelGGL((k_cg_update_smooth<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
+                               S->r, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red, rzx);
         else if (fuse)
-            hipLaunchKernelGGL((k_cg_update_smooth<double, double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream,
-                               n3, S->x, S->r, S->p64, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
-                               rzx, lazy);
+            hipLaunchKernelGGL((k_cg_update_smooth<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
+                               S->r, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red, rzx);
         else
             hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
                                S->r, S->p64, S->ap64, S->st, S->red);
@@ -4268,9 +4257,8 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         if (fuse)
-            hipLaunchKernelGGL((k_cg_update_smooth<float, float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->x, S->r, S->p32, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red,
-                               rzx, lazy);
+            hipLaunchKernelGGL((k_cg_update_smooth<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
+                               S->r, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red, rzx);
         else
             hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                                S->p32, S->ap32, S->st, S->red);
