@@ -242,7 +242,13 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
  * written per node: the reference for HBM byte counters), 5 = copy of one FP64
  * level-0 vector, 6 = FP64 dot product of two level-0 vectors, 7 = the level-0
  * stencil kernel alone, without the surface pass (FP32 or FP64; fails with
- * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form). */
+ * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form), 8 = the same kernel as
+ * it runs inside CG iterations: `reps` iterations as in 2 (after one untimed),
+ * launched eagerly, with each launch of the level-0 stencil kernel (three per
+ * iteration: the CG operator and the cycle's two) bracketed by its own events;
+ * the mean over those launches.  Kind 7 repeats one launch on the same vectors,
+ * which the last-level cache can partly hold; kind 8 sees the vectors as the
+ * solve leaves them.  reps is at most 4096. */
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);
 
 /* Host-only inspection of the hierarchy vfem_assemble builds (no device call; no

@@ -309,6 +309,27 @@ def test_skip_level_solve(ke0):
     assert e.value.status == V.api.ERR_UNSUPPORTED
 
 
+def test_in_cg_stencil_timing(systems):
+    """Kind 8 of vfem_time_kernel times the level-0 stencil kernel launch by launch
+    inside CG iterations; it rejects out-of-range arguments, and a solve after it
+    is bitwise the solve before it (the timing leaves no state behind)."""
+    A, ora, _, _ = systems["ell24"]
+    f = np.zeros(A.n)
+    f[2::3] = -1.0 / (A.n // 3)
+    f[np.repeat(ora.fixed(0), 3)] = 0
+    say("in-CG stencil timing")
+    u1, s1 = A.solve(f, tol=1e-9, precision=V.FP32, cycle=V.CYCLE_DAMPED)
+    t8 = A.time_kernel(8, V.FP32, reps=4)
+    t7 = A.time_kernel(7, V.FP32, reps=4)
+    assert np.isfinite(t8) and 0 < t8 < 100 and 0 < t7 < 100, (t8, t7)
+    for which, reps in ((9, 4), (8, 0), (8, 4097)):
+        with pytest.raises(V.VfemError):
+            A.time_kernel(which, V.FP32, reps=reps)
+    u2, s2 = A.solve(f, tol=1e-9, precision=V.FP32, cycle=V.CYCLE_DAMPED)
+    assert s1["iterations"] == s2["iterations"]
+    assert np.array_equal(u1, u2)
+
+
 def test_no_graph_path_matches_graph(systems):
     A, ora, _, _ = systems["ell24"]
     f = np.zeros(A.n)

@@ -3158,6 +3158,10 @@ struct vfem_system {
     CGState *st = nullptr;
     Reduction red{};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // launch timing of the level-0 stencil kernel (vfem_time_kernel kind 8): while
+    // probe_n < probe_cap, each launch is bracketed by events probe_ev[2n], [2n+1]
+    hipEvent_t *probe_ev = nullptr;
+    int probe_n = 0, probe_cap = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     double setup_ms = 0;
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
@@ -3779,9 +3783,9 @@ const StencilTiles &stencil_tiles(const vfem_system *S, const LevelStencil &LS)
     return use_sf(S, LS) ? LS.tiles_sf : LS.tiles;
 }
 
-template <typename T, int ZC, bool EPI = false>
-void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
-                       int honor_done, SmoothEpi<T> ep = {})
+template <typename T, int ZC, bool EPI>
+void launch_stencil_kernel(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
+                           int honor_done, SmoothEpi<T> ep)
 {
     if (use_sf(S, LS)) {
         if (LS.perm == 1)
@@ -3798,6 +3802,16 @@ void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS
     else
         hipLaunchKernelGGL((k_matvec_stencil<T, 0, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
                            st_coef<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+}
+
+template <typename T, int ZC, bool EPI = false>
+void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
+                       int honor_done, SmoothEpi<T> ep = {})
+{
+    const bool probe = &LS == &S->stl[0] && S->probe_n < S->probe_cap;
+    if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n], S->stream);
+    launch_stencil_kernel<T, ZC, EPI>(S, L, LS, x, y, fin, honor_done, ep);
+    if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n++ + 1], S->stream);
 }
 
 // stencil matvec of level l plus its surface recomputation; the reduction (level
@@ -4401,6 +4415,9 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     if (int rc = read_states(S, h)) return rc;
     std::vector<int> true_count(nc, warm ? 1 : 0);
     const int chunk = mixed ? std::max(1, o.true_res_every) : std::max(1, o.check_every);
+    int pos = 0;  // iterations launched in the current interval (mixed: its last one takes the true residual)
+    std::vector<double> prev_rr(nc);
+    std::vector<int> prev_it(nc);
     int64_t launched = 0;
     const int64_t cap = (int64_t)o.max_iter + 2 * chunk + 8;
     std::vector<int> capped(nc, 0);
@@ -4433,6 +4450,7 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
                 enqueue_precond(S, P, S->r, FIN_RZ);
                 enqueue_pupdate(S, P);
                 if (int rc = read_states(S, h)) return rc;
+                pos = 0;  // a new interval starts after the replaced residual
             }
             continue;
         }
@@ -4444,8 +4462,31 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
         }
         const int iter_before = h[0].iter;
         const double rr_before = h[0].rr;
-        for (int k = 0; k < chunk; k++) {
-            const bool wt = mixed && k == chunk - 1;
+        // iterations before the next poll: to the end of the interval, or fewer when
+        // the residual's decay since the last poll puts the convergence test earlier
+        // (the kernels of iterations launched past it only find the column done)
+        int n_launch = chunk - pos;
+        if (!P.learned) {
+            int pred = 0;
+            bool ok = true;
+            for (int c = 0; c < nc && ok; c++) {
+                if (h[c].done || capped[c]) continue;
+                const int its = h[c].iter - prev_it[c];
+                ok = its > 0 && h[c].rr > h[c].threshold && prev_rr[c] > h[c].rr;
+                if (ok) {
+                    const double need = std::log(h[c].threshold / h[c].rr) / (std::log(h[c].rr / prev_rr[c]) / its);
+                    ok = need < chunk;
+                    if (ok) pred = std::max(pred, (int)std::ceil(need));
+                }
+            }
+            if (ok && pred >= 1 && pred < n_launch) n_launch = pred;
+        }
+        for (int c = 0; c < nc; c++) {
+            prev_rr[c] = h[c].rr;
+            prev_it[c] = h[c].iter;
+        }
+        for (int k = 0; k < n_launch; k++) {
+            const bool wt = mixed && pos + k == chunk - 1;
             hipGraphExec_t g = wt ? g_true : g_norm;
             if (g) HIP_TRY(hipGraphLaunch(g, S->stream));
             else enqueue_iteration(S, P, wt);
@@ -4453,7 +4494,8 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
                 for (int c = 0; c < nc; c++)
                     if (!h[c].done) true_count[c]++;
         }
-        launched += chunk;
+        pos = (pos + n_launch) % chunk;
+        launched += n_launch;
         if (int rc = read_states(S, h)) return rc;
         if (P.learned) {
             const int done_its = h[0].iter - iter_before;
@@ -5133,11 +5175,11 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
 
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms)
 {
-    if (!sys || !avg_ms || reps < 1 || which < 0 || which > 7) {
+    if (!sys || !avg_ms || reps < 1 || reps > 4096 || which < 0 || which > 8) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
-    if (which == 7 && !sys->stl[0].ok) {
+    if ((which == 7 || which == 8) && !sys->stl[0].ok) {
         g_last_error = "level 0 has no stencil form";
         return VFEM_ERR_UNSUPPORTED;
     }
@@ -5187,6 +5229,38 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
         }
     };
     one();
+    if (which == 8) {
+        // `reps` CG iterations, eager, each launch of the level-0 stencil kernel
+        // timed on its own; *avg_ms = the mean over those launches
+        const int cap = 8 * reps;
+        std::vector<hipEvent_t> ev(2 * cap, nullptr);
+        hipError_t e = hipSuccess;
+        for (auto &v : ev)
+            if (e == hipSuccess) e = hipEventCreate(&v);
+        float sum = 0;
+        int n = 0;
+        if (e == hipSuccess) {
+            sys->probe_ev = ev.data();
+            sys->probe_n = 0;
+            sys->probe_cap = cap;
+            for (int i = 0; i < reps; i++) one();
+            n = sys->probe_n;
+            sys->probe_cap = 0;
+            sys->probe_ev = nullptr;
+            e = n > 0 ? hipEventSynchronize(ev[2 * n - 1]) : hipErrorInvalidValue;
+            for (int i = 0; i < n && e == hipSuccess; i++) {
+                float ms = 0;
+                e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
+                sum += ms;
+            }
+        }
+        for (auto v : ev)
+            if (v) (void)hipEventDestroy(v);
+        HIP_TRY(e);
+        HIP_TRY(hipGetLastError());
+        *avg_ms = sum / n;
+        return VFEM_OK;
+    }
     HIP_TRY(hipEventRecord(sys->ev0, sys->stream));
     for (int i = 0; i < reps; i++) one();
     HIP_TRY(hipEventRecord(sys->ev1, sys->stream));
