@@ -248,7 +248,8 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
  * iteration: the CG operator and the cycle's two) bracketed by its own events;
  * the mean over those launches.  Kind 7 repeats one launch on the same vectors,
  * which the last-level cache can partly hold; kind 8 sees the vectors as the
- * solve leaves them.  reps is at most 4096. */
+ * solve leaves them; VFEM_ERR_UNSUPPORTED when the plan's iterations launch
+ * no level-0 stencil kernel (element form at level 0).  reps is at most 4096. */
 int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t reps, double *avg_ms);
 
 /* Host-only inspection of the hierarchy vfem_assemble builds (no device call; no

@@ -325,6 +325,13 @@ def test_in_cg_stencil_timing(systems):
     for which, reps in ((9, 4), (8, 0), (8, 4097)):
         with pytest.raises(V.VfemError):
             A.time_kernel(which, V.FP32, reps=reps)
+    A.set_level0_kernel(V.L0_ELEMENT)  # no stencil launches to time
+    try:
+        with pytest.raises(V.VfemError) as e:
+            A.time_kernel(8, V.FP32, reps=2)
+        assert e.value.status == V.api.ERR_UNSUPPORTED
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
     u2, s2 = A.solve(f, tol=1e-9, precision=V.FP32, cycle=V.CYCLE_DAMPED)
     assert s1["iterations"] == s2["iterations"]
     assert np.array_equal(u1, u2)

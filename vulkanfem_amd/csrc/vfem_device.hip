@@ -5247,7 +5247,7 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
             n = sys->probe_n;
             sys->probe_cap = 0;
             sys->probe_ev = nullptr;
-            e = n > 0 ? hipEventSynchronize(ev[2 * n - 1]) : hipErrorInvalidValue;
+            e = n > 0 ? hipEventSynchronize(ev[2 * n - 1]) : hipStreamSynchronize(sys->stream);
             for (int i = 0; i < n && e == hipSuccess; i++) {
                 float ms = 0;
                 e = hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]);
@@ -5258,6 +5258,10 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
             if (v) (void)hipEventDestroy(v);
         HIP_TRY(e);
         HIP_TRY(hipGetLastError());
+        if (n == 0) {
+            g_last_error = "the plan's iterations launch no level-0 stencil kernel (element-form level 0)";
+            return VFEM_ERR_UNSUPPORTED;
+        }
         *avg_ms = sum / n;
         return VFEM_OK;
     }
