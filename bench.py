@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only launch the level-0 matvec variants and the copy reference this many times each "
                          "(for rocprofv3 counter runs), then exit")
+    ap.add_argument("--profile-solve", action="store_true",
+                    help="only run one mixed solve with its kernels launched one by one (no graph), for "
+                         "rocprofv3 counter runs, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -173,6 +176,9 @@ def main():
         return
     prec = V.FP32 if args.precision == "fp32" else V.BF16
     mixed = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=OMEGA, true_res_every=20)
+    if args.profile_solve:
+        log("profile solve", A.solve_resident(use_graph=0, **mixed))
+        return
 
     for _ in range(args.warmup):
         st = A.solve_resident(**mixed)
