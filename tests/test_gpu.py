@@ -309,6 +309,45 @@ def test_skip_level_solve(ke0):
     assert e.value.status == V.api.ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("seed", [0, 2, 5, 8, 12, 16])
+def test_random_geometries_match_oracle(ke0, seed):
+    """Random rough ellipsoid unions (16^3 and 23^3, every third with the skip level):
+    the operator and the reference V-cycle agree with the oracle to rounding, the
+    FP64 solves (reference cycle, and damped at the library's omega 0.5) reach the
+    oracle's solution, FP32 reaches it to 1e-6, and iteration counts agree up to the
+    rounding drift of long CG runs (exact below ~200 iterations in practice)."""
+    n = 16 + (seed * 7) % 14
+    vox, _ = problems.ellipsoid_union(n, 6 + seed % 7, 2.0, n / 3.0, 100 + seed)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    skip = 1 if seed % 3 == 0 else 0
+    say("random geometry", seed, n, "skip", skip)
+    A = V.assemble_system_matrix(vox, ke0, fixed, 3, skip_levels=skip)
+    ora = O.OracleSystem(vox, ke0, fixed, 3, skip=skip)
+    r = np.random.default_rng(seed).standard_normal(A.n)
+    r[np.repeat(ora.fixed(0), 3)] = 0
+    assert rel(A.matvec(r), ora.matvec(r)) < 1e-14
+    assert rel(V.MultigridPreconditioner(A).solve(r), ora.vcycle(r, O.REF_ADDITIVE)) < 1e-12
+    zs = np.nonzero((ids >= 0).any(axis=(1, 2)))[0]
+    top = ids[zs.max()]
+    f = np.zeros(A.n)
+    f[3 * top[top >= 0] + 2] = -1.0 / np.count_nonzero(top >= 0)
+    f[np.repeat(ora.fixed(0), 3)] = 0
+    plans = [({}, dict(precond=O.REF_ADDITIVE))]
+    if not skip:
+        plans.append((dict(cycle=V.CYCLE_DAMPED), dict(precond=O.DAMPED, omega=0.5)))
+    for lib, orc in plans:
+        u_ref, it_ref, _ = ora.pcg(f, tol=1e-10, **orc)
+        u, st = A.solve(f, tol=1e-10, **lib)
+        assert st["converged"] == 1 and rel(u, u_ref) < 1e-8
+        assert abs(st["iterations"] - it_ref) <= max(1, it_ref // 50), (st["iterations"], it_ref)
+        if lib:
+            u32, s32 = A.solve(f, tol=1e-10, precision=V.FP32, **lib)
+            assert s32["converged"] == 1 and rel(u32, u_ref) < 1e-6
+    ora.close()
+    A.close()
+
+
 def test_in_cg_stencil_timing(systems):
     """Kind 8 of vfem_time_kernel times the level-0 stencil kernel launch by launch
     inside CG iterations; it rejects out-of-range arguments, and a solve after it
