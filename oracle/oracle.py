@@ -1,7 +1,7 @@
 """ctypes binding of the CPU FP64 oracle (oracle/vfem_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package
+cpu_baseline leg of bench.py, to compare results with.  The product package
 (vulkanfem_amd) never imports this module.
 """
 import ctypes

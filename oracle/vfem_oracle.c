@@ -1,7 +1,7 @@
 /*
  * vfem_oracle.c -- CPU FP64 restatement of the reference solver path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library: only
+ * TEST INFRASTRUCTURE ONLY.  The HIP library's results are compared with it: only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may build, load
  * or call it.  The product library (vulkanfem_amd/csrc) never links or calls it.
  *
