@@ -250,7 +250,7 @@ def main():
         out_extra.update(learned_solve(A, p.loads[0], nlev))
     if not args.no_batched:
         out_extra.update(batched_solves())
-    kernel = "k_matvec_sf<float,1,8,false>"
+    kernel = "k_matvec_sf<float,1,12,false>"
     traffic = measured_traffic(kernel, p.name)
     cpu = None
     if not args.no_cpu and "ref_cycle_fp64_iterations" in out_extra:

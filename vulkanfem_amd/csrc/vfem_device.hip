@@ -742,10 +742,12 @@ __global__ __launch_bounds__(kBlock) void k_matvec_stencil(DevLevel L, StencilCo
 constexpr int SF_X = 14;                // output columns per block
 // planes per block of the stencil kernels.  Short chunks balance the grid: at
 // 256^3, 32-plane chunks (2160 blocks, 8.4 waves per SIMD) left a tail of lightly
-// occupied SIMDs; 8 planes (2 halo planes per 8) took the FP32 sum-factorised
-// kernel from 111 to 91 us and the FP64 one from 190 to 175 us.  Levels under
-// kStencilShortNodes nodes use 4 planes, for enough blocks to fill the GPU.
-constexpr int kStencilZ = 8;
+// occupied SIMDs.  Inside the CG iterations, where the halo planes come from HBM,
+// 12 planes (5120 blocks, 2 halo planes per 12) beat 8 and 16: the FP32
+// sum-factorised kernel takes 108 us per launch there, against 113 and 115.
+// Levels under kStencilShortNodes nodes use 4 planes, for enough blocks to fill
+// the GPU.
+constexpr int kStencilZ = 12;
 constexpr int kStencilZShort = 4;
 constexpr int64_t kStencilShortNodes = 4000000;
 constexpr int SF_W = 16;                // staged columns (= DPP row)
