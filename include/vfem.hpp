@@ -66,7 +66,9 @@ template <> class MatrixFreeSparse<double> {
         return y;
     }
 
-    /* Options used by solveWithKompute / solveWithHIP for this matrix. */
+    /* Options used by solveWithKompute / solveWithHIP for this matrix: the C-ABI
+     * defaults with the reference's compiled tolerance, TOLERANCE 1e-6
+     * (include/FEMDefines.hpp:7). */
     vfem_options options() const { return opts_; }
     void setOptions(const vfem_options &o) { opts_ = o; }
 
@@ -79,6 +81,7 @@ template <> class MatrixFreeSparse<double> {
     vfem_options opts_ = [] {
         vfem_options o;
         vfem_default_options(&o);
+        o.tol = 1e-6;
         return o;
     }();
     vfem_stats stats_{};

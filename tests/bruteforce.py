@@ -141,3 +141,42 @@ class Hierarchy:
             x = self.P[l] @ x + 2 * self.invdiag(l) * rs[l]
         x[fixed_dof_mask(len(x) // 3, self.fixed[0])] = 0
         return x
+
+    def level_operator(self, l):
+        """A_l with the fixed rows zeroed: the operator of the damped cycle on
+        level l (MatrixFreeSparse.hpp:210-213 applied on every level)."""
+        keep = ~fixed_dof_mask(self.A[l].shape[0] // 3, self.fixed[l])
+        return (sp.diags(keep.astype(np.float64)) @ self.A[l]).tocsr()
+
+    def kc_galerkin(self):
+        """Free-DOF block of the coarsest Galerkin operator (the damped cycle's Kc)."""
+        return self.A[-1].toarray()[np.ix_(self.free, self.free)]
+
+    def vcycle_damped(self, r, omega):
+        """Multiplicative V(1,1) with damped Jacobi written out with explicit
+        matrices: x = w D^-1 r; x += P (cycle of P^T (r - A x)); x += w D^-1 (r - A x);
+        fixed DOFs held at zero on every level; exact solve on the coarsest free DOFs."""
+        L = self.nlev
+
+        def level(l, rl):
+            fm = fixed_dof_mask(len(rl) // 3, self.fixed[l])
+            if l == L - 1:
+                x = np.zeros_like(rl)
+                x[self.free] = np.linalg.solve(self.kc_galerkin(), rl[self.free])
+                return x
+            A = self.level_operator(l)
+            d = self.invdiag(l)
+            x = omega * d * rl
+            x[fm] = 0
+            rc = self.P[l].T @ (rl - A @ x)
+            rc[fixed_dof_mask(len(rc) // 3, self.fixed[l + 1])] = 0
+            t = self.P[l] @ level(l + 1, rc)
+            t[fm] = 0
+            x = x + t
+            x = x + omega * d * (rl - A @ x)
+            x[fm] = 0
+            return x
+
+        r0 = np.array(r, dtype=np.float64)
+        r0[fixed_dof_mask(len(r0) // 3, self.fixed[0])] = 0
+        return level(0, r0)

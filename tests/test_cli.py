@@ -92,3 +92,35 @@ def test_cli_solves_reference_format_model(tmp_path, ke0, plan):
     assert u.size == ndof
     # the reference's writer keeps six decimals (std::to_string)
     assert np.max(np.abs(u - u_ref)) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_cli_default_tolerance_and_non_binary_voxels(tmp_path, ke0):
+    """Without --tol the CLI solves to the reference's compiled TOLERANCE 1e-6
+    (include/FEMDefines.hpp:7), and a voxel value other than 1 is void, as in the
+    reference's test voxelModel[...] == 1 (src/FEM.cpp:75)."""
+    from oracle import oracle as O
+
+    marked = np.ones((8, 8, 12), np.int32)
+    marked[2:6, 3:5, 4:9] = 2          # a hole written as 2s: void for the reference
+    solid = (marked == 1).astype(np.int32)
+    ids = problems.compact_ids(solid)  # node ids of the model with the hole
+    fixed = np.sort(ids[:, :, 0].reshape(-1))
+    loaded = np.sort(ids[:, :, 12].reshape(-1))
+    loads = np.tile([0.0, 0.0, -1.0 / loaded.size], (loaded.size, 1))
+    write_model(str(tmp_path), "box", marked, ke0, fixed, loaded, loads)
+    out = run_cli(tmp_path, "-n", "3")
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0, out.stderr
+    assert f"Elements: {int(solid.sum())}" in out.stdout
+    ndof = 3 * int(problems.compact_ids(solid).max() + 1)
+    assert f"Degrees of freedom: {ndof}" in out.stdout
+    m = re.search(r"true relative residual (\S+)\)", out.stdout)
+    assert m and 1e-10 < float(m.group(1)) < 1e-6, out.stdout
+    f = problems.apply_boundary_conditions(ndof, loaded, loads)
+    ora = O.OracleSystem(solid, ke0, fixed, 3)
+    u_ref, _, _ = ora.pcg(f, tol=1e-12, precond=O.DAMPED)
+    ora.close()
+    c_ref = float(f @ u_ref)
+    c = float(re.search(r"Compliance:\s+(\S+)", out.stdout).group(1))
+    assert abs(c - c_ref) <= 1e-4 * abs(c_ref), (c, c_ref)

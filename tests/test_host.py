@@ -165,3 +165,13 @@ def test_generators_reject_empty_models():
         problems.patch_load_cases(empty, 1, seed=0)
     with pytest.raises(ValueError):
         problems.batched_problem(32, 20, 2.0, 9.0, seed=107, nrhs=1)   # no ellipsoid reaches z = 0
+
+
+def test_level0_node_limit_is_rejected(ke0):
+    """Node ids travel packed with two flag bits in 32-bit words on the device,
+    so a level 0 of 2^29 used nodes or more must fail cleanly at setup
+    (1022 x 1022 x 513 voxels: 1023 * 1023 * 514 = 537,907,506 nodes)."""
+    vox = np.ones((513, 1022, 1022), np.int32)
+    with pytest.raises(V.VfemError) as e:
+        V.inspect_hierarchy(vox, ke0, np.zeros(0, np.int64), 2, 0, 0)
+    assert e.value.status == V.api.ERR_ARGUMENT and "2^29" in str(e.value)

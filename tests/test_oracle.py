@@ -108,6 +108,49 @@ def test_additive_vcycle_matches_explicit_formula(ke0, small_case):
     np.testing.assert_allclose(z, zb, rtol=1e-10, atol=1e-12 * np.abs(zb).max())
 
 
+@pytest.fixture(scope="module")
+def damped_cases(ke0, small_case):
+    """The 14^3 ellipsoid union (3 levels) and a skip-free 16^3 one (3 levels)."""
+    vox16, _ = problems.ellipsoid_union(16, 9, 2.0, 6.5, 77)
+    ids = problems.compact_ids(vox16)
+    return [small_case, (vox16, np.sort(ids[0][ids[0] >= 0]))]
+
+
+def test_damped_level_operators_match_galerkin_brute_force(ke0, damped_cases):
+    """The damped cycle's coarse operators: level l >= 1 sums, per element, the
+    unique Galerkin K of its children (taken full with 2^(l-1) Ke,
+    FEM.cpp:364-395,463) or 2^l Ke on a full element; both must equal the
+    explicit P^T A P of the brute-force hierarchy, fixed rows zeroed.  A wrong
+    coarse scale, child pattern or transfer pair fails here."""
+    rng = np.random.default_rng(11)
+    for vox, fixed in damped_cases:
+        s = O.OracleSystem(vox, ke0, fixed, 3)
+        bf = BF.Hierarchy(vox, ke0, fixed, 3)
+        for l in range(1, 3):
+            x = rng.standard_normal(3 * s.nodes(l))
+            ref = bf.level_operator(l) @ x
+            y = s.level_matvec(l, x)
+            assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), l
+        kc = bf.kc_galerkin()
+        assert np.abs(s.kc(1) - kc).max() <= 1e-12 * np.abs(kc).max()
+        s.close()
+
+
+@pytest.mark.parametrize("omega", [0.6, 0.5])
+def test_damped_vcycle_matches_explicit_v11(ke0, damped_cases, omega):
+    """vcycle_damped_level against V(1,1) written out with the brute-force
+    P, A and 1/diag(A) (no shared code)."""
+    rng = np.random.default_rng(12)
+    for vox, fixed in damped_cases:
+        s = O.OracleSystem(vox, ke0, fixed, 3)
+        bf = BF.Hierarchy(vox, ke0, fixed, 3)
+        r = rng.standard_normal(s.ndof)
+        z = s.vcycle(r, O.DAMPED, omega)
+        zb = bf.vcycle_damped(r, omega)
+        assert np.abs(z - zb).max() <= 1e-12 * np.abs(zb).max()
+        s.close()
+
+
 def test_damped_vcycle_is_symmetric_positive(ke0, small_case):
     vox, fixed = small_case
     s = O.OracleSystem(vox, ke0, fixed, 3)

@@ -73,3 +73,25 @@ def test_bf16_rounding():
     r = U.bf16_round(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -7      # ties to even
     assert abs(r[3] + 3.140625) < 1e-12
+
+
+def test_conv_and_upsample_match_torch():
+    """The oracle's convolution against torch.nn.functional.conv3d (zero
+    padding 1, stride 1 and 2; 1x1x1 head), and its upsampling against
+    trilinear interpolation with aligned corners where the fine extent is
+    2m - 1 (node-aligned by construction)."""
+    torch = pytest.importorskip("torch")
+    F = torch.nn.functional
+    rng = np.random.default_rng(4)
+    for stride, k, shape in ((1, 3, (2, 5, 9, 7, 11)), (2, 3, (2, 5, 9, 7, 10)), (2, 3, (1, 3, 5, 4, 3)),
+                             (1, 1, (2, 16, 6, 5, 4))):
+        x = rng.standard_normal(shape)
+        w = rng.standard_normal((4, shape[1], k, k, k))
+        ref = F.conv3d(torch.from_numpy(x), torch.from_numpy(w), stride=stride, padding=k // 2).numpy()
+        y = U.conv3d(x, w, stride)
+        assert y.shape == ref.shape
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    c = rng.standard_normal((2, 3, 4, 5, 3))
+    size = (7, 9, 5)
+    ref = F.interpolate(torch.from_numpy(c), size=size, mode="trilinear", align_corners=True).numpy()
+    assert np.abs(U.upsample(c, size) - ref).max() <= 1e-13

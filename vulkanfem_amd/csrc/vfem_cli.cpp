@@ -11,12 +11,14 @@
  *              [--omega w] [--tol t] [--no-save]
  *
  * Files, relative to the working directory as in the reference:
- *   <model>.dat          "X Y Z" then Z lines of X*Y comma-separated 0/1 (x fastest)
+ *   <model>.dat          "X Y Z" then Z lines of X*Y comma-separated integers (x fastest);
+ *                        a voxel is solid only where its value is 1 (src/FEM.cpp:75)
  *   <model>_Ke0.dat      24 lines of 24 comma-separated values (else data/Ke0.dat)
  *   <model>_fixed.dat    count, then one 1-based node id per line
  *   <model>_loading.dat  count, then "id<TAB>fx<TAB>fy<TAB>fz" (1-based id)
  * Options beyond the reference's (-n, --skip) select the solver plan; the
- * defaults reproduce the reference: FP64 and its additive cycle.
+ * defaults reproduce the reference: FP64, its additive cycle and its compiled
+ * tolerance 1e-6 (include/FEMDefines.hpp:7).
  */
 #include <chrono>
 #include <cstdio>
@@ -73,7 +75,7 @@ std::vector<int> read_voxels(const std::string &model, int &nx, int &ny, int &nz
         std::stringstream ss(line);
         for (int j = 0; j < nx * ny; j++) {
             if (!std::getline(ss, tok, ',')) fail("short voxel layer " + std::to_string(z) + " in " + model + ".dat");
-            v[k++] = std::atoi(tok.c_str()) != 0 ? 1 : 0;
+            v[k++] = std::atoi(tok.c_str());  // passed through: only 1 is solid (src/FEM.cpp:75)
         }
     }
     return v;
@@ -182,7 +184,7 @@ int main(int argc, char **argv)
             vfem::assembleSystemMatrix<double>(voxels.data(), vfem::Vec3i{nx, ny, nz}, ke, fixed, a.levels, a.skip);
         const int64_t ndof = K.rows();
         long long solid = 0;
-        for (int v : voxels) solid += v;
+        for (int v : voxels) solid += (v == 1);
         std::cout << "System statistics:" << std::endl;
         std::cout << "\tVoxel model resolution: " << nx << " x " << ny << " x " << nz << std::endl;
         std::cout << "\tElements: " << solid << std::endl;

@@ -3127,6 +3127,9 @@ struct LevelStencil {
     bool dcon_interior = false; // every used, non-fixed node outside the surface list has D^-1 = dcon
 };
 
+// captured-iteration cache key: the plan's discrete choices and omega's bits
+using GraphKey = std::pair<int64_t, uint64_t>;
+
 struct vfem_system {
     Hierarchy H;
     int device = 0;
@@ -3164,7 +3167,7 @@ struct vfem_system {
     // probe_n < probe_cap, each launch is bracketed by events probe_ev[2n], [2n+1]
     hipEvent_t *probe_ev = nullptr;
     int probe_n = 0, probe_cap = 0;
-    std::map<int64_t, hipGraphExec_t> graphs;
+    std::map<GraphKey, hipGraphExec_t> graphs;  // captured CG iterations by plan
     double setup_ms = 0;
     bool use_mfma = true;       // FP32 level-0 matvec on the matrix cores
     bool use_mfma64 = true;     // FP64 level-0 matvec on the matrix cores (exact FP64 products)
@@ -4337,15 +4340,17 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
 {
     int64_t key = P.precision * 16 + P.cycle * 4 + (with_true ? 1 : 0) + (P.bf16 ? 2 : 0);
     key += 1024 * (P.op_passes + 8 * P.cycle_passes);
-    // omega is baked into the graph's kernel arguments
-    key += std::llround(P.omega * 1000) * 1024 * 64;
     key += (int64_t)S->ncols << 40;
     key += (int64_t)(P.mfma64 ? 1 : 0) << 50;
     key += (int64_t)(P.learned ? 1 : 0) << 51;
     key += (int64_t)(P.force_flex ? 1 : 0) << 52;
     key += (int64_t)(P.op64 ? 1 : 0) << 53;
     key += (int64_t)(P.coarse64 ? 1 : 0) << 54;
-    auto it = S->graphs.find(key);
+    // omega is baked into the graph's kernel arguments: the cache key holds its bit pattern
+    uint64_t obits;
+    std::memcpy(&obits, &P.omega, sizeof obits);
+    const GraphKey gk{key, obits};
+    auto it = S->graphs.find(gk);
     if (it != S->graphs.end()) {
         *out = it->second;
         return 0;
@@ -4357,7 +4362,7 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     hipGraphExec_t ge;
     HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     HIP_TRY(hipGraphDestroy(g));
-    S->graphs[key] = ge;
+    S->graphs[gk] = ge;
     *out = ge;
     return 0;
 }

@@ -9,6 +9,9 @@
 namespace vfem {
 
 constexpr int kMaxLevels = 16;
+// largest level-0 node count: device kernels pack a node id with two flag bits
+// into an int32 and index DOFs as 3 * id
+constexpr int64_t kMaxNodes = (int64_t)1 << 29;
 
 // One multigrid level, stored as a grid: every node row (y, z) of the level's
 // node grid is split into 64-node segments.  A segment holds a 64-bit mask of

@@ -344,6 +344,12 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
     pack_elements(*F, present);
     std::vector<uint8_t> used = used_from_elements(*F, present);
     F->nn = pack_nodes(*F, used);
+    // the device kernels pack a node id with two flag bits into an int32
+    // (id << 2 | used << 1 | fixed) and index DOFs as 3 * id in 32 bits
+    if (F->nn >= kMaxNodes) {
+        err = "more than 2^29 - 1 used nodes on level 0 are not supported";
+        return 1;
+    }
     std::vector<int32_t> ids = node_ids(*F, used);
     // fixed compact ids -> grid bits
     {
