@@ -230,6 +230,20 @@ int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_o
 /* Replaces shaders/VecDotVec.comp: *out = a . b over vfem_num_dofs entries (FP64). */
 int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out);
 
+/* Allocation-free variant of vfem_dot (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  *out = a . b over n
+ * entries, a and b DEVICE pointers to n doubles on the current device, out a
+ * host pointer; no vfem_system is needed.  The same kernels as vfem_dot (same
+ * grid for the same n, so the same result bit for bit), run on the current
+ * device's null stream; synchronous.  It calls no hipMalloc, hipFree or
+ * allocating routine: the block partials and the finalizer's state live in a
+ * fixed per-device scratch array of 32.9 KB (2 x 2048 partials + one state)
+ * that stays allocated until the process ends.  The grid is capped at 2048
+ * blocks, so any n >= 0 fits it.  Every call on a device shares that device's
+ * scratch: calls that overlap on one device (other host threads) race. */
+int vfem_dot_noalloc(const double *a, const double *b, int64_t n, double *out);
+
 /* Kernel timing hook (no reference counterpart): runs `reps` launches of one
  * kernel on the library's stream between two hipEvents and returns the mean
  * duration.  which: 0 = level-0 matvec of the CG operator in the form chosen by

@@ -113,6 +113,7 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int32)]
     L.vfem_precondition.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options)]
     L.vfem_dot.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_double)]
+    L.vfem_dot_noalloc.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     L.vfem_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.POINTER(ctypes.c_double)]
     L.vfem_inspect_hierarchy.argtypes = [_i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _f64p, _i64p,
@@ -474,6 +475,14 @@ def solve_with_hip(A, f, u, **opts):
     sol, st = A.solve(f, **opts)
     u[:] = sol
     return st
+
+
+def dot_noalloc(a_ptr, b_ptr, n):
+    """a . b over n doubles at device addresses a_ptr, b_ptr (vfem_dot_noalloc:
+    no device allocation, fixed per-device scratch)."""
+    out = ctypes.c_double()
+    _check(lib().vfem_dot_noalloc(ctypes.c_void_p(a_ptr), ctypes.c_void_p(b_ptr), int(n), ctypes.byref(out)))
+    return out.value
 
 
 def inspect_hierarchy(voxels, ke, fixed_nodes, num_levels, skip_levels, level):

@@ -3035,6 +3035,18 @@ __global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int
     }
 }
 
+// Scratch of the allocation-free dot product (vfem_dot_noalloc): the block
+// partials of its two-launch reduction (two sums of kNoAllocBlocks blocks) and
+// one CG state for the finalizer.  One copy per device, held until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device.
+constexpr int kNoAllocBlocks = 2048;
+struct NoAllocScratch {
+    double partials[2 * kNoAllocBlocks];
+    CGState st;
+};
+static __device__ __align__(256) NoAllocScratch scratch[1];
+
 }  // namespace vfem
 
 // ===========================================================================
@@ -4563,6 +4575,28 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
     return worst;
 }
 
+// Device address of `scratch` for each device, looked up once per device with
+// that device current.  The array's name is a host-side stand-in and must never
+// be passed to a kernel or a copy.
+std::mutex g_scratch_mu;
+std::map<int, NoAllocScratch *> g_scratch_ptr;
+
+int noalloc_scratch_ready()
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(dev)) return VFEM_OK;
+    void *ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        g_last_error = "hipGetSymbolAddress returned a null scratch address";
+        return VFEM_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = (NoAllocScratch *)ptr;
+    return VFEM_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -5177,6 +5211,39 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
     CGState h{};
     if (int rc = read_state(sys, h)) return rc;
     *out = h.result;
+    return VFEM_OK;
+}
+
+int vfem_dot_noalloc(const double *a, const double *b, int64_t n, double *out)
+{
+    if (!a || !b || !out || n < 0) {
+        g_last_error = "invalid argument (a, b: device pointers; out: host pointer; n >= 0)";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (int rc = noalloc_scratch_ready()) return rc;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    NoAllocScratch *sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        sc = g_scratch_ptr[dev];
+    }
+    const int gf = grid_for_flat(n);
+    if (gf > kNoAllocBlocks) {  // grid_for_flat caps the grid at 2048 blocks
+        g_last_error = "dot grid exceeds the scratch capacity";
+        return VFEM_ERR_ARGUMENT;
+    }
+    Reduction red{sc->partials, nullptr, kNoAllocBlocks};
+    CGState *st = &sc->st;
+    // the state keeps the previous call's contents: reset it (no_stop, done) first
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, 0, st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, 0, n, a, b, st, red, (int)FIN_STORE, 0);
+    hipLaunchKernelGGL(k_finalize, dim3(1, 1), dim3(kBlock), 0, 0, red, gf, st, (int)FIN_STORE, 0);
+    HIP_TRY(hipGetLastError());
+    double v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, &st->result, sizeof(double), hipMemcpyDeviceToHost, 0));
+    HIP_TRY(hipStreamSynchronize(0));
+    *out = v;
     return VFEM_OK;
 }
 
