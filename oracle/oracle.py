@@ -46,7 +46,7 @@ def lib():
     L.vo_restrict.argtypes = [vp, ctypes.c_int, _f64p, _f64p]
     L.vo_interpolate.argtypes = [vp, ctypes.c_int, _f64p, _f64p]
     L.vo_pcg.argtypes = [vp, _f64p, _f64p, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                         ctypes.POINTER(ctypes.c_double)]
+                         ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     L.vo_pcg.restype = ctypes.c_int
     L.vo_num_levels.argtypes = [vp]
     L.vo_num_levels.restype = ctypes.c_int
@@ -229,11 +229,12 @@ class OracleSystem:
         self._lib.vo_vcycle(self._h, r, z, int(mode), float(omega))
         return z
 
-    def pcg(self, b, tol=1e-8, max_iter=30000, precond=REF_ADDITIVE, omega=0.6):
-        """Returns (x, iterations, final recursive relative residual)."""
+    def pcg(self, b, tol=1e-8, max_iter=30000, precond=REF_ADDITIVE, omega=0.6, x0=None):
+        """Returns (x, iterations, final recursive relative residual); x0 = start
+        vector (Eigen's solveWithGuess), zero by default."""
         b = np.ascontiguousarray(b, dtype=np.float64)
-        x = np.zeros(self.ndof)
+        x = np.zeros(self.ndof) if x0 is None else np.array(x0, dtype=np.float64)
         rr = ctypes.c_double()
         it = self._lib.vo_pcg(self._h, b, x, float(tol), int(max_iter), int(precond), float(omega),
-                              ctypes.byref(rr))
+                              ctypes.byref(rr), 0 if x0 is None else 1)
         return x, int(it), float(rr.value)

@@ -779,18 +779,24 @@ static double dot(const double *a, const double *b, int64_t n)
 }
 
 /* Eigen 3.4 ConjugateGradient semantics (FEM.cpp:607-628; loop as mirrored by
- * FEMGPU.cpp:416-541).  x starts at zero.  precond: 0/1/2 as vo_vcycle mode,
+ * FEMGPU.cpp:416-541).  x starts at zero, or at the x passed in when warm != 0
+ * (Eigen's solveWithGuess: r = b - A x0).  precond: 0/1/2 as vo_vcycle mode,
  * -1 = identity (IdentityPreconditioner, FEM.cpp:610).  Returns iterations. */
 int vo_pcg(const vo_sys *s, const double *b, double *x, double tol, int maxit, int precond, double omega,
-           double *relres_out)
+           double *relres_out, int warm)
 {
     int64_t n = 3 * s->L[0].nn;
-    memset(x, 0, sizeof(double) * n);
+    if (!warm) memset(x, 0, sizeof(double) * n);
     double rhs2 = dot(b, b, n);
-    if (rhs2 == 0) { if (relres_out) *relres_out = 0; return 0; }
+    if (rhs2 == 0) { memset(x, 0, sizeof(double) * n); if (relres_out) *relres_out = 0; return 0; }
     double *r = (double *)malloc(sizeof(double) * n), *p = (double *)malloc(sizeof(double) * n);
     double *z = (double *)malloc(sizeof(double) * n), *t = (double *)malloc(sizeof(double) * n);
-    memcpy(r, b, sizeof(double) * n); /* r = b - A*0 */
+    if (warm) {
+        vo_matvec(s, x, t);
+        for (int64_t k = 0; k < n; k++) r[k] = b[k] - t[k];
+    } else {
+        memcpy(r, b, sizeof(double) * n); /* r = b - A*0 */
+    }
     double threshold = tol * tol * rhs2;
     if (threshold < DBL_MIN) threshold = DBL_MIN;
     double r2 = dot(r, r, n);

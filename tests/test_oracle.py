@@ -283,3 +283,20 @@ def test_default_omega_keeps_damped_jacobi_convergent(ke0):
     lam = max(lams)
     assert 3.4 < lam < 4.0, lams
     assert 0.5 * lam < 2.0 < 0.6 * lam, lams
+
+
+def test_pcg_warm_start(goldens, ke0):
+    """Eigen's solveWithGuess: r = b - A x0.  Started at the direct solution it
+    stops before any iteration; started from a perturbed one it reaches the
+    same solution as the cold start, in fewer iterations."""
+    s = O.OracleSystem(np.ones((16, 16, 32), np.int32), ke0, goldens["box_fixed"], 3)
+    f, u = goldens["box_f"], goldens["box_u"]
+    _, it, rr = s.pcg(f, tol=1e-8, precond=O.DAMPED, x0=u)
+    assert it == 0
+    x_cold, it_cold, _ = s.pcg(f, tol=1e-12, precond=O.DAMPED)
+    x0 = u * (1 + 1e-4 * np.random.default_rng(0).standard_normal(u.size))
+    x0[np.repeat(s.fixed(0), 3)] = 0
+    x_warm, it_warm, rr = s.pcg(f, tol=1e-12, precond=O.DAMPED, x0=x0)
+    assert rr < 1e-12 and it_warm < it_cold
+    assert np.linalg.norm(x_warm - u) < 1e-9 * np.linalg.norm(u)
+    s.close()
