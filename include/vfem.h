@@ -113,9 +113,11 @@ typedef struct {
 void vfem_default_options(vfem_options *o);
 
 /* Replaces assembleSystemMatrix<double>(voxelModel, dims, Ke, fixedNodes, numLevels,
- * skipLevels) (include/FEM.hpp:33-34, src/FEM.cpp:157-517): numbers the nodes,
- * builds the multigrid hierarchy (levels may stop early, FEM.cpp:465-470) and
- * uploads it to the current HIP device.  fixed_nodes: compact node ids.
+ * skipLevels) (include/FEM.hpp:33-34, src/FEM.cpp:157-517): numbers the nodes and
+ * builds the multigrid hierarchy (levels may stop early, FEM.cpp:465-470) on the
+ * current HIP device -- element and node masks, compact numbering, fixed-node
+ * propagation and inverse diagonals by grid kernels, the coarsest-level inverses
+ * by blocked Gauss-Jordan elimination.  fixed_nodes: compact node ids.
  * skip_levels: 0 or 1 (FEM.cpp:266-272). */
 int vfem_assemble(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, const double *ke,
                   const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
@@ -277,6 +279,13 @@ int vfem_inspect_hierarchy(const int32_t *voxels, int32_t nx, int32_t ny, int32_
                            const int64_t *fixed_nodes, int64_t n_fixed, int32_t num_levels, int32_t skip_levels,
                            int32_t level, int32_t *n_levels_out, int64_t *n_nodes, int64_t *n_fixed_out,
                            double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out);
+
+/* The hierarchy vfem_assemble built on the device (test hook; no reference
+ * counterpart): for `level`, the inverse diagonal (3 per node, compact order)
+ * and the fixed-node mask (1 byte per node), copied from the device when the
+ * pointers are non-null, and the free coarsest DOFs.  Equal, bit for bit, to
+ * what vfem_inspect_hierarchy builds on the host for the same input. */
+int vfem_level_data(vfem_system *sys, int32_t level, double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out);
 
 /* ---- learned coarse-grid correction (no reference counterpart) --------------
  * A 3-D conv U-Net (4 resolutions, 16-64 channels, 274,800 weights, BF16

@@ -113,6 +113,7 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int32)]
     L.vfem_precondition.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options)]
     L.vfem_dot.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_double)]
+    L.vfem_level_data.argtypes = [vp, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
     L.vfem_dot_noalloc.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     L.vfem_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                    ctypes.POINTER(ctypes.c_double)]
@@ -269,6 +270,16 @@ class MatrixFreeSparse:
     @property
     def numElements(self):  # the reference's name for the DOF count (FEM.cpp:186,221)
         return self.n
+
+    def level_data(self, level):
+        """The device-built hierarchy of `level`: inverse diagonal, fixed mask, free coarsest DOFs."""
+        nn = self.level_info(level)["nodes"]
+        inv = np.zeros(3 * nn)
+        mask = np.zeros(nn, np.uint8)
+        nfree = ctypes.c_int64()
+        _check(self._lib.vfem_level_data(self._h, int(level), inv.ctypes.data_as(ctypes.c_void_p),
+                                         mask.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nfree)))
+        return {"invdiag": inv, "fixed": mask.astype(bool), "free_dofs": nfree.value}
 
     def level_info(self, level):
         ne, nn, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvfem.so")
 CLI = os.path.join(HERE, "vfem_cli")
 
-SOURCES = ["vfem_setup.cpp", "vfem_device.hip", "vfem_unet.hip"]
+SOURCES = ["vfem_setup.cpp", "vfem_setup_device.hip", "vfem_device.hip", "vfem_unet.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

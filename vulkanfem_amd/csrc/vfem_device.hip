@@ -3035,6 +3035,85 @@ __global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int
     }
 }
 
+// ---- setup of the stencil form: surface lists and tile lists ------------------
+
+// Surface nodes of level L (used, not fixed, with an absent element around them
+// or, on a coarse level, a partial one), in compact order.  One wavefront per
+// node segment word.  Pass 1 (OUT = false) counts them per word and flags (in
+// *dbad) a used, non-fixed node outside the list whose D^-1 differs from dc;
+// pass 2 writes entry k = off[word] + rank at e[k], rb[r * n + k], pat[k].
+template <bool OUT>
+__global__ __launch_bounds__(kBlock) void k_surface(DevLevel L, int coarse, const double *__restrict__ dinv, double dc0,
+                                                     double dc1, double dc2, int32_t *__restrict__ cnt,
+                                                     const int32_t *__restrict__ off, int64_t n, int4 *__restrict__ e,
+                                                     int32_t *__restrict__ rb, uint2 *__restrict__ pat, int *dbad)
+{
+    const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (wid >= (int64_t)L.ny * L.nz * L.S) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int64_t row = wid / L.S;
+    const int x = 64 * (int)(wid % L.S) + lane, y = (int)(row % L.ny), z = (int)(row / L.ny);
+    const uint64_t w = L.nbits[wid];
+    const bool used = (w >> lane) & 1;
+    const bool fixed = (L.fbits[wid] >> lane) & 1;
+    unsigned present = 0, full = 0;
+    uint32_t pw[2] = {0, 0};
+    if (used)
+        for (int q = 0; q < 8; q++) {
+            const int ex = x - 1 + (q & 1), ey = y - 1 + ((q >> 1) & 1), ez = z - 1 + (q >> 2);
+            unsigned pe = 0;
+            if (ex >= 0 && ey >= 0 && ez >= 0 && ex < L.ex && ey < L.ey && ez < L.ez) {
+                if (coarse) pe = L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex];
+                else pe = ((L.ebits[((int64_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) ? 255u : 0u;
+            }
+            if (pe) present |= 1u << q;
+            if (pe == 255) full |= 1u << q;
+            pw[q >> 2] |= pe << (8 * (q & 3));
+        }
+    const bool surf = used && !fixed && full != 255u;
+    const uint64_t sb = __ballot(surf);
+    const int64_t id = L.nbase[wid] + __popcll(w & ((1ull << lane) - 1));
+    if constexpr (!OUT) {
+        if (lane == 0) cnt[wid] = __popcll(sb);
+        if (used && !fixed && !surf &&
+            (dinv[3 * id] != dc0 || dinv[3 * id + 1] != dc1 || dinv[3 * id + 2] != dc2))
+            atomicOr(dbad, 1);
+        return;
+    }
+    if (!surf) return;
+    const int64_t k = off[wid] + __popcll(sb & ((1ull << lane) - 1));
+    unsigned usedb = 0;
+    for (int r = 0; r < 9; r++) {
+        const int yy = y - 1 + r % 3, zz = z - 1 + r / 3;
+        int32_t b = 0;
+        if (yy >= 0 && yy < L.ny && zz >= 0 && zz < L.nz) {
+            const int64_t rw = ((int64_t)zz * L.ny + yy) * L.S;
+            const int xs = x > 0 ? x - 1 : 0;
+            const uint64_t ww = L.nbits[rw + (xs >> 6)];
+            b = L.nbase[rw + (xs >> 6)] + __popcll(ww & ((1ull << (xs & 63)) - 1));
+            for (int dx = 0; dx < 3; dx++) {
+                const int xx = x - 1 + dx;
+                if (xx >= 0 && xx < L.nx && ((L.nbits[rw + (xx >> 6)] >> (xx & 63)) & 1)) usedb |= 1u << (3 * r + dx);
+            }
+        }
+        rb[r * n + k] = b;
+    }
+    e[k] = make_int4((int)id, (int)present, (int)usedb, 0);
+    if (coarse) pat[k] = make_uint2(pw[0], pw[1]);
+}
+
+// busy[tile] = 1 for the bx x by x bz node tiles of level L that hold a used node
+__global__ __launch_bounds__(kBlock) void k_tiles_busy(DevLevel L, int bx, int by, int bz, int tx, int ty,
+                                                        uint8_t *__restrict__ busy)
+{
+    const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (wid >= (int64_t)L.ny * L.nz * L.S) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = wid / L.S;
+    const int x = 64 * (int)(wid % L.S) + lane, y = (int)(row % L.ny), z = (int)(row / L.ny);
+    if ((L.nbits[wid] >> lane) & 1) busy[((int64_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
+}
+
 // Scratch of the allocation-free dot product (vfem_dot_noalloc): the block
 // partials of its two-launch reduction (two sums of kNoAllocBlocks blocks) and
 // one CG state for the finalizer.  One copy per device, held until the process
@@ -3161,7 +3240,6 @@ struct vfem_system {
     double *ktab64[kMaxLevels] = {};
     float *ktab32[kMaxLevels] = {};
     double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
-    float *kinv_ref32 = nullptr, *kinv_gal32 = nullptr;
     int32_t *freedof = nullptr;
     int64_t n3 = 0;
     // level-0 vectors
@@ -3249,36 +3327,50 @@ int build_schedule(vfem_system *S)
     const HostLevel &L = S->H.L[0];
     MfmaTiles T = mfma_tiles(S->dl[0]);
     const int64_t ntiles = (int64_t)T.tx * T.ty * T.tz;
-    std::vector<uint32_t> off(ntiles * MSTEPS + 1, 0);
-    std::vector<uint16_t> list;
-    list.reserve((size_t)L.ne * 5 / 4 + 64);
     auto present = [&](int ex, int ey, int ez) {
         if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return false;
         return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) != 0;
     };
-    for (int tz = 0; tz < T.tz; tz++)
-        for (int ty = 0; ty < T.ty; ty++)
-            for (int tx = 0; tx < T.tx; tx++) {
-                const int64_t t = ((int64_t)tz * T.ty + ty) * T.tx + tx;
-                const int x0 = tx * MT_X, y0 = ty * MT_Y, zb = tz * MZ, ze = std::min(zb + MZ, L.nz);
-                for (int st = 0; st < MSTEPS; st++) {
-                    off[t * MSTEPS + st] = (uint32_t)list.size();
-                    const int ez = zb - 1 + st;
-                    if (ez >= ze) continue;
-                    for (int eyl = 0; eyl < ME_Y; eyl++)
-                        for (int exl = 0; exl < ME_X - 1; exl++)
-                            if (present(x0 - 1 + exl, y0 - 1 + eyl, ez)) list.push_back((uint16_t)(eyl << 6 | exl));
-                }
-            }
-    off[ntiles * MSTEPS] = (uint32_t)list.size();
-    if (list.size() >= (size_t)UINT32_MAX) {
-        g_last_error = "element schedule too large";
-        return VFEM_ERR_ARGUMENT;
+    // the present elements of tile t, layer step st, in (ey, ex) order; counted
+    // first, then written at the prefix offsets (tiles on host threads)
+    auto visit = [&](int64_t t, int st, auto &&emit) {
+        const int tx = (int)(t % T.tx), ty = (int)((t / T.tx) % T.ty), tz = (int)(t / ((int64_t)T.tx * T.ty));
+        const int x0 = tx * MT_X, y0 = ty * MT_Y, zb = tz * MZ, ze = std::min(zb + MZ, L.nz);
+        const int ez = zb - 1 + st;
+        if (ez >= ze) return;
+        for (int eyl = 0; eyl < ME_Y; eyl++)
+            for (int exl = 0; exl < ME_X - 1; exl++)
+                if (present(x0 - 1 + exl, y0 - 1 + eyl, ez)) emit((uint16_t)(eyl << 6 | exl));
+    };
+    std::vector<uint32_t> cnt(ntiles * MSTEPS, 0);
+    parallel_for(ntiles, [&](int64_t t) {
+        for (int st = 0; st < MSTEPS; st++) {
+            uint32_t c = 0;
+            visit(t, st, [&](uint16_t) { c++; });
+            cnt[t * MSTEPS + st] = c;
+        }
+    });
+    std::vector<uint32_t> off(ntiles * MSTEPS + 1, 0);
+    uint64_t tot = 0;
+    for (int64_t i = 0; i < ntiles * MSTEPS; i++) {
+        off[i] = (uint32_t)tot;
+        tot += cnt[i];
+        if (tot >= (uint64_t)UINT32_MAX) {
+            g_last_error = "element schedule too large";
+            return VFEM_ERR_ARGUMENT;
+        }
     }
+    off[ntiles * MSTEPS] = (uint32_t)tot;
+    std::vector<uint16_t> list(std::max<uint64_t>(tot, 1), 0);
+    parallel_for(ntiles, [&](int64_t t) {
+        for (int st = 0; st < MSTEPS; st++) {
+            uint32_t k = off[t * MSTEPS + st];
+            visit(t, st, [&](uint16_t v) { list[k++] = v; });
+        }
+    });
     uint32_t *doff;
     uint16_t *dlist;
     if (int rc = S->upload(&doff, off)) return rc;
-    if (list.empty()) list.push_back(0);
     if (int rc = S->upload(&dlist, list)) return rc;
     T.off = doff;
     T.list = dlist;
@@ -3286,22 +3378,27 @@ int build_schedule(vfem_system *S)
     return 0;
 }
 
-// blocks (tx | ty << 11 | tz << 22) of a tiling of level L into bx x by x bz
-// nodes that hold a used node
-int upload_tiles(vfem_system *S, const HostLevel &L, int bx, int by, int bz, StencilTiles &out)
+// blocks (tx | ty << 11 | tz << 22) of a tiling of level l into bx x by x bz
+// nodes that hold a used node, in ascending (tz, ty, tx) order
+int upload_tiles(vfem_system *S, int l, int bx, int by, int bz, StencilTiles &out)
 {
+    const HostLevel &L = S->H.L[l];
     const int tx = (L.nx + bx - 1) / bx, ty = (L.ny + by - 1) / by, tz = (L.nz + bz - 1) / bz;
-    std::vector<uint8_t> busy((size_t)tx * ty * tz, 0);
-    for (int z = 0; z < L.nz; z++)
-        for (int y = 0; y < L.ny; y++)
-            for (int s = 0; s < L.S; s++) {
-                uint64_t w = L.nbits[((size_t)z * L.ny + y) * L.S + s];
-                while (w) {
-                    const int x = 64 * s + __builtin_ctzll(w);
-                    w &= w - 1;
-                    busy[((size_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
-                }
-            }
+    const size_t nt = (size_t)tx * ty * tz;
+    uint8_t *dbusy = nullptr;
+    HIP_TRY(hipMalloc(&dbusy, nt));
+    std::vector<uint8_t> busy(nt);
+    hipError_t e = hipMemsetAsync(dbusy, 0, nt, S->stream);
+    const int64_t nseg = (int64_t)L.ny * L.nz * L.S;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_tiles_busy, dim3((int)((nseg + kWaves - 1) / kWaves)), dim3(kBlock), 0, S->stream, S->dl[l],
+                           bx, by, bz, tx, ty, dbusy);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(busy.data(), dbusy, nt, hipMemcpyDeviceToHost, S->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
+    (void)hipFree(dbusy);
+    HIP_TRY(e);
     std::vector<uint32_t> tiles;
     for (int k = 0; k < tz; k++)
         for (int j = 0; j < ty; j++)
@@ -3477,75 +3574,56 @@ int build_level_stencil(vfem_system *S, int l)
     LS.zc = L.nn < kStencilShortNodes ? kStencilZShort : kStencilZ;
     // tile indices are packed in 11 + 11 + 10 bits
     if ((L.nx + SF_X - 1) / SF_X > 2047 || (L.ny + ST_Y - 1) / ST_Y > 2047 || (L.nz + LS.zc - 1) / LS.zc > 1023) return 0;
-    if (int rc = upload_tiles(S, L, ST_X, ST_Y, LS.zc, LS.tiles)) return rc;
+    if (int rc = upload_tiles(S, l, ST_X, ST_Y, LS.zc, LS.tiles)) return rc;
     if (LS.sf)
-        if (int rc = upload_tiles(S, L, SF_X, ST_Y, LS.zc, LS.tiles_sf)) return rc;
-    std::vector<int4> surf;
-    std::vector<uint2> pats;
-    std::vector<int32_t> rb[9];
-    // child pattern of element (ex, ey, ez): 0 absent, 255 full
-    auto pattern = [&](int ex, int ey, int ez) -> unsigned {
-        if (ex < 0 || ey < 0 || ez < 0 || ex >= L.ex || ey >= L.ey || ez >= L.ez) return 0;
-        if (coarse) return L.epat[((size_t)ez * L.ey + ey) * L.ex + ex];
-        return ((L.ebits[((size_t)ez * L.ey + ey) * L.SE + (ex >> 6)] >> (ex & 63)) & 1) ? 255u : 0u;
-    };
-    int64_t id = 0;
-    bool dcon_ok = true;
-    for (int z = 0; z < L.nz; z++)
-        for (int y = 0; y < L.ny; y++)
-            for (int x = 0; x < L.nx; x++) {
-                const size_t seg = ((size_t)z * L.ny + y) * L.S + (x >> 6);
-                if (!((L.nbits[seg] >> (x & 63)) & 1)) continue;
-                const bool fixed = (L.fbits[seg] >> (x & 63)) & 1;
-                unsigned present = 0, full = 0;
-                uint32_t pw[2] = {0, 0};
-                for (int e = 0; e < 8; e++) {
-                    const unsigned pe = pattern(x - 1 + (e & 1), y - 1 + ((e >> 1) & 1), z - 1 + (e >> 2));
-                    if (pe) present |= 1u << e;
-                    if (pe == 255) full |= 1u << e;
-                    pw[e >> 2] |= pe << (8 * (e & 3));
-                }
-                if (full != 255u && !fixed) {
-                    unsigned used = 0;
-                    for (int r = 0; r < 9; r++) {
-                        const int yy = y - 1 + r % 3, zz = z - 1 + r / 3;
-                        int32_t b = 0;
-                        if (yy >= 0 && yy < L.ny && zz >= 0 && zz < L.nz) {
-                            const size_t row = ((size_t)zz * L.ny + yy) * L.S;
-                            const int xs = std::max(x - 1, 0);
-                            const uint64_t w = L.nbits[row + (xs >> 6)];
-                            b = L.nbase[row + (xs >> 6)] + __builtin_popcountll(w & ((1ull << (xs & 63)) - 1));
-                            for (int dx = 0; dx < 3; dx++) {
-                                const int xx = x - 1 + dx;
-                                if (xx >= 0 && xx < L.nx && ((L.nbits[row + (xx >> 6)] >> (xx & 63)) & 1)) used |= 1u << (3 * r + dx);
-                            }
-                        }
-                        rb[r].push_back(b);
-                    }
-                    surf.push_back(make_int4((int)id, (int)present, (int)used, 0));
-                    if (coarse) pats.push_back(make_uint2(pw[0], pw[1]));
-                } else if (!fixed && std::memcmp(L.invdiag.data() + 3 * id, S->dcon[l], 3 * sizeof(double)) != 0) {
-                    dcon_ok = false;
-                }
-                id++;
-            }
-    LS.surf.n = (int64_t)surf.size();
-    LS.dcon_interior = dcon_ok;
-    LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
-    std::vector<int32_t> rbs;
-    rbs.reserve(9 * surf.size() + 9);
-    for (int r = 0; r < 9; r++) rbs.insert(rbs.end(), rb[r].begin(), rb[r].end());
-    if (surf.empty()) {
-        surf.push_back(make_int4(0, 0, 0, 0));
-        rbs.assign(9, 0);
+        if (int rc = upload_tiles(S, l, SF_X, ST_Y, LS.zc, LS.tiles_sf)) return rc;
+    // surface list on the device: counts per node segment word, their scan, the entries
+    const int64_t nseg = (int64_t)L.ny * L.nz * L.S;
+    const int grid = (int)((nseg + kWaves - 1) / kWaves);
+    const DevLevel &DL = S->dl[l];
+    int32_t *cnt = nullptr, *off = nullptr;
+    int *dbad = nullptr;
+    HIP_TRY(hipMalloc(&cnt, (nseg + 1) * sizeof(int32_t)));
+    struct Free {
+        void *p;
+        ~Free() { if (p) (void)hipFree(p); }
+    } f1{cnt};
+    HIP_TRY(hipMalloc(&off, (nseg + 1) * sizeof(int32_t)));
+    Free f2{off};
+    HIP_TRY(hipMalloc(&dbad, sizeof(int)));
+    Free f3{dbad};
+    HIP_TRY(hipMemsetAsync(cnt, 0, (nseg + 1) * sizeof(int32_t), S->stream));
+    HIP_TRY(hipMemsetAsync(dbad, 0, sizeof(int), S->stream));
+    const double *dc = S->dcon[l];
+    hipLaunchKernelGGL(k_surface<false>, dim3(grid), dim3(kBlock), 0, S->stream, DL, coarse ? 1 : 0, S->dinv64[l], dc[0],
+                       dc[1], dc[2], cnt, (const int32_t *)nullptr, (int64_t)0, (int4 *)nullptr, (int32_t *)nullptr,
+                       (uint2 *)nullptr, dbad);
+    HIP_TRY(hipGetLastError());
+    int64_t nsurf = 0;
+    std::string err;
+    if (int rc = device_exclusive_scan(cnt, nseg, off, &nsurf, S->stream, err)) {
+        g_last_error = err;
+        return rc;
     }
-    if (pats.empty()) pats.push_back(make_uint2(0, 0));
+    int hbad = 0;
+    HIP_TRY(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    LS.surf.n = nsurf;
+    LS.dcon_interior = hbad == 0;
+    LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
     int4 *ds;
     int32_t *drb;
     uint2 *dp;
-    if (int rc = S->upload(&ds, surf)) return rc;
-    if (int rc = S->upload(&drb, rbs)) return rc;
-    if (int rc = S->upload(&dp, pats)) return rc;
+    if (int rc = S->alloc(&ds, std::max<int64_t>(nsurf, 1))) return rc;
+    if (int rc = S->alloc(&drb, 9 * std::max<int64_t>(nsurf, 1))) return rc;
+    if (int rc = S->alloc(&dp, std::max<int64_t>(nsurf, 1))) return rc;
+    if (nsurf > 0) {
+        hipLaunchKernelGGL(k_surface<true>, dim3(grid), dim3(kBlock), 0, S->stream, DL, coarse ? 1 : 0, S->dinv64[l],
+                           dc[0], dc[1], dc[2], (int32_t *)nullptr, (const int32_t *)off, nsurf, ds, drb, dp,
+                           (int *)nullptr);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(S->stream));
+    }
     LS.surf.e = ds;
     LS.surf.rb = drb;
     LS.surf.pat = dp;
@@ -3569,8 +3647,8 @@ int build_stencil(vfem_system *S)
         if ((C.nx + RT_X - 1) / RT_X > 2047 || (C.ny + RT_Y - 1) / RT_Y > 2047 || (C.nz + rz - 1) / rz > 1023 ||
             (F.nz + pz - 1) / pz > 1023)
             continue;
-        if (int rc = upload_tiles(S, C, RT_X, RT_Y, rz, S->rtiles[l])) return rc;
-        if (int rc = upload_tiles(S, F, 2 * RT_X, 2 * RT_Y, pz, S->ptiles[l])) return rc;
+        if (int rc = upload_tiles(S, l + 1, RT_X, RT_Y, rz, S->rtiles[l])) return rc;
+        if (int rc = upload_tiles(S, l, 2 * RT_X, 2 * RT_Y, pz, S->ptiles[l])) return rc;
         S->short_z[l] = sz;
         S->tiled[l] = true;
         S->tiled_auto[l] = F.nn >= kTiledMinNodes;  // small levels: the row-segment kernels are faster
@@ -3638,63 +3716,36 @@ int alloc_vectors(vfem_system *S, int nb)
     return 0;
 }
 
-int upload_system(vfem_system *S)
+// Device handles of the system: its device, stream and timing events.
+int init_device(vfem_system *S)
 {
-    Hierarchy &H = S->H;
     HIP_TRY(hipGetDevice(&S->device));
     HIP_TRY(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&S->ev0));
     HIP_TRY(hipEventCreate(&S->ev1));
+    return 0;
+}
+
+// Takes over the device-built hierarchy (level masks, D^-1, flags, coarsest
+// inverses) and builds the operator data around it: element tables, the
+// element schedule, the stencil form's tile and surface lists, the vectors.
+int upload_system(vfem_system *S, const DevHierarchy &DH)
+{
+    Hierarchy &H = S->H;
     for (int l = 0; l < H.nlev; l++) {
-        HostLevel &L = H.L[l];
+        const HostLevel &L = H.L[l];
+        const DevLevelArrays &A = DH.L[l];
         DevLevel &D = S->dl[l];
         D.nx = L.nx; D.ny = L.ny; D.nz = L.nz; D.S = L.S;
         D.ex = L.ex; D.ey = L.ey; D.ez = L.ez; D.SE = L.SE;
         D.div = L.div;
         D.nn = L.nn;
-        uint64_t *nb, *fb, *eb;
-        int32_t *base;
-        uint8_t *ep = nullptr;
-        if (int rc = S->upload(&nb, L.nbits)) return rc;
-        if (int rc = S->upload(&fb, L.fbits)) return rc;
-        if (int rc = S->upload(&eb, L.ebits)) return rc;
-        if (int rc = S->upload(&base, L.nbase)) return rc;
-        if (!L.epat.empty())
-            if (int rc = S->upload(&ep, L.epat)) return rc;
-        D.nbits = nb; D.fbits = fb; D.ebits = eb; D.nbase = base; D.epat = ep;
-        {
-            std::vector<uint32_t> fc((L.nn + 31) / 32 + 1, 0);
-            int64_t k = 0;
-            for (size_t q = 0; q < L.nbits.size(); q++)
-                for (int b = 0; b < 64; b++)
-                    if ((L.nbits[q] >> b) & 1) {
-                        if ((L.fbits[q] >> b) & 1) fc[k >> 5] |= 1u << (k & 31);
-                        k++;
-                    }
-            if (int rc = S->upload(&S->fixc[l], fc)) return rc;
-        }
-        if (int rc = S->upload(&S->dinv64[l], L.invdiag)) return rc;
-        if (int rc = S->upload(&S->dinv32[l], convert<float>(L.invdiag))) return rc;
-        {
-            // the common D^-1 triple: majority vote over the nodes (interior nodes
-            // dominate every level), then flag the nodes that hold it bit for bit
-            const double *d = L.invdiag.data();
-            auto same = [&](int64_t a, int64_t b) { return std::memcmp(d + 3 * a, d + 3 * b, 3 * sizeof(double)) == 0; };
-            int64_t cand = 0, votes = 0;
-            for (int64_t i = 0; i < L.nn; i++) {
-                if (votes == 0) {
-                    cand = i;
-                    votes = 1;
-                } else {
-                    votes += same(i, cand) ? 1 : -1;
-                }
-            }
-            std::vector<uint32_t> fl((L.nn + 31) / 32 + 1, 0u);
-            for (int64_t i = 0; i < L.nn; i++)
-                if (same(i, cand)) fl[i >> 5] |= 1u << (i & 31);
-            for (int c = 0; c < 3; c++) S->dcon[l][c] = L.nn > 0 ? d[3 * cand + c] : 0.0;
-            if (int rc = S->upload(&S->dflat[l], fl)) return rc;
-        }
+        D.nbits = A.nbits; D.fbits = A.fbits; D.ebits = A.ebits; D.nbase = A.nbase; D.epat = A.epat;
+        S->fixc[l] = A.fixc;
+        S->dinv64[l] = A.dinv64;
+        S->dinv32[l] = A.dinv32;
+        S->dflat[l] = A.dflat;
+        for (int c = 0; c < 3; c++) S->dcon[l][c] = A.dcon[c];
         if (l >= 1 && l < (int)H.ktab.size() && !H.ktab[l].empty()) {
             if (int rc = S->upload(&S->ktab64[l], H.ktab[l])) return rc;
             if (int rc = S->upload(&S->ktab32[l], convert<float>(H.ktab[l]))) return rc;
@@ -3705,14 +3756,11 @@ int upload_system(vfem_system *S)
     if (int rc = S->upload(&S->ke32, convert<float>(ke))) return rc;
     if (int rc = build_schedule(S)) return rc;
     if (int rc = build_stencil(S)) return rc;
-    if (int rc = S->upload(&S->kinv_ref64, H.kcinv_ref)) return rc;
-    if (int rc = S->upload(&S->kinv_gal64, H.kcinv_gal)) return rc;
-    if (int rc = S->upload(&S->kinv_ref32, convert<float>(H.kcinv_ref))) return rc;
-    if (int rc = S->upload(&S->kinv_gal32, convert<float>(H.kcinv_gal))) return rc;
+    S->kinv_ref64 = DH.kinv_ref;
+    S->kinv_gal64 = DH.kinv_gal;
     if (int rc = S->upload(&S->freedof, H.freedof)) return rc;
     S->n3 = 3 * H.L[0].nn;
     return alloc_vectors(S, 1);
-    return 0;
 }
 
 // ---- launch helpers -------------------------------------------------------
@@ -3749,9 +3797,7 @@ template <> const float *kefine<float>(vfem_system *S) { return S->ke32; }
 template <typename T> const T *ktab(vfem_system *S, int l);
 template <> const double *ktab<double>(vfem_system *S, int l) { return S->ktab64[l]; }
 template <> const float *ktab<float>(vfem_system *S, int l) { return S->ktab32[l]; }
-template <typename T> const T *kinv(vfem_system *S, bool gal);
-template <> const double *kinv<double>(vfem_system *S, bool gal) { return gal ? S->kinv_gal64 : S->kinv_ref64; }
-template <> const float *kinv<float>(vfem_system *S, bool gal) { return gal ? S->kinv_gal32 : S->kinv_ref32; }
+const double *kinv(vfem_system *S, bool gal) { return gal ? S->kinv_gal64 : S->kinv_ref64; }
 
 // level-0 matvec.  FP64: node-centric VALU kernel.  FP32: matrix cores with
 // `passes` BF16 products -- 6 (three-way split, about FP32 accuracy) for the CG
@@ -3979,7 +4025,7 @@ template <typename T> void launch_coarse_solve(vfem_system *S, const T *rc, T *z
     int n = (int)S->H.nfree;
     if (n == 0) return;
     hipLaunchKernelGGL((k_coarse_solve<double, T>), dim3((n + kWaves - 1) / kWaves, S->ncols), dim3(kBlock), 0, S->stream,
-                       kinv<double>(S, gal), n, S->freedof, rc, zc, S->st, (int64_t)3 * S->dl[S->H.nlev - 1].nn);
+                       kinv(S, gal), n, S->freedof, rc, zc, S->st, (int64_t)3 * S->dl[S->H.nlev - 1].nn);
 }
 
 // Reference additive V-cycle (MultigridPreconditioner.hpp:158-193) on the
@@ -4656,14 +4702,21 @@ int vfem_assemble(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, con
     *out = nullptr;
     auto t0 = std::chrono::steady_clock::now();
     vfem_system *S = new vfem_system();
+    int rc = init_device(S);
+    if (rc) {
+        delete S;
+        return rc;
+    }
     std::string err;
-    int rc = build_hierarchy(voxels, nx, ny, nz, ke, fixed_nodes, n_fixed, num_levels, skip_levels, S->H, err);
+    DevHierarchy DH;
+    rc = build_hierarchy_device(voxels, nx, ny, nz, ke, fixed_nodes, n_fixed, num_levels, skip_levels, S->stream, S->H,
+                                DH, S->allocs, err);
     if (rc) {
         g_last_error = err;
         delete S;
         return rc;
     }
-    rc = upload_system(S);
+    rc = upload_system(S, DH);
     if (rc) {
         delete S;
         return rc;
@@ -4703,6 +4756,24 @@ int vfem_inspect_hierarchy(const int32_t *voxels, int32_t nx, int32_t ny, int32_
             }
     }
     if (free_dofs_out) *free_dofs_out = H.nfree;
+    return VFEM_OK;
+}
+
+int vfem_level_data(vfem_system *sys, int32_t level, double *invdiag, uint8_t *fixed_mask, int64_t *free_dofs_out)
+{
+    if (!sys || level < 0 || level >= sys->H.nlev) {
+        g_last_error = "invalid level";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const HostLevel &L = sys->H.L[level];
+    if (invdiag)
+        HIP_TRY(hipMemcpy(invdiag, sys->dinv64[level], sizeof(double) * 3 * L.nn, hipMemcpyDeviceToHost));
+    if (fixed_mask) {
+        std::vector<uint32_t> fc((L.nn + 31) / 32 + 1);
+        HIP_TRY(hipMemcpy(fc.data(), sys->fixc[level], fc.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < L.nn; k++) fixed_mask[k] = (fc[k >> 5] >> (k & 31)) & 1;
+    }
+    if (free_dofs_out) *free_dofs_out = sys->H.nfree;
     return VFEM_OK;
 }
 

@@ -2,8 +2,11 @@
 // (vfem_setup.cpp) and the device solver (vfem_device.hip).
 #pragma once
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace vfem {
@@ -47,6 +50,69 @@ struct Hierarchy {
     // div-2 coarse level; empty at level 0 (uniform Ke) and at a div-4 level
     std::vector<std::vector<double>> ktab;
 };
+
+// Device arrays of one level, built and owned by build_hierarchy_device (the
+// pointers are also listed in its `allocs`): row masks as in HostLevel, the
+// inverse diagonal in FP64 and FP32, and compact-order bit masks of the fixed
+// nodes (fixc) and of the nodes whose D^-1 triple equals dcon bit for bit
+// (dflat); dcon is the D^-1 triple of a node with eight full elements.
+struct DevLevelArrays {
+    uint64_t *nbits = nullptr, *fbits = nullptr, *ebits = nullptr;
+    int32_t *nbase = nullptr;
+    uint8_t *epat = nullptr;
+    double *dinv64 = nullptr;
+    float *dinv32 = nullptr;
+    uint32_t *fixc = nullptr, *dflat = nullptr;
+    double dcon[3] = {0, 0, 0};
+};
+
+struct DevHierarchy {
+    DevLevelArrays L[kMaxLevels];
+    double *kinv_ref = nullptr, *kinv_gal = nullptr;  // inverses of the two coarsest matrices (nfree^2)
+};
+
+// Runs f(i) for every i in [0, count) on up to 16 host threads.  Indices are
+// handed out in ascending order from a shared counter, so callers put the
+// costliest rows first; every entry is computed by one fixed sequence of
+// operations, so the result does not depend on the thread count.
+template <class F> void parallel_for(int64_t count, const F &f)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (count < 32) nt = 1;
+    std::atomic<int64_t> next{0};
+    auto work = [&]() {
+        for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < count;) f(i);
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto &x : th) x.join();
+}
+
+// Pieces shared by the host and the device builders.
+int check_hierarchy_arguments(const int32_t *voxels, int vx, int vy, int vz, const double *ke_in, const int64_t *fixed,
+                              int64_t nfixed, int num_levels, int skip, std::string &err);
+void symmetrise_ke(const double *ke_in, double *ke);   // from the upper triangle (FEM.cpp:200-210)
+// extents, coarsening factor and element scales of coarse level l from level l-1 (FEM.cpp:253-272)
+void level_dims(const HostLevel &F, HostLevel &C, int l, int skip);
+// free DOFs of the coarsest level (H.nfree, H.freedof) and its two matrices on
+// them: the reference Kc (FEM.cpp:473-513) and the Galerkin-consistent one of the
+// damped cycle (cmask: child masks of the coarsest elements, null for one level)
+void coarsest_matrices(Hierarchy &H, const std::vector<uint64_t> *cmask, std::vector<double> &kref,
+                       std::vector<double> &kgal);
+
+// Builds the hierarchy on the device (stream: a hipStream_t) into DH, with the
+// small host-side parts (extents, counts, masks, element tables, free DOFs) in H;
+// H.L[l].invdiag and H.kcinv_* stay empty.  The same results as build_hierarchy,
+// bit for bit.  Device allocations are appended to allocs.
+int build_hierarchy_device(const int32_t *voxels, int nx, int ny, int nz, const double *ke, const int64_t *fixed,
+                           int64_t nfixed, int num_levels, int skip, void *stream, Hierarchy &H, DevHierarchy &DH,
+                           std::vector<void *> &allocs, std::string &err);
+
+// Exclusive scan of n int32 counts on the device (stream: a hipStream_t) into
+// base[0..n], base[n] = *total; fails when the total exceeds the int32 range.
+int device_exclusive_scan(const int32_t *cnt, int64_t n, int32_t *base, int64_t *total, void *stream,
+                          std::string &err);
 
 // Builds the hierarchy on the host.  Returns 0 or a vfem_status code; *err gets a message.
 int build_hierarchy(const int32_t *voxels, int nx, int ny, int nz, const double *ke, const int64_t *fixed,

@@ -99,24 +99,6 @@ std::vector<uint8_t> used_from_elements(const HostLevel &L, const std::vector<ui
     return used;
 }
 
-// Runs f(i) for every i in [0, count) on up to 16 host threads.  Indices are
-// handed out in ascending order from a shared counter, so callers put the
-// costliest rows first; every entry is computed by one fixed sequence of
-// operations, so the result does not depend on the thread count.
-template <class F> void parallel_rows(int64_t count, const F &f)
-{
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (count < 32) nt = 1;
-    std::atomic<int64_t> next{0};
-    auto work = [&]() {
-        for (int64_t i; (i = next.fetch_add(1, std::memory_order_relaxed)) < count;) f(i);
-    };
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt; t++) th.emplace_back(work);
-    work();
-    for (auto &x : th) x.join();
-}
-
 // a[0..m) . b[0..m) with four partial sums
 inline double dot4(const double *a, const double *b, int64_t m)
 {
@@ -157,14 +139,14 @@ bool cholesky(std::vector<double> &A, int64_t n)
             }
         }
         if (ke == n) break;
-        parallel_rows(n - ke, [&](int64_t r) {
+        parallel_for(n - ke, [&](int64_t r) {
             double *Ai = &A[(ke + r) * n];
             for (int64_t j = kb; j < ke; j++) {
                 const double *Aj = &A[j * n];
                 Ai[j] = (Ai[j] - dot4(Ai + kb, Aj + kb, j - kb)) / Aj[j];
             }
         });
-        parallel_rows(n - ke, [&](int64_t r) {
+        parallel_for(n - ke, [&](int64_t r) {
             const int64_t i = n - 1 - r;  // longest rows first
             double *Ai = &A[i * n];
             for (int64_t j = ke; j <= i; j++) Ai[j] -= dot4(Ai + kb, &A[j * n] + kb, ke - kb);
@@ -183,7 +165,7 @@ std::vector<double> spd_inverse(std::vector<double> Lf, int64_t n, bool &ok)
     ok = cholesky(Lf, n);
     if (!ok) return inv;
     std::vector<double> WT((size_t)n * n, 0.0);
-    parallel_rows(n, [&](int64_t j) {
+    parallel_for(n, [&](int64_t j) {
         double *w = &WT[j * n];
         w[j] = 1.0 / Lf[j * n + j];
         for (int64_t i = j + 1; i < n; i++) {
@@ -192,7 +174,7 @@ std::vector<double> spd_inverse(std::vector<double> Lf, int64_t n, bool &ok)
         }
     });
     inv.assign((size_t)n * n, 0.0);
-    parallel_rows(n, [&](int64_t i) {
+    parallel_for(n, [&](int64_t i) {
         const double *wi = &WT[i * n];
         for (int64_t j = i; j < n; j++) {
             const double v = dot4(wi + j, &WT[j * n] + j, n - j);
@@ -305,8 +287,8 @@ void galerkin_element(const double *ke, int div, uint64_t mask, double *K)
     }
 }
 
-int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double *ke_in, const int64_t *fixed,
-                    int64_t nfixed, int num_levels, int skip, Hierarchy &H, std::string &err)
+int check_hierarchy_arguments(const int32_t *voxels, int vx, int vy, int vz, const double *ke_in, const int64_t *fixed,
+                              int64_t nfixed, int num_levels, int skip, std::string &err)
 {
     if (!voxels || !ke_in || (nfixed > 0 && !fixed) || vx <= 0 || vy <= 0 || vz <= 0 || num_levels < 1 ||
         num_levels > kMaxLevels || skip < 0 || skip > 1) {
@@ -317,10 +299,91 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
         err = "grid extents above 1022 voxels are not supported";
         return 1;
     }
+    return 0;
+}
+
+void symmetrise_ke(const double *ke_in, double *ke)
+{
+    for (int i = 0; i < 24; i++)
+        for (int j = 0; j < 24; j++) ke[i * 24 + j] = (i <= j) ? ke_in[i * 24 + j] : ke_in[j * 24 + i];
+}
+
+void level_dims(const HostLevel &F, HostLevel &C, int l, int skip)
+{
+    C.rdx = F.rdx / 2 + F.rdx % 2; C.rdy = F.rdy / 2 + F.rdy % 2; C.rdz = F.rdz / 2 + F.rdz % 2;
+    C.div = (l == 1 && skip > 0) ? (1 << (skip + 1)) : 2;
+    C.ex = (F.ex - 1) / C.div + 1; C.ey = (F.ey - 1) / C.div + 1; C.ez = (F.ez - 1) / C.div + 1;
+    C.nx = C.ex + 1; C.ny = C.ey + 1; C.nz = C.ez + 1;
+    C.fullscale = std::ldexp(1.0, l + skip);
+    C.partscale = std::ldexp(1.0, l - 1);
+}
+
+void coarsest_matrices(Hierarchy &H, const std::vector<uint64_t> *cmask, std::vector<double> &kref,
+                       std::vector<double> &kgal)
+{
+    const int nlev = H.nlev, skip = H.skip;
+    const HostLevel &Cc = H.L[nlev - 1];
+    std::vector<int32_t> freeidx(Cc.nn, -1);
+    {
+        int32_t nf = 0;
+        int64_t k = 0;
+        for (int64_t r = 0; r < (int64_t)Cc.ny * Cc.nz; r++)
+            for (int x = 0; x < Cc.nx; x++) {
+                uint64_t w = Cc.nbits[r * Cc.S + x / 64];
+                if (!((w >> (x % 64)) & 1)) continue;
+                bool fx = (Cc.fbits[r * Cc.S + x / 64] >> (x % 64)) & 1;
+                freeidx[k++] = fx ? -1 : nf++;
+            }
+        H.nfree = 3 * (int64_t)nf;
+    }
+    H.freedof.resize(H.nfree);
+    for (int64_t k = 0; k < Cc.nn; k++)
+        if (freeidx[k] >= 0)
+            for (int c = 0; c < 3; c++) H.freedof[3 * freeidx[k] + c] = (int32_t)(3 * k + c);
+    const int64_t n = H.nfree;
+    kref.assign((size_t)n * n, 0.0);
+    kgal.assign((size_t)n * n, 0.0);
+    if (n == 0) return;
+    const double kscale = std::ldexp(1.0, nlev + skip - 1);
+    const uint64_t fullmask = (Cc.div == 4) ? ~0ull : 0xFFull;
+    std::vector<double> G(576);
+    for (int z = 0; z < Cc.ez; z++)
+        for (int y = 0; y < Cc.ey; y++)
+            for (int x = 0; x < Cc.ex; x++) {
+                if (!((Cc.ebits[((size_t)z * Cc.ey + y) * Cc.SE + (x >> 6)] >> (x & 63)) & 1)) continue;
+                const int64_t e = lin(x, y, z, Cc.ex, Cc.ey);
+                int32_t fn[8];
+                for (int L = 0; L < 8; L++)
+                    fn[L] = freeidx[node_id(Cc, x + (L & 1), y + ((L >> 1) & 1), z + ((L >> 2) & 1))];
+                const bool partial = cmask && (*cmask)[e] != fullmask;
+                if (partial) {
+                    galerkin_element(H.ke, Cc.div, (*cmask)[e], G.data());
+                    for (int q = 0; q < 576; q++) G[q] *= Cc.partscale;
+                } else {
+                    for (int q = 0; q < 576; q++) G[q] = Cc.fullscale * H.ke[q];
+                }
+                for (int a = 0; a < 8; a++) {
+                    if (fn[a] < 0) continue;
+                    for (int b = 0; b < 8; b++) {
+                        if (fn[b] < 0) continue;
+                        for (int i = 0; i < 3; i++)
+                            for (int j = 0; j < 3; j++) {
+                                size_t idx = (size_t)(3 * fn[a] + i) * n + 3 * fn[b] + j;
+                                kref[idx] += kscale * H.ke[(3 * a + i) * 24 + 3 * b + j];
+                                kgal[idx] += G[(3 * a + i) * 24 + 3 * b + j];
+                            }
+                    }
+                }
+            }
+}
+
+int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double *ke_in, const int64_t *fixed,
+                    int64_t nfixed, int num_levels, int skip, Hierarchy &H, std::string &err)
+{
+    if (int rc = check_hierarchy_arguments(voxels, vx, vy, vz, ke_in, fixed, nfixed, num_levels, skip, err)) return rc;
     H = Hierarchy();
     H.skip = skip;
-    for (int i = 0; i < 24; i++)
-        for (int j = 0; j < 24; j++) H.ke[i * 24 + j] = (i <= j) ? ke_in[i * 24 + j] : ke_in[j * 24 + i];
+    symmetrise_ke(ke_in, H.ke);
 
     // ---- level 0 ----------------------------------------------------------
     H.L.emplace_back();
@@ -391,7 +454,7 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
                 if (pres[e] && m != fullmask && partial.emplace(m, std::array<double, 24>{}).second) todo.push_back(m);
             }
             // the map is not modified below: threads look up and fill distinct entries
-            parallel_rows((int64_t)todo.size(), [&](int64_t t) {
+            parallel_for((int64_t)todo.size(), [&](int64_t t) {
                 std::vector<double> K(576);
                 galerkin_element(H.ke, L.div, todo[t], K.data());
                 std::array<double, 24> &d = partial.find(todo[t])->second;
@@ -399,7 +462,7 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
             });
         }
         L.invdiag.assign(3 * L.nn, 0.0);
-        parallel_rows(L.nz, [&](int64_t z) {
+        parallel_for(L.nz, [&](int64_t z) {
             for (int y = 0; y < L.ny; y++)
                 for (int x = 0; x < L.nx; x++) {
                     const int32_t id = nid[lin(x, y, (int)z, L.nx, L.ny)];
@@ -430,13 +493,8 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
         H.L.emplace_back();
         F = &H.L[l - 1];
         HostLevel &C = H.L[l];
-        C.rdx = F->rdx / 2 + F->rdx % 2; C.rdy = F->rdy / 2 + F->rdy % 2; C.rdz = F->rdz / 2 + F->rdz % 2;
-        int div = (l == 1 && skip > 0) ? (1 << (skip + 1)) : 2;
-        C.div = div;
-        C.ex = (F->ex - 1) / div + 1; C.ey = (F->ey - 1) / div + 1; C.ez = (F->ez - 1) / div + 1;
-        C.nx = C.ex + 1; C.ny = C.ey + 1; C.nz = C.ez + 1;
-        C.fullscale = std::ldexp(1.0, l + skip);
-        C.partscale = std::ldexp(1.0, l - 1);
+        level_dims(*F, C, l, skip);
+        const int div = C.div;
         cmask.assign((size_t)C.ex * C.ey * C.ez, 0);
         for (int z = 0; z < F->ez; z++)
             for (int y = 0; y < F->ey; y++)
@@ -513,58 +571,10 @@ int build_hierarchy(const int32_t *voxels, int vx, int vy, int vz, const double 
     H.ktab.resize(nlev);
 
     // ---- coarsest level: free DOFs, Kc (FEM.cpp:473-513) --------------------
-    HostLevel &Cc = H.L[nlev - 1];
-    std::vector<int32_t> freeidx(Cc.nn, -1);
-    {
-        int32_t nf = 0;
-        int64_t k = 0;
-        for (int64_t r = 0; r < (int64_t)Cc.ny * Cc.nz; r++)
-            for (int x = 0; x < Cc.nx; x++) {
-                uint64_t w = Cc.nbits[r * Cc.S + x / 64];
-                if (!((w >> (x % 64)) & 1)) continue;
-                bool fx = (Cc.fbits[r * Cc.S + x / 64] >> (x % 64)) & 1;
-                freeidx[k++] = fx ? -1 : nf++;
-            }
-        H.nfree = 3 * (int64_t)nf;
-    }
-    H.freedof.resize(H.nfree);
-    for (int64_t k = 0; k < Cc.nn; k++)
-        if (freeidx[k] >= 0)
-            for (int c = 0; c < 3; c++) H.freedof[3 * freeidx[k] + c] = (int32_t)(3 * k + c);
-    int64_t n = H.nfree;
+    std::vector<double> kref, kgal;
+    coarsest_matrices(H, nlev > 1 ? &cmask : nullptr, kref, kgal);
+    const int64_t n = H.nfree;
     if (n > 0) {
-        std::vector<double> kref((size_t)n * n, 0.0), kgal((size_t)n * n, 0.0);
-        double kscale = std::ldexp(1.0, nlev + skip - 1);
-        uint64_t fullmask = (Cc.div == 4) ? ~0ull : 0xFFull;
-        std::vector<double> G(576);
-        for (int z = 0; z < Cc.ez; z++)
-            for (int y = 0; y < Cc.ey; y++)
-                for (int x = 0; x < Cc.ex; x++) {
-                    int64_t e = lin(x, y, z, Cc.ex, Cc.ey);
-                    if (!present[e]) continue;
-                    int32_t fn[8];
-                    for (int L = 0; L < 8; L++)
-                        fn[L] = freeidx[ids[lin(x + (L & 1), y + ((L >> 1) & 1), z + ((L >> 2) & 1), Cc.nx, Cc.ny)]];
-                    bool partial = nlev > 1 && cmask[e] != fullmask;
-                    if (partial) {
-                        galerkin_element(H.ke, Cc.div, cmask[e], G.data());
-                        for (int q = 0; q < 576; q++) G[q] *= Cc.partscale;
-                    } else {
-                        for (int q = 0; q < 576; q++) G[q] = Cc.fullscale * H.ke[q];
-                    }
-                    for (int a = 0; a < 8; a++) {
-                        if (fn[a] < 0) continue;
-                        for (int b = 0; b < 8; b++) {
-                            if (fn[b] < 0) continue;
-                            for (int i = 0; i < 3; i++)
-                                for (int j = 0; j < 3; j++) {
-                                    size_t idx = (size_t)(3 * fn[a] + i) * n + 3 * fn[b] + j;
-                                    kref[idx] += kscale * H.ke[(3 * a + i) * 24 + 3 * b + j];
-                                    kgal[idx] += G[(3 * a + i) * 24 + 3 * b + j];
-                                }
-                        }
-                    }
-                }
         bool ok1, ok2;
         H.kcinv_ref = spd_inverse(kref, n, ok1);
         H.kcinv_gal = spd_inverse(kgal, n, ok2);
