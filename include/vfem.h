@@ -149,6 +149,15 @@ int vfem_apply_boundary_conditions(double *f, int64_t n_dofs, const int64_t *loa
  * vfem_num_dofs doubles.  opts may be NULL (defaults); stats may be NULL. */
 int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options *opts, vfem_stats *stats);
 
+/* Repeated solves on a changing geometry (no reference counterpart; the reference
+ * is the solver of a topology-optimisation loop, Readme.md:76-77): sets the
+ * resident solution of `to` (its vfem_solve_resident start vector under
+ * warm_start = 1) from the resident solution of `from`, node by node at the same
+ * grid position: nodes both systems use keep their displacement, nodes only `to`
+ * uses and `to`'s fixed nodes start at zero.  Both systems must be assembled on
+ * the same device from voxel grids of the same extents. */
+int vfem_map_solution(const vfem_system *from, vfem_system *to);
+
 /* Multi-RHS block (no reference counterpart; the reference solves one load case,
  * include/FEMGPU.hpp:11-12): nrhs load cases on one geometry, f and u are
  * nrhs x ndof row-major.  Up to 256 columns run as one block -- every kernel of

@@ -467,3 +467,51 @@ def test_block_autotuner(systems):
     assert all(v[3] > 0 for v in table.values())
     assert table[best][3] == min(v[3] for v in table.values())
     A.set_block_config(0)
+
+
+def test_changed_geometry_warm_start(ke0):
+    """Repeated solves on a changing geometry, as in a topology-optimisation loop:
+    the solution on geometry A, moved node by node onto geometry B (A with a
+    hole), starts B's solve; it reaches B's cold-start solution and the oracle's,
+    in fewer iterations."""
+    vox_a, _ = problems.ellipsoid_union(32, 16, 4.0, 12.0, 5)
+    vox_b = vox_a.copy()
+    vox_b[10:16, 12:20, 12:20] = 0
+    assert vox_b.sum() < vox_a.sum()
+
+    def setup(vox):
+        ids = problems.compact_ids(vox)
+        fixed = np.sort(ids[0][ids[0] >= 0])
+        zs = np.nonzero((ids >= 0).any(axis=(1, 2)))[0]
+        top = ids[zs.max()]
+        f = np.zeros(3 * int(ids.max() + 1))
+        f[3 * top[top >= 0] + 2] = -1.0 / np.count_nonzero(top >= 0)
+        return fixed, f
+
+    fa, f_a = setup(vox_a)
+    fb, f_b = setup(vox_b)
+    A = V.assemble_system_matrix(vox_a, ke0, fa, 3)
+    B = V.assemble_system_matrix(vox_b, ke0, fb, 3)
+    kw = dict(tol=1e-10, precision=V.FP64, cycle=V.CYCLE_DAMPED)
+    A.set_rhs(f_a)
+    sa = A.solve_resident(**kw)
+    B.set_rhs(f_b)
+    s_cold = B.solve_resident(**kw)
+    u_cold = B.get_solution()
+    B.map_solution_from(A)
+    s_warm = B.solve_resident(warm_start=1, **kw)
+    u_warm = B.get_solution()
+    ora = O.OracleSystem(vox_b, ke0, fb, 3)
+    u_ref, _, _ = ora.pcg(f_b, tol=1e-12, precond=O.DAMPED, omega=0.5)
+    ora.close()
+    say("changed geometry: A %d iterations; B cold %d, warm from A %d; warm err %.2e"
+        % (sa["iterations"], s_cold["iterations"], s_warm["iterations"], rel(u_warm, u_ref)))
+    assert s_warm["converged"] == 1 and s_warm["rel_residual"] < 1e-10
+    assert rel(u_warm, u_ref) < 1e-8 and rel(u_cold, u_ref) < 1e-8
+    assert s_warm["iterations"] < s_cold["iterations"]
+    # fixed DOFs of B start (and stay) at zero
+    assert not u_warm[np.repeat(np.isin(np.arange(B.n // 3), fb), 3)].any()
+    with pytest.raises(V.VfemError):
+        B.map_solution_from(V.assemble_system_matrix(vox_a[:, :, :16], ke0, setup(vox_a[:, :, :16])[0], 2))
+    A.close()
+    B.close()

@@ -113,6 +113,7 @@ def lib():
                                       ctypes.POINTER(ctypes.c_int32)]
     L.vfem_precondition.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options)]
     L.vfem_dot.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(ctypes.c_double)]
+    L.vfem_map_solution.argtypes = [vp, vp]
     L.vfem_level_data.argtypes = [vp, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
     L.vfem_dot_noalloc.argtypes = [vp, vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)]
     L.vfem_time_kernel.argtypes = [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
@@ -344,6 +345,12 @@ class MatrixFreeSparse:
         if rc not in (OK, ERR_NOT_CONVERGED):
             _check(rc)
         return st.as_dict()
+
+    def map_solution_from(self, other):
+        """Start vector of the next warm-started resident solve: the resident solution
+        of `other` (same voxel grid extents) moved node by node onto this geometry
+        (vfem_map_solution)."""
+        _check(self._lib.vfem_map_solution(other._h, self._h))
 
     def get_solution(self):
         u = np.zeros(self.n)

@@ -3114,6 +3114,24 @@ __global__ __launch_bounds__(kBlock) void k_tiles_busy(DevLevel L, int bx, int b
     if ((L.nbits[wid] >> lane) & 1) busy[((int64_t)(z / bz) * ty + y / by) * tx + x / bx] = 1;
 }
 
+// Warm start on a changed geometry: xt (nodes of level T) takes the values xf
+// has at the same grid node of level F (same node grid), 0 at nodes F does not
+// use and at T's fixed nodes.  One wavefront per node segment word of T.
+__global__ __launch_bounds__(kBlock) void k_map_solution(DevLevel F, const double *__restrict__ xf, DevLevel T,
+                                                          double *__restrict__ xt)
+{
+    const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (wid >= (int64_t)T.ny * T.nz * T.S) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t wt = T.nbits[wid];
+    if (!((wt >> lane) & 1)) return;
+    const int64_t it = T.nbase[wid] + __popcll(wt & ((1ull << lane) - 1));
+    const uint64_t wf = F.nbits[wid];
+    const bool keep = ((wf >> lane) & 1) && !((T.fbits[wid] >> lane) & 1);
+    const int64_t jf = keep ? F.nbase[wid] + __popcll(wf & ((1ull << lane) - 1)) : 0;
+    for (int c = 0; c < 3; c++) xt[3 * it + c] = keep ? xf[3 * jf + c] : 0.0;
+}
+
 // Scratch of the allocation-free dot product (vfem_dot_noalloc): the block
 // partials of its two-launch reduction (two sums of kNoAllocBlocks blocks) and
 // one CG state for the finalizer.  One copy per device, held until the process
@@ -4821,6 +4839,26 @@ int vfem_set_rhs(vfem_system *sys, const double *f)
         return VFEM_ERR_ARGUMENT;
     }
     HIP_TRY(hipMemcpy(sys->f, f, sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    return VFEM_OK;
+}
+
+int vfem_map_solution(const vfem_system *from, vfem_system *to)
+{
+    if (!from || !to) {
+        g_last_error = "null system";
+        return VFEM_ERR_ARGUMENT;
+    }
+    const DevLevel &F = from->dl[0], &T = to->dl[0];
+    if (F.nx != T.nx || F.ny != T.ny || F.nz != T.nz || from->device != to->device) {
+        g_last_error = "the two systems must share the voxel grid extents and the device";
+        return VFEM_ERR_ARGUMENT;
+    }
+    HIP_TRY(hipStreamSynchronize(from->stream));
+    const int64_t nseg = (int64_t)T.ny * T.nz * T.S;
+    hipLaunchKernelGGL(k_map_solution, dim3((int)((nseg + kWaves - 1) / kWaves)), dim3(kBlock), 0, to->stream, F,
+                       (const double *)from->x, T, to->x);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(to->stream));
     return VFEM_OK;
 }
 
