@@ -12,9 +12,11 @@ def sym_upper(ke):
     return np.triu(ke) + np.triu(ke, 1).T
 
 
-def assemble(occ, ke, scale=1.0):
-    """Global K over the elements occ == 1 (local node L = x + 2y + 4z)."""
-    ids = problems.compact_ids(occ)
+def assemble(occ, ke, scale=1.0, ids=None):
+    """Global K over the elements occ == 1 (local node L = x + 2y + 4z); node
+    numbering `ids` (default: the compact numbering of occ itself)."""
+    if ids is None:
+        ids = problems.compact_ids(occ)
     ez, ey, ex = np.nonzero(np.asarray(occ) == 1)
     nodes = np.stack([ids[ez + ((L >> 2) & 1), ey + ((L >> 1) & 1), ex + (L & 1)] for L in range(8)], axis=1)
     dofs = (3 * nodes[:, :, None] + np.arange(3)[None, None, :]).reshape(-1, 24)
@@ -96,28 +98,51 @@ def coarse_fixed(fine_occ, coarse_occ, fixed_fine, div=2):
     return np.array(sorted(out), np.int64)
 
 
+def full_parents(fine_occ, coarse_occ, div):
+    """Coarse elements all of whose div^3 children are present fine elements."""
+    f = np.asarray(fine_occ) == 1
+    cz, cy, cx = np.asarray(coarse_occ).shape
+    pad = np.zeros((cz * div, cy * div, cx * div), bool)
+    pad[:f.shape[0], :f.shape[1], :f.shape[2]] = f
+    return pad.reshape(cz, div, cy, div, cx, div).all(axis=(1, 3, 5))
+
+
 class Hierarchy:
     """Explicit-matrix hierarchy: occupancies, P_l, fixed sets, the level
-    operators (level 0: K; level l >= 1: elements of level l-1 taken full with
-    2^(l-1) Ke, Galerkin-projected -- the reference's unique-K rule,
-    FEM.cpp:364-395,463) and the reference coarsest matrix Kc (FEM.cpp:513)."""
+    operators and the reference coarsest matrix Kc (FEM.cpp:513).  Level 0: K.
+    Level 1: P^T K P.  Level l >= 2, the reference's element rule
+    (FEM.cpp:364-395,463): a full coarse element is 2^(l+skip) Ke, a partial
+    one the Galerkin sum of its present children taken at 2^(l-1) Ke.  skip = 1
+    makes level 1 a x4 coarsening (FEM.cpp:266-272)."""
 
-    def __init__(self, voxels, ke, fixed, nlev):
+    def __init__(self, voxels, ke, fixed, nlev, skip=0):
         ke = sym_upper(ke)
         self.occ = [np.asarray(voxels, np.int32)]
         self.fixed = [np.asarray(fixed, np.int64)]
         self.P = []
         self.A = [assemble(self.occ[0], ke)]
         for l in range(1, nlev):
-            c = coarsen(self.occ[-1])
-            P = prolongation(self.occ[-1], c)
+            div = 4 if (l == 1 and skip) else 2
+            fine = self.occ[-1]
+            c = coarsen(fine, div)
+            P = prolongation(fine, c, div)
             self.P.append(P)
-            fin = self.A[0] if l == 1 else assemble(self.occ[-1], ke, 2.0 ** (l - 1))
-            self.A.append((P.T @ fin @ P).tocsr())
-            self.fixed.append(coarse_fixed(self.occ[-1], c, self.fixed[-1]))
+            if l == 1:
+                A = P.T @ self.A[0] @ P
+            else:
+                full = full_parents(fine, c, div)
+                kids = np.array(fine == 1)
+                nz, ny, nx = kids.shape
+                par = full[np.arange(nz)[:, None, None] // div, np.arange(ny)[None, :, None] // div,
+                           np.arange(nx)[None, None, :] // div]
+                kids &= ~par
+                A = (P.T @ assemble(kids.astype(np.int32), ke, 2.0 ** (l - 1), problems.compact_ids(fine)) @ P
+                     + assemble(full.astype(np.int32), ke, 2.0 ** (l + skip), problems.compact_ids(c)))
+            self.A.append(A.tocsr())
+            self.fixed.append(coarse_fixed(fine, c, self.fixed[-1], div))
             self.occ.append(c)
         self.nlev = nlev
-        self.kc_full = assemble(self.occ[-1], ke, 2.0 ** (nlev - 1)).toarray()
+        self.kc_full = assemble(self.occ[-1], ke, 2.0 ** (nlev - 1 + skip)).toarray()
         nn = self.kc_full.shape[0] // 3
         self.free = np.nonzero(~fixed_dof_mask(nn, self.fixed[-1]))[0]
 

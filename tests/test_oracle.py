@@ -151,6 +151,31 @@ def test_damped_vcycle_matches_explicit_v11(ke0, damped_cases, omega):
         s.close()
 
 
+def test_damped_skip_level_cycle_matches_brute_force(ke0):
+    """The damped cycle over a x4 (skip) level 1 (FEM.cpp:266-272): the oracle's
+    level operators, coarsest Galerkin matrix and V(1,1) cycle against the
+    brute-force hierarchy with the same element rule."""
+    vox, _ = problems.ellipsoid_union(18, 9, 2.0, 7.0, 41)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    s = O.OracleSystem(vox, ke0, fixed, 3, skip=1)
+    bf = BF.Hierarchy(vox, ke0, fixed, 3, skip=1)
+    assert s.num_levels == 3 and s.rmap(1).shape[1] == 125
+    rng = np.random.default_rng(13)
+    for l in range(1, 3):
+        np.testing.assert_allclose(s.invdiag(l), bf.invdiag(l), rtol=1e-12)
+        x = rng.standard_normal(3 * s.nodes(l))
+        ref = bf.level_operator(l) @ x
+        assert np.abs(s.level_matvec(l, x) - ref).max() <= 1e-12 * np.abs(ref).max(), l
+    kc = bf.kc_galerkin()
+    assert np.abs(s.kc(1) - kc).max() <= 1e-12 * np.abs(kc).max()
+    np.testing.assert_allclose(s.kc(0), bf.kc(), rtol=1e-12, atol=1e-12 * np.abs(bf.kc()).max())
+    r = rng.standard_normal(s.ndof)
+    z, zb = s.vcycle(r, O.DAMPED, 0.5), bf.vcycle_damped(r, 0.5)
+    assert np.abs(z - zb).max() <= 1e-12 * np.abs(zb).max()
+    s.close()
+
+
 def test_damped_vcycle_is_symmetric_positive(ke0, small_case):
     vox, fixed = small_case
     s = O.OracleSystem(vox, ke0, fixed, 3)

@@ -1890,7 +1890,8 @@ __device__ __forceinline__ void coarse_pair(const DevLevel &L, const T *__restri
     for (int a = 0; a < 2; a++) {
         const int ex = xx - 1 + a, ey = yy - 1 + B, ez = zz - 1 + C;
         const bool inside = ex >= 0 && ex < L.ex && ey >= 0 && ey < L.ey && ez >= 0 && ez < L.ez;
-        const int pat = inside ? L.epat[((int64_t)ez * L.ey + ey) * L.ex + ex] : 0;
+        const int64_t e = ((int64_t)ez * L.ey + ey) * L.ex + ex;
+        const int pat = inside ? (L.epat ? (int)L.epat[e] : L.eidx[e]) : 0;
         if (pat == 0) continue;
         const int Ln = (1 - a) + 2 * (1 - B) + 4 * (1 - C);
         const T *K = tab + (int64_t)pat * 576 + 3 * Ln * 24;
@@ -3758,7 +3759,7 @@ int upload_system(vfem_system *S, const DevHierarchy &DH)
         D.ex = L.ex; D.ey = L.ey; D.ez = L.ez; D.SE = L.SE;
         D.div = L.div;
         D.nn = L.nn;
-        D.nbits = A.nbits; D.fbits = A.fbits; D.ebits = A.ebits; D.nbase = A.nbase; D.epat = A.epat;
+        D.nbits = A.nbits; D.fbits = A.fbits; D.ebits = A.ebits; D.nbase = A.nbase; D.epat = A.epat; D.eidx = A.eidx;
         S->fixc[l] = A.fixc;
         S->dinv64[l] = A.dinv64;
         S->dinv32[l] = A.dinv32;
@@ -4127,10 +4128,14 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         else
             hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
                                S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
-    } else {
-        // div-2 levels only (the damped cycle needs skip_levels = 0): one row segment per block
+    } else if (C.div == 2) {
+        // one coarse row segment per block
         const int segs = (int)std::min<int64_t>((int64_t)C.nz * C.ny * C.S, 8192);
         hipLaunchKernelGGL((k_restrict2<TR, T, TN>), dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
+                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
+    } else {
+        // the skip level (div 4, RestrictDouble.comp): the general row-segment gather
+        hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
                            (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
     }
     if (pre) S->pre_level = l + 1;
@@ -4401,13 +4406,6 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
         (P.cycle_passes != 1 && P.cycle_passes != 3 && P.cycle_passes != 6)) {
         g_last_error = "op_passes must be 3 or 6 and cycle_passes 1, 3 or 6";
         return VFEM_ERR_ARGUMENT;
-    }
-    if (o.cycle == VFEM_CYCLE_DAMPED) {
-        for (int l = 1; l < S->H.nlev; l++)
-            if (S->H.L[l].div != 2) {
-                g_last_error = "the damped cycle needs skip_levels = 0";
-                return VFEM_ERR_UNSUPPORTED;
-            }
     }
     return 0;
 }

@@ -46,8 +46,10 @@ struct Hierarchy {
     std::vector<int32_t> freedof;    // coarsest DOF index of each free DOF
     std::vector<double> kcinv_ref;   // inverse of the reference Kc (FEM.cpp:498-513)
     std::vector<double> kcinv_gal;   // inverse of Kc assembled from the level's element matrices
-    // element matrices per child pattern (256 x 576) for the operator of each
-    // div-2 coarse level; empty at level 0 (uniform Ke) and at a div-4 level
+    // element matrices of each coarse level's operator: per child pattern (256 x
+    // 576) on a div-2 level; on a div-4 level row 0 is zero (absent element) and
+    // row r + 1 belongs to the r-th distinct child mask (DevLevelArrays::eidx);
+    // empty at level 0 (uniform Ke)
     std::vector<std::vector<double>> ktab;
 };
 
@@ -60,6 +62,7 @@ struct DevLevelArrays {
     uint64_t *nbits = nullptr, *fbits = nullptr, *ebits = nullptr;
     int32_t *nbase = nullptr;
     uint8_t *epat = nullptr;
+    int32_t *eidx = nullptr;  // div-4 coarse level: row of each element in H.ktab[l] (0 = absent)
     double *dinv64 = nullptr;
     float *dinv32 = nullptr;
     uint32_t *fixc = nullptr, *dflat = nullptr;

@@ -25,6 +25,7 @@ struct DevLevel {
     const uint64_t *fbits; // [nz*ny*S]
     const uint64_t *ebits; // [ez*ey*SE]
     const uint8_t *epat;   // [ez*ey*ex] child pattern (div-2 coarse levels)
+    const int32_t *eidx;   // [ez*ey*ex] element-table row (div-4 coarse level), 0 = absent
 };
 
 // Scalars of the CG recurrence, resident on the device.

@@ -720,6 +720,24 @@ int build_hierarchy_device(const int32_t *voxels, int vx, int vy, int vz, const 
             for (int q = 0; q < 24; q++) dtab[(size_t)full_row * 24 + q] = C.fullscale * H.ke[q * 24 + q];
             if (int rc = dalloc(c, &eidx, ne_all, false, false)) return rc;
             SETUP_TRY(hipMemcpyAsync(eidx, ei.data(), ne_all * sizeof(int32_t), hipMemcpyHostToDevice, c.s));
+            // the level operator: element matrices of the distinct masks (row 0: absent)
+            H.ktab.resize(l + 1);
+            std::vector<double> &T = H.ktab[l];
+            T.assign((masks.size() + 1) * 576, 0.0);
+            parallel_for((int64_t)masks.size(), [&](int64_t t) {
+                double *K = &T[(size_t)(t + 1) * 576];
+                if (masks[t] == ~0ull) {
+                    for (int q = 0; q < 576; q++) K[q] = C.fullscale * H.ke[q];
+                    return;
+                }
+                galerkin_element(H.ke, div, masks[t], K);
+                for (int q = 0; q < 576; q++) K[q] *= C.partscale;
+            });
+            std::vector<int32_t> row1(ne_all);
+            for (int64_t e = 0; e < ne_all; e++) row1[e] = ei[e] + 1;
+            if (int rc = dalloc(c, &D.eidx, ne_all, true, false)) return rc;
+            SETUP_TRY(hipMemcpyAsync(D.eidx, row1.data(), ne_all * sizeof(int32_t), hipMemcpyHostToDevice, c.s));
+            SETUP_TRY(hipStreamSynchronize(c.s));
         }
         if (int rc = build_invdiag(c, C, D, dtab, full_row, eidx)) return rc;
         if (int rc = download_masks(c, C, D)) return rc;
