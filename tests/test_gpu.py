@@ -306,11 +306,19 @@ def test_skip_level_solve(ke0):
     assert st["converged"] == 1 and rel(u, u_ref) < 1e-8 and st["iterations"] == it
     # the damped V(1,1) cycle over the x4 level (RestrictDouble.comp / InterpolateDouble.comp)
     r[np.repeat(ora.fixed(0), 3)] = 0
-    for om in (0.6, 0.5):
-        z = V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=om).solve(r)
-        ez = rel(z, ora.vcycle(r, O.DAMPED, om))
-        say("skip level damped cycle omega %.1f: %.2e" % (om, ez))
-        assert ez < 1e-12
+    # L0_AUTO keeps the row-segment x4 transfers on a level this small; L0_STENCIL
+    # forces the plane-tiled ones (k_restrict4_tile, k_prolong4_tile)
+    try:
+        for kind in (V.L0_AUTO, V.L0_STENCIL):
+            A.set_level0_kernel(kind)
+            for om in (0.6, 0.5):
+                z_ref = ora.vcycle(r, O.DAMPED, om)
+                ez = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=om).solve(r), z_ref)
+                e32 = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=om, precision=V.FP32).solve(r), z_ref)
+                say("skip level damped cycle form %d omega %.1f: fp64 %.2e fp32 %.2e" % (kind, om, ez, e32))
+                assert ez < 1e-12 and e32 < 1e-4
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
     u_d, it_d, _ = ora.pcg(f, tol=1e-10, precond=O.DAMPED, omega=0.5)
     for prec in (V.FP64, V.FP32):
         u, st = A.solve(f, tol=1e-10, cycle=V.CYCLE_DAMPED, precision=prec)

@@ -185,3 +185,26 @@ def test_config5_learned_correction(ell256):
             assert rel(u, u0) < 1e-6
     finally:
         A.attach_unet(None, 0)
+
+
+def test_skip_level_damped_cycle_at_256(ke0):
+    """The 256^3 workload with the reference's skip level (x4 first coarsening,
+    src/FEM.cpp:266-272): the damped cycle with the plane-tiled x4 transfers
+    against the oracle's twin (FP64 to 1e-12), and an FP32 solve to relative
+    residual 1e-8 that matches the plain FP64 solve."""
+    p = problems.workload("ellipsoids256")
+    A = V.assemble_system_matrix(p.voxels, ke0, p.fixed, 6, skip_levels=1)
+    ora = O.OracleSystem(p.voxels, ke0, p.fixed, 6, skip=1)
+    r = np.random.default_rng(21).standard_normal(A.n)
+    r[np.repeat(ora.fixed(0), 3)] = 0
+    z_ref = ora.vcycle(r, O.DAMPED, 0.5)
+    ora.close()
+    ez = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.5).solve(r), z_ref)
+    u32, s32 = A.solve(p.loads[0], tol=1e-8, precision=V.FP32, cycle=V.CYCLE_DAMPED, omega=0.5)
+    u64, s64 = A.solve(p.loads[0], tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.5)
+    say("256^3 skip: cycle err %.2e; fp32 %d iterations %.1f ms, fp64 %d iterations %.1f ms, diff %.2e"
+        % (ez, s32["iterations"], s32["solve_ms"], s64["iterations"], s64["solve_ms"], rel(u32, u64)))
+    assert ez < 1e-12
+    assert s32["converged"] == 1 and s64["converged"] == 1 and s32["rel_residual"] < 1e-8
+    assert rel(u32, u64) < 1e-6
+    A.close()

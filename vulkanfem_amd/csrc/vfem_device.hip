@@ -2573,6 +2573,339 @@ __global__ __launch_bounds__(kBlock) void k_prolong_tile(DevLevel F, DevLevel C,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tiled transfers across the skip level (div 4, RestrictDouble.comp /
+// InterpolateDouble.comp), with weights w(d) = prod_k (1 - |d_k| / 4); the
+// same plane-marching structure and load pipeline as the div-2 kernels.
+//
+// k_restrict4_tile: a block owns 8 x 8 coarse nodes in x-y and ZC coarse
+// planes.  Fine plane f (35 x 35 nodes around the tile) is staged as r - t;
+// four threads share a coarse node, each summing the weighted fine rows
+// dy = q - 3 and q + 1 of its 7 x 7 window (q = thread % 4), and the quad adds
+// its partial sums; the sum enters the two coarse planes Z with |f - 4 Z| < 4
+// with weight w(f - 4 Z).
+//
+// k_prolong4_tile (z_f = z_in + P z_c; z_in may be z_f): a block owns 32 x 32
+// fine nodes in x-y and ZC fine planes; coarse planes (9 x 9 nodes) are staged
+// in turn and each thread interpolates its 2 x 2 fine nodes from the corners of
+// their coarse cell, for the fine planes between the last two staged planes.
+// ---------------------------------------------------------------------------
+constexpr int R4_T = 8;                 // coarse nodes per restrict block edge
+constexpr int R4_F = 4 * R4_T + 3;      // staged fine row (35)
+constexpr int R4_N = R4_F * R4_F;       // staged fine nodes per plane (1225)
+constexpr int R4_SLOTS = (R4_N + kBlock - 1) / kBlock;
+constexpr int P4_X = 9, P4_N = P4_X * P4_X;  // staged coarse nodes of a prolong block
+constexpr int P4_SLOTS = (P4_N + kBlock - 1) / kBlock;
+constexpr int kRestrict4Z = 4, kProlong4Z = 16;
+static_assert(4 * R4_T * R4_T == kBlock, "four threads per coarse column");
+
+// sum over the quad (lanes 4k .. 4k+3) of a wave
+template <typename T> __device__ __forceinline__ T quad_sum(T v)
+{
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    return v;
+}
+
+template <typename Tf, typename Tt, typename Tc, int ZC>
+__global__ __launch_bounds__(kBlock) void k_restrict4_tile(DevLevel F, DevLevel C, const Tf *__restrict__ rf,
+                                                            const Tt *__restrict__ tf, Tc *__restrict__ rc,
+                                                            int maskfixed, const CGState *st, StencilTiles tiles,
+                                                            PreSmooth<Tc> ps)
+{
+    {
+        const int64_t cf = col_offset(3 * F.nn);
+        rf += cf;
+        tf += cf;
+        rc += col_offset(3 * C.nn);
+        if (ps.z) ps.z += col_offset(3 * C.nn);
+        st = col_state(st);
+    }
+    if (st->done) return;
+    __shared__ Tc pl[2][3 * R4_N];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) return;
+    const int X0 = (int)(tl & 2047) * R4_T, Y0 = (int)((tl >> 11) & 2047) * R4_T, Z0 = (int)(tl >> 22) * ZC;
+    const int Z1 = min(Z0 + ZC, C.nz);
+    const int t = threadIdx.x, q = t & 3, cn = t >> 2, lx = cn % R4_T, ly = cn / R4_T;
+    const int fbeg = max(4 * Z0 - 3, 0), fend = min(4 * (Z1 - 1) + 3, F.nz - 1);
+    // staged fine slots: segment offset within a plane, bit, inside the grid
+    int so[R4_SLOTS], sb[R4_SLOTS];
+    bool sin[R4_SLOTS];
+#pragma unroll
+    for (int k = 0; k < R4_SLOTS; k++) {
+        const int s = t + k * kBlock;
+        const int ss = s < R4_N ? s : 0;
+        const int gx = 4 * X0 - 3 + ss % R4_F, gy = 4 * Y0 - 3 + ss / R4_F;
+        sin[k] = s < R4_N && (unsigned)gx < (unsigned)F.nx && (unsigned)gy < (unsigned)F.ny;
+        so[k] = sin[k] ? gy * F.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    const int pstride = F.ny * F.S;
+    uint64_t mw[R4_SLOTS];
+    int mbase[R4_SLOTS];
+    bool mn[R4_SLOTS], vu[R4_SLOTS];
+    Tf vr[R4_SLOTS][3];
+    Tt vt[R4_SLOTS][3];
+    auto meta = [&](int f) {
+        const bool zin = (unsigned)f < (unsigned)F.nz && f <= fend;
+#pragma unroll
+        for (int k = 0; k < R4_SLOTS; k++) {
+            const int seg = so[k] + (zin ? f : 0) * pstride;
+            mw[k] = F.nbits[seg];
+            mbase[k] = F.nbase[seg];
+            mn[k] = sin[k] && zin;
+        }
+    };
+    auto values = [&]() {
+#pragma unroll
+        for (int k = 0; k < R4_SLOTS; k++) {
+            const bool used = mn[k] && ((mw[k] >> sb[k]) & 1);
+            const int id = used ? mbase[k] + __popcll(mw[k] & ((1ull << sb[k]) - 1)) : 0;
+            const int64_t qq = 3 * (int64_t)id;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                vr[k][c] = rf[qq + c];
+                vt[k][c] = tf[qq + c];
+            }
+            vu[k] = used;
+        }
+    };
+    auto out = [&](int Z, const Tc (&a)[3]) {
+        if (Z < Z0 || Z >= Z1 || q != 0) return;
+        bool used, fixed;
+        const int id = node_info(C, X0 + lx, Y0 + ly, Z, used, fixed);
+        if (!used) return;
+        const bool zero = maskfixed && fixed;
+        const Tc o0 = zero ? Tc(0) : a[0], o1 = zero ? Tc(0) : a[1], o2 = zero ? Tc(0) : a[2];
+        rc[3 * (int64_t)id + 0] = o0;
+        rc[3 * (int64_t)id + 1] = o1;
+        rc[3 * (int64_t)id + 2] = o2;
+        if (ps.z) pre_smooth_out(ps, id, o0, o1, o2);
+    };
+    meta(fbeg);
+    values();
+    meta(fbeg + 1);
+    Tc lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};  // coarse planes floor(f / 4) and floor(f / 4) + 1
+    int zl = fbeg / 4;
+    // the thread's fine rows of the 7 x 7 window: dy = q - 3, and q + 1 for q < 3
+    const int r0 = 4 * ly + q, r1 = 4 * ly + q + 4;
+    const int cx = 4 * lx + 3;
+    const Tc wy0 = Tc(1 + q) * Tc(0.25), wy1 = Tc(3 - q) * Tc(0.25);
+    for (int f = fbeg; f <= fend; f++) {
+        Tc *buf = pl[f & 1];
+#pragma unroll
+        for (int k = 0; k < R4_SLOTS; k++) {
+            const int s = t + k * kBlock;
+            if (s < R4_N) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) buf[3 * s + c] = vu[k] ? Tc(vr[k][c]) - Tc(vt[k][c]) : Tc(0);
+            }
+        }
+        if (f < fend) {
+            values();
+            meta(f + 2);
+        }
+        lds_barrier();
+        Tc p[3] = {0, 0, 0};
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (h == 1 && q == 3) break;
+            const int row = h ? r1 : r0;
+            const Tc wy = h ? wy1 : wy0;
+            Tc a[3] = {0, 0, 0};
+#pragma unroll
+            for (int dx = -3; dx <= 3; dx++) {
+                const Tc w = Tc(4 - (dx < 0 ? -dx : dx)) * Tc(0.25);
+                const int o = 3 * (row * R4_F + cx + dx);
+#pragma unroll
+                for (int c = 0; c < 3; c++) a[c] = fma(w, buf[o + c], a[c]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; c++) p[c] = fma(wy, a[c], p[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) p[c] = quad_sum(p[c]);
+        const int zf = f / 4, rz = f - 4 * zf;
+        if (zf != zl) {  // f = 4 zf: plane zl = zf - 1 has had all its fine planes
+            out(zl, lo);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                lo[c] = hi[c];
+                hi[c] = Tc(0);
+            }
+            zl = zf;
+        }
+        const Tc wl = Tc(4 - rz) * Tc(0.25), wh = Tc(rz) * Tc(0.25);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            lo[c] = fma(wl, p[c], lo[c]);
+            hi[c] = fma(wh, p[c], hi[c]);
+        }
+    }
+    out(zl, lo);
+    if (zl + 1 < C.nz) out(zl + 1, hi);
+}
+
+template <typename Tc, typename Tz, int ZC>
+__global__ __launch_bounds__(kBlock) void k_prolong4_tile(DevLevel F, DevLevel C, const Tc *__restrict__ zc,
+                                                           const Tz *zin, Tz *zf, int maskfixed, const CGState *st,
+                                                           StencilTiles tiles)
+{
+    {
+        zc += col_offset(3 * C.nn);
+        zin += col_offset(3 * F.nn);
+        zf += col_offset(3 * F.nn);
+        st = col_state(st);
+    }
+    if (st->done) return;
+    __shared__ Tz pl[2][3 * P4_N];
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) return;
+    const int fx0 = (int)(tl & 2047) * 32, fy0 = (int)((tl >> 11) & 2047) * 32, fz0 = (int)(tl >> 22) * ZC;
+    const int fz1 = min(fz0 + ZC, F.nz);
+    const int X0 = fx0 / 4, Y0 = fy0 / 4, c0 = fz0 / 4, c1 = (fz1 - 1 + 3) / 4;
+    const int t = threadIdx.x, lx = t % 16, ly = t / 16;
+    int so[P4_SLOTS], sb[P4_SLOTS];
+    bool sin[P4_SLOTS];
+#pragma unroll
+    for (int k = 0; k < P4_SLOTS; k++) {
+        const int s = t + k * kBlock;
+        const int ss = s < P4_N ? s : 0;
+        const int gx = X0 + ss % P4_X, gy = Y0 + ss / P4_X;
+        sin[k] = s < P4_N && (unsigned)gx < (unsigned)C.nx && (unsigned)gy < (unsigned)C.ny;
+        so[k] = sin[k] ? gy * C.S + (gx >> 6) : 0;
+        sb[k] = sin[k] ? gx & 63 : 0;
+    }
+    const int pstride = C.ny * C.S;
+    uint64_t mw[P4_SLOTS];
+    int mbase[P4_SLOTS];
+    bool mn[P4_SLOTS], vu[P4_SLOTS];
+    Tc vz[P4_SLOTS][3];
+    auto meta = [&](int c) {
+        const bool zin_ = (unsigned)c < (unsigned)C.nz && c <= c1;
+#pragma unroll
+        for (int k = 0; k < P4_SLOTS; k++) {
+            const int seg = so[k] + (zin_ ? c : 0) * pstride;
+            mw[k] = C.nbits[seg];
+            mbase[k] = C.nbase[seg];
+            mn[k] = sin[k] && zin_;
+        }
+    };
+    auto values = [&]() {
+#pragma unroll
+        for (int k = 0; k < P4_SLOTS; k++) {
+            const bool used = mn[k] && ((mw[k] >> sb[k]) & 1);
+            const int id = used ? mbase[k] + __popcll(mw[k] & ((1ull << sb[k]) - 1)) : 0;
+#pragma unroll
+            for (int m = 0; m < 3; m++) vz[k][m] = zc[3 * (int64_t)id + m];
+            vu[k] = used;
+        }
+    };
+    // the thread's fine nodes (fx0 + 2 lx + i, fy0 + 2 ly + j), i, j in {0, 1}: coarse
+    // cell (lx / 2, ly / 2) of the staged plane, at offsets 2 (lx % 2) + i, 2 (ly % 2) + j
+    const int fx = fx0 + 2 * lx, fb = fx & 63;
+    const bool fxin = fx < F.nx, fx1in = fx + 1 < F.nx;
+    const int cxl = lx / 2, cyl = ly / 2, ox = 2 * (lx & 1), oy = 2 * (ly & 1);
+    auto emit = [&](int fz, const Tz (&o)[2][2][3]) {
+        int ids[2][2];
+        bool us[2][2], zs[2][2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int fy = fy0 + 2 * ly + j;
+            const bool in = fxin && fy < F.ny && fz < F.nz;
+            const int seg = in ? (fz * F.ny + fy) * F.S + (fx >> 6) : 0;
+            const uint64_t w = F.nbits[seg], fw = F.fbits[seg];
+            const int b0 = F.nbase[seg] + __popcll(w & ((1ull << fb) - 1));
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const bool u = in && (i == 0 || fx1in) && ((w >> (fb + i)) & 1);
+                us[j][i] = u;
+                zs[j][i] = maskfixed && ((fw >> (fb + i)) & 1);
+                ids[j][i] = u ? b0 + (i ? (int)((w >> fb) & 1) : 0) : 0;
+            }
+        }
+        Tz a[2][2][3];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int m = 0; m < 3; m++) a[j][i][m] = zin[3 * (int64_t)ids[j][i] + m];
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                if (!us[j][i]) continue;
+                Tz *qz = zf + 3 * (int64_t)ids[j][i];
+#pragma unroll
+                for (int m = 0; m < 3; m++) qz[m] = zs[j][i] ? Tz(0) : a[j][i][m] + o[j][i][m];
+            }
+    };
+    meta(c0);
+    values();
+    meta(c0 + 1);
+    Tz prev[2][2][3] = {};
+    for (int c = c0; c <= c1; c++) {
+        Tz *buf = pl[c & 1];
+#pragma unroll
+        for (int k = 0; k < P4_SLOTS; k++) {
+            const int s = t + k * kBlock;
+            if (s < P4_N) {
+#pragma unroll
+                for (int m = 0; m < 3; m++) buf[3 * s + m] = vu[k] ? Tz(vz[k][m]) : Tz(0);
+            }
+        }
+        if (c < c1) {
+            values();
+            meta(c + 2);
+        }
+        lds_barrier();
+        // bilinear interpolation of the staged plane at the thread's 2 x 2 fine columns
+        Tz cur[2][2][3];
+        {
+            const int a = 3 * (cyl * P4_X + cxl);
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const Tz v00 = buf[a + m], v10 = buf[a + 3 + m], v01 = buf[a + 3 * P4_X + m],
+                         v11 = buf[a + 3 * P4_X + 3 + m];
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+                        const Tz wx = Tz(ox + i) * Tz(0.25), wy = Tz(oy + j) * Tz(0.25);
+                        cur[j][i][m] = (Tz(1) - wy) * ((Tz(1) - wx) * v00 + wx * v10) + wy * ((Tz(1) - wx) * v01 + wx * v11);
+                    }
+            }
+        }
+        if (c > c0) {
+            // fine planes 4 (c - 1) .. 4 c - 1 between coarse planes c - 1 and c
+#pragma unroll 1
+            for (int rz = 0; rz < 4; rz++) {
+                const int f = 4 * (c - 1) + rz;
+                if (f < fz0 || f >= fz1) continue;
+                const Tz w1 = Tz(rz) * Tz(0.25), w0 = Tz(1) - w1;
+                Tz o[2][2][3];
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int m = 0; m < 3; m++) o[j][i][m] = w0 * prev[j][i][m] + w1 * cur[j][i][m];
+                emit(f, o);
+            }
+        }
+        // the last fine plane 4 c (when it is in range and c = c1): coarse plane c alone
+        if (c == c1 && 4 * c >= fz0 && 4 * c < fz1) emit(4 * c, cur);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int m = 0; m < 3; m++) prev[j][i][m] = cur[j][i][m];
+    }
+}
+
 // Damped Jacobi on one level, flat over the level's DOFs (fully coalesced).
 //   SMOOTH_INIT: z = omega D^-1 r          SMOOTH_POST: z += omega D^-1 (r - t)
 // fixed rows zeroed (compact-order fixed bitmask); fin >= 0 reduces r.z (and
@@ -3660,6 +3993,17 @@ int build_stencil(vfem_system *S)
     for (int l = 0; l + 1 < H.nlev; l++) {
         const HostLevel &F = H.L[l], &C = H.L[l + 1];
         S->tiled[l] = false;
+        if (C.div == 4) {
+            if ((F.nx + 31) / 32 > 2047 || (F.ny + 31) / 32 > 2047 ||
+                (C.nz + kRestrict4Z - 1) / kRestrict4Z > 1023 || (F.nz + kProlong4Z - 1) / kProlong4Z > 1023)
+                continue;
+            if (int rc = upload_tiles(S, l + 1, R4_T, R4_T, kRestrict4Z, S->rtiles[l])) return rc;
+            if (int rc = upload_tiles(S, l, 32, 32, kProlong4Z, S->ptiles[l])) return rc;
+            S->short_z[l] = false;
+            S->tiled[l] = true;
+            S->tiled_auto[l] = F.nn >= kTiledMinNodes;
+            continue;
+        }
         if (C.div != 2) continue;
         const bool sz = F.nn < kShortZNodes;
         const int rz = sz ? kRestrictZShort : kRestrictZ, pz = sz ? kProlongZShort : kProlongZ;
@@ -4121,7 +4465,10 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     // not the learned correction), saving the separate smoothing launch.
     const bool pre = l + 2 < nl && !(S->unet_on && l + 1 == S->unet_level);
     const PreSmooth<TN> ps{pre ? level_z<TN>(S, l + 1) : (TN *)nullptr, dinv_ref<TN>(S, l + 1), omega};
-    if (tiled) {
+    if (tiled && C.div == 4) {
+        hipLaunchKernelGGL((k_restrict4_tile<TR, T, TN, kRestrict4Z>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
+                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+    } else if (tiled) {
         if (S->short_z[l])
             hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZShort>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock),
                                0, S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
@@ -4149,7 +4496,11 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     bool fuse = false;
     if constexpr (std::is_same<TR, double>::value && std::is_same<T, double>::value)
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_sf(S, S->stl[0]);
-    if (tiled) {
+    if (tiled && C.div == 4) {
+        hipLaunchKernelGGL((k_prolong4_tile<TN, T, kProlong4Z>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
+                           S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
+                           S->ptiles[l]);
+    } else if (tiled) {
         if (S->short_z[l])
             hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZShort>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
                                S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
