@@ -88,6 +88,15 @@ def batched_solves():
     best, table = A.autotune_block(nrhs=p.loads.shape[0], passes=6, reps=3)
     out = {"batched_workload": p.name, "batched_nrhs": int(p.loads.shape[0]),
            "batched_autotuned_tile": list(table[best][:3]), "batched_tile_ms": round(table[best][3], 4)}
+    # the single-RHS figure on the same geometry: column 0 alone, FP32 cycle
+    A.set_rhs(p.loads[0])
+    kw1 = dict(tol=1e-8, precision=V.FP32, cycle=V.CYCLE_DAMPED, omega=OMEGA)
+    A.solve_resident(**kw1)
+    st1 = A.solve_resident(**kw1)
+    single = A.n * st1["iterations"] / (st1["solve_ms"] / 1e3) / 1e9
+    out.update({"single_rhs_128_fp32_iterations": st1["iterations"],
+                "single_rhs_128_fp32_time_s": round(st1["solve_ms"] / 1e3, 5),
+                "single_rhs_128_fp32_gdof_iter_per_s": round(single, 2)})
     A.set_rhs_block(p.loads)
     for name, prec in (("fp64", V.FP64), ("fp32", V.FP32)):
         kw = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=OMEGA)
@@ -100,6 +109,25 @@ def batched_solves():
         out.update({f"batched_{name}_time_s": round(t, 4), f"batched_{name}_max_iterations": max(its),
                     f"batched_{name}_all_converged": all(s["converged"] == 1 for s in st),
                     f"batched_{name}_gdof_iter_per_s": round(A.n * sum(its) / t / 1e9, 2)})
+    out["batched_fp32_vs_single_rhs"] = round(out["batched_fp32_gdof_iter_per_s"] / single, 2)
+    A.close()
+    return out
+
+
+def skip_level_solves(p, ke):
+    """The reference's canonical plan shape (-n levels --skip 1, .vscode/launch.json:12):
+    the same workload with the x4 first coarsening, damped cycle, FP32 and FP64."""
+    A = V.assemble_system_matrix(p.voxels, ke, p.fixed, problems.WORKLOAD_LEVELS["ellipsoids256"], skip_levels=1)
+    A.set_rhs(p.loads[0])
+    out = {}
+    for name, prec in (("fp32", V.FP32), ("fp64", V.FP64)):
+        kw = dict(tol=1e-8, precision=prec, cycle=V.CYCLE_DAMPED, omega=OMEGA)
+        A.solve_resident(**kw)
+        st = A.solve_resident(**kw)
+        log(f"skip level {name}", st)
+        out.update({f"skip1_{name}_time_s": round(st["solve_ms"] / 1e3, 5),
+                    f"skip1_{name}_iterations": st["iterations"]})
+    A.close()
     return out
 
 
@@ -206,6 +234,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-correction solve")
     ap.add_argument("--no-batched", action="store_true", help="skip the 16-RHS batched128x16 block solve")
+    ap.add_argument("--no-skip", action="store_true", help="skip the solves with the x4 skip level")
     ap.add_argument("--profile-kernels", type=int, default=0,
                     help="only launch the level-0 matvec variants and the copy reference this many times each "
                          "(for rocprofv3 counter runs), then exit")
@@ -222,6 +251,9 @@ def main():
     p = problems.workload(args.workload)
     ke = problems.ke0(1.0, 0.3)
     log(f"workload {p.name}: ndof={p.ndof} levels={nlev} info={p.info}")
+    # HIP runtime start-up (context, first allocations) is not part of the setup time
+    q = problems.cantilever_box(8, 4, 4)
+    V.assemble_system_matrix(q.voxels, ke, q.fixed, 2).close()
     t0 = time.perf_counter()
     A = V.assemble_system_matrix(p.voxels, ke, p.fixed, nlev)
     t_setup = time.perf_counter() - t0
@@ -321,6 +353,8 @@ def main():
         sols.update({f"learned_{k}": v for k, v in sol_l.items()})
     if not args.no_batched:
         out_extra.update(batched_solves())
+    if not args.no_skip:
+        out_extra.update(skip_level_solves(p, ke))
     kernel = "k_matvec_sf<float,1,12,false>"
     traffic = measured_traffic(kernel, p.name)
     cpu = None

@@ -220,17 +220,24 @@ int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level);
 int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs, int32_t precision);
 
 /* Tile shapes of the multi-RHS kernel: elements per layer tex x tey, node planes
- * per block mz. */
+ * per block mz.  The last index (vfem_block_config_count() - 1) is not a shape
+ * of that kernel but the stencil form of the level-0 operator (tex = 0, tey = 1
+ * column per block row, mz planes per block), the other candidate of a
+ * full-accuracy FP32 batch; vfem_set_block_config with it selects the stencil
+ * form for the FP32 batch, with any other index the matrix-core shape. */
 int32_t vfem_block_config_count(void);
 int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz);
 int vfem_set_block_config(vfem_system *sys, int32_t index);
 
-/* Autotuner of the multi-RHS kernel (no reference counterpart): times every tile
- * shape for nrhs columns and `passes` BF16 products (1, 3 or 6) over `reps`
- * launches on a deterministic pseudo-random block, rejects any shape whose product
- * differs from shape 0's, selects the fastest for this system and returns it in
- * *best; ms_out (vfem_block_config_count() entries, may be null) receives the mean
- * time of each shape (-1 for a rejected shape). */
+/* Autotuner of the multi-RHS level-0 operator (no reference counterpart): times
+ * every tile shape of the matrix-core kernel for nrhs columns and `passes` BF16
+ * products (1, 3 or 6) over `reps` launches on a deterministic pseudo-random
+ * block, rejects any shape whose product differs from shape 0's; at 6 products
+ * (FP32 accuracy) it also times the stencil form (last index), accepted when its
+ * product agrees with shape 0's to 1e-5 of the largest entry.  The fastest is
+ * kept for this system -- at 6 products it is the operator the FP32 batched solve
+ * runs -- and returned in *best; ms_out (vfem_block_config_count() entries, may
+ * be null) receives the mean time of each candidate (-1: rejected or not timed). */
 int vfem_autotune_block(vfem_system *sys, int32_t nrhs, int32_t passes, int32_t reps, double *ms_out,
                         int32_t *best);
 

@@ -3615,6 +3615,10 @@ struct vfem_system {
     bool use_mfma64 = true;     // FP64 level-0 matvec on the matrix cores (exact FP64 products)
     int cycle_passes = 6;       // BF16 products of the level-0 cycle matvec (set while enqueueing a cycle)
     int block_cfg = 0;          // kBlockCfgs entry of the multi-RHS level-0 matvec (set by the autotuner)
+    // the multi-RHS FP32 level-0 operator at full accuracy (6 products): kStencilCfg
+    // (the stencil form, one column per blockIdx.y) or a kBlockCfgs entry of the
+    // matrix-core kernel; vfem_autotune_block keeps the fastest
+    int batch_l0 = -1;
     MfmaTiles tiles0{};         // level-0 tiles of the matrix-core matvec with their element schedule
     bool l0_mfma = false;       // FP64 level-0 matvecs on the matrix cores (set while enqueueing a VFEM_MIXED plan)
     vfem_unet *unet = nullptr;  // learned coarse correction (not owned)
@@ -4307,6 +4311,11 @@ void launch_matvec_fine(vfem_system *S, const T *x, T *y, int fin, int honor_don
                 return;
             }
         } else {
+            if (passes == 6 && S->ncols > 1 && S->batch_l0 >= 0 && S->use_mfma && S->use_block) {
+                // the autotuner found the matrix-core block kernel faster for this batch
+                launch_matvec_block(S, (const float *)x, (float *)y, fin, honor_done, 6, S->batch_l0);
+                return;
+            }
             if (passes == 6) {
                 launch_matvec_stencil<T>(S, 0, x, y, fin, honor_done);
                 return;
@@ -5492,13 +5501,19 @@ int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs
     return VFEM_OK;
 }
 
-int32_t vfem_block_config_count(void) { return kNumBlockCfgs; }
+int32_t vfem_block_config_count(void) { return kNumBlockCfgs + 1; }
 
 int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz)
 {
-    if (index < 0 || index >= kNumBlockCfgs || !tex || !tey || !mz) {
+    if (index < 0 || index > kNumBlockCfgs || !tex || !tey || !mz) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
+    }
+    if (index == kNumBlockCfgs) {  // the stencil form: one column per block row, kStencilZ planes
+        *tex = 0;
+        *tey = 1;
+        *mz = kStencilZ;
+        return VFEM_OK;
     }
     *tex = kBlockCfgs[index].tex;
     *tey = kBlockCfgs[index].tey;
@@ -5541,11 +5556,12 @@ int32_t vfem_level0_kernel(const vfem_system *sys)
 
 int vfem_set_block_config(vfem_system *sys, int32_t index)
 {
-    if (!sys || index < 0 || index >= kNumBlockCfgs) {
+    if (!sys || index < 0 || index > kNumBlockCfgs) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
-    sys->block_cfg = index;
+    if (index == kNumBlockCfgs) sys->batch_l0 = -1;  // the stencil form for the FP32 batch
+    else sys->block_cfg = index;
     for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
     sys->graphs.clear();
     return VFEM_OK;
@@ -5599,8 +5615,41 @@ int vfem_autotune_block(vfem_system *sys, int32_t nrhs, int32_t passes, int32_t 
         g_last_error = "no tile shape reproduced the reference product";
         return VFEM_ERR_DEVICE;
     }
+    // at full FP32 accuracy the stencil form (where level 0 has it) competes: its
+    // product must agree with shape 0's to FP32 rounding
+    if (ms_out) ms_out[kNumBlockCfgs] = -1.0;
+    if (passes == 6 && sys->st_ok) {
+        const int keep = sys->batch_l0;
+        sys->batch_l0 = -1;
+        auto stencil = [&]() { launch_matvec_stencil<float>(sys, 0, sys->p32, sys->ap32, -1, 0); };
+        stencil();
+        HIP_TRY(hipEventRecord(sys->ev0, sys->stream));
+        for (int r = 0; r < reps; r++) stencil();
+        HIP_TRY(hipEventRecord(sys->ev1, sys->stream));
+        HIP_TRY(hipEventSynchronize(sys->ev1));
+        HIP_TRY(hipGetLastError());
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, sys->ev0, sys->ev1));
+        HIP_TRY(hipMemcpy(out.data(), sys->ap32, nall * sizeof(float), hipMemcpyDeviceToHost));
+        double dmax = 0, rmax = 0;
+        for (int64_t i = 0; i < nall; i++) {
+            dmax = std::max(dmax, (double)std::fabs(out[i] - ref[i]));
+            rmax = std::max(rmax, (double)std::fabs(ref[i]));
+        }
+        sys->batch_l0 = keep;
+        const bool same = dmax <= 1e-5 * rmax;
+        const double avg = same ? ms / reps : -1.0;
+        if (ms_out) ms_out[kNumBlockCfgs] = avg;
+        if (same && avg < best_ms) {
+            best_ms = avg;
+            best = kNumBlockCfgs;
+        }
+    }
     if (best_out) *best_out = best;
-    return vfem_set_block_config(sys, best);
+    if (int rc = vfem_set_block_config(sys, best)) return rc;
+    // a full-accuracy tuning also decides the form of the FP32 batch's level-0 operator
+    if (passes == 6 && sys->st_ok) sys->batch_l0 = best == kNumBlockCfgs ? -1 : best;
+    return VFEM_OK;
 }
 
 int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
