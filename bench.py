@@ -265,6 +265,8 @@ def main():
         for which, prec_k, form, name in (
                 (8, V.FP32, V.L0_AUTO, "stencil kernel fp32 inside CG iterations"),
                 (7, V.FP32, V.L0_AUTO, "stencil kernel fp32"), (0, V.FP32, V.L0_AUTO, "operator fp32 (stencil + surface)"),
+                (7, V.FP32, V.L0_STENCIL_PLANES, "plane-staged sum-factorised stencil kernel fp32"),
+                (7, V.FP64, V.L0_STENCIL_PLANES, "plane-staged sum-factorised stencil kernel fp64"),
                 (7, V.FP32, V.L0_STENCIL_DIRECT, "structured (not sum-factorised) stencil kernel fp32"),
                 (7, V.FP64, V.L0_AUTO, "stencil kernel fp64"), (0, V.FP64, V.L0_AUTO, "operator fp64 (stencil + surface)"),
                 (0, V.BF16, V.L0_AUTO, "element form bf16 1-pass (matrix cores)"),
@@ -355,7 +357,7 @@ def main():
         out_extra.update(batched_solves())
     if not args.no_skip:
         out_extra.update(skip_level_solves(p, ke))
-    kernel = "k_matvec_sf<float,1,12,false>"
+    kernel = "k_matvec_rows<float,1,12,false>"
     traffic = measured_traffic(kernel, p.name)
     cpu = None
     if not args.no_cpu:
@@ -386,8 +388,8 @@ def main():
                    "cycle": f"damped V(1,1), omega {OMEGA}", "tol": 1e-8, "true_res_every": 20},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": kernel + " (level-0 sum-factorised stencil matvec of the CG operator and the cycle; "
-                               "its surface pass k_stencil_fix runs after it)",
+                     "kernel": kernel + " (level-0 sum-factorised stencil matvec of the CG operator and the cycle, "
+                               "one wavefront per node row; its surface pass k_stencil_fix runs after it)",
                      "duration_us": round(t_st32_it * 1e6, 2),
                      "timing": "mean of the kernel's launches inside 20 CG iterations of the FP32 plan, each "
                                "between its own HIP events on the library stream",

@@ -59,9 +59,13 @@ enum vfem_level0 {
     VFEM_L0_AUTO = 0,     /* the stencil form where it is available and faster (default) */
     VFEM_L0_ELEMENT = 1,  /* element form: Ke times each element's 24 values, on the matrix cores */
     VFEM_L0_STENCIL = 2,  /* assembled 27-point stencil plus a recomputation of the surface nodes;
-                             sum-factorised planes when the stencil is a tensor-product form */
-    VFEM_L0_STENCIL_DIRECT = 3  /* as VFEM_L0_STENCIL with the structured 27-point kernel (no sum
+                             sum-factorised (one wavefront per node row) when the stencil is a
+                             tensor-product form */
+    VFEM_L0_STENCIL_DIRECT = 3, /* as VFEM_L0_STENCIL with the structured 27-point kernel (no sum
                                    factorisation), for comparison */
+    VFEM_L0_STENCIL_PLANES = 4  /* as VFEM_L0_STENCIL with the sum-factorised kernel that stages
+                                   16 x 18-node planes in LDS (the row kernel's predecessor), for
+                                   comparison */
 };
 
 /* multigrid cycle */
@@ -197,6 +201,8 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
  *     tensor-product form.
  *   VFEM_L0_STENCIL_DIRECT: as VFEM_L0_STENCIL with the structured 27-point
  *     kernel everywhere (no sum factorisation), for comparison.
+ *   VFEM_L0_STENCIL_PLANES: as VFEM_L0_STENCIL with the sum-factorised kernel
+ *     that stages each plane of a 14 x 16-node tile in LDS, for comparison.
  *   VFEM_L0_AUTO (default): the stencil form on level 0 and on coarse levels of
  *     at least 100000 nodes; smaller levels are latency-bound and faster in
  *     element form.
@@ -207,7 +213,8 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
  * kinds fail with VFEM_ERR_UNSUPPORTED when level 0 has no stencil form.
  * vfem_level0_kernel returns the form in effect on level 0 and
  * vfem_level_operator_form on `level` (VFEM_L0_ELEMENT, VFEM_L0_STENCIL for the
- * sum-factorised kernel or VFEM_L0_STENCIL_DIRECT for the structured one), or -1
+ * sum-factorised row kernel, VFEM_L0_STENCIL_PLANES for the plane-staged one or
+ * VFEM_L0_STENCIL_DIRECT for the structured one), or -1
  * for a null system or a level out of range. */
 int vfem_set_level0_kernel(vfem_system *sys, int32_t kind);
 int32_t vfem_level0_kernel(const vfem_system *sys);

@@ -66,21 +66,26 @@ def test_matvec_parity(systems, name):
 @pytest.mark.parametrize("name", list(CASES))
 def test_stencil_form_matches_element_form(systems, name):
     """The assembled-stencil level-0 operator plus its surface correction against
-    the element form and the oracle (FP64 to rounding, FP32 to single precision)."""
+    the element form and the oracle (FP64 to rounding, FP32 to single precision).
+    The row-wave and the plane-staged sum-factorised kernels do the same
+    arithmetic at every node the surface pass keeps: their products are equal bit
+    for bit."""
     A, ora, _, _ = systems[name]
     assert A.level0_kernel == V.L0_STENCIL  # Ke0's stencil has the tensor form: sum-factorised kernel
     x = np.random.default_rng(5).standard_normal(A.n)
     ref = ora.matvec(x)
     out = {}
     try:
-        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT, V.L0_STENCIL_PLANES):
             A.set_level0_kernel(kind)
             assert A.level0_kernel == kind
             out[kind] = (A.matvec(x, V.FP64), A.matvec(x, V.FP32))
     finally:
         A.set_level0_kernel(V.L0_AUTO)
     e64, e32 = out[V.L0_ELEMENT]
-    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+    assert np.array_equal(out[V.L0_STENCIL][0], out[V.L0_STENCIL_PLANES][0])
+    assert np.array_equal(out[V.L0_STENCIL][1], out[V.L0_STENCIL_PLANES][1])
+    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT, V.L0_STENCIL_PLANES):
         s64, s32 = out[kind]
         say("stencil kind %d %s: fp64 %.2e (element %.2e), fp32 %.2e (element %.2e)"
             % (kind, name, rel(s64, ref), rel(e64, ref), rel(s32, ref), rel(e32, ref)))
@@ -103,7 +108,7 @@ def test_large_level_kernels_match_element_form(ke0):
     r[3 * np.repeat(np.asarray(p.fixed), 3) + np.tile([0, 1, 2], len(p.fixed))] = 0
     out = {}
     try:
-        for kind in (V.L0_ELEMENT, V.L0_STENCIL):
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_PLANES):
             A.set_level0_kernel(kind)
             say("large system, operator form", kind)
             out[kind] = (A.matvec(x, V.FP64), A.matvec(x, V.FP32),
@@ -116,6 +121,7 @@ def test_large_level_kernels_match_element_form(ke0):
     assert rel(s64, e64) < 1e-14
     assert rel(s32, s64) < 2e-6
     assert rel(zs, ze) < 1e-12
+    assert np.array_equal(s64, out[V.L0_STENCIL_PLANES][0]) and np.array_equal(s32, out[V.L0_STENCIL_PLANES][1])
 
 
 @pytest.mark.parametrize("name", ["ell24", "ell33"])
@@ -132,7 +138,7 @@ def test_stencil_cycle_matches_element_cycle(systems, name):
     z_d = ora.vcycle(r, O.DAMPED, 0.6)
     out = {}
     try:
-        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+        for kind in (V.L0_ELEMENT, V.L0_STENCIL, V.L0_STENCIL_DIRECT, V.L0_STENCIL_PLANES):
             A.set_level0_kernel(kind)
             if kind != V.L0_ELEMENT:  # every div-2 level with full elements
                 assert all(A.level_operator_form(l) == kind for l in range(A.num_levels - 1))
@@ -142,7 +148,7 @@ def test_stencil_cycle_matches_element_cycle(systems, name):
         A.set_level0_kernel(V.L0_AUTO)
     say("cycle %s: fp64 %.2e fp32 %.2e (element fp32 %.2e)" % (name, rel(out[2][0], z_d), rel(out[2][1], z_d),
                                                                rel(out[1][1], z_d)))
-    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT):
+    for kind in (V.L0_STENCIL, V.L0_STENCIL_DIRECT, V.L0_STENCIL_PLANES):
         assert rel(out[kind][0], z_d) < 1e-12
         assert rel(out[kind][0], out[V.L0_ELEMENT][0]) < 1e-12
         assert rel(out[kind][1], z_d) < 1e-5

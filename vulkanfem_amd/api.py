@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "vfem.h")
 
 FP64, FP32, BF16, MIXED = 0, 1, 2, 3
 CYCLE_REFERENCE, CYCLE_DAMPED = 0, 1
-L0_AUTO, L0_ELEMENT, L0_STENCIL, L0_STENCIL_DIRECT = 0, 1, 2, 3
+L0_AUTO, L0_ELEMENT, L0_STENCIL, L0_STENCIL_DIRECT, L0_STENCIL_PLANES = 0, 1, 2, 3, 4
 
 OK, ERR_ARGUMENT, ERR_EMPTY, ERR_DEVICE, ERR_SINGULAR, ERR_NOT_CONVERGED, ERR_BREAKDOWN, ERR_UNSUPPORTED = range(8)
 

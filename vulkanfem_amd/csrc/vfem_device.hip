@@ -1003,6 +1003,330 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
     if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
+// ---------------------------------------------------------------------------
+// k_matvec_rows: the sum-factorised stencil of k_matvec_sf with one wavefront
+// per node row.  A block's 4 waves own 4 consecutive y rows of one 62-node x
+// range and march through a chunk of ZC planes; lane j holds x = x0 - 1 + j, so
+// lanes 0 and 63 are halo columns that only feed their neighbours.  The row
+// words of the three rows a wave reads on each plane (y-1, y, y+1) are
+// wave-uniform: scalar loads, shifted into one 64-bit lane mask in scalar
+// registers, and the compact id of a lane is the row's base plus v_mbcnt of that
+// mask, so the values of a row are one contiguous run read from the row base
+// with a 32-bit lane offset.  The y pass takes the three rows from registers and
+// the x pass exchanges values across the wave with DPP wave shifts: no LDS and
+// no barrier.  Against k_matvec_sf this drops the staging (LDS writes and reads,
+// the per-lane bit arithmetic of the row words and 64-bit addresses) and the
+// halo share of the lanes falls from 2/16 to 2/64.
+//
+// Values of unused nodes are not zeroed: an unused lane loads the value of a
+// used node of the same row (an in-range id) and the products it feeds reach
+// only outputs at unused nodes (not stored) or at surface nodes, which the
+// surface pass recomputes, because every neighbour of a node whose 8 elements
+// are present is used.  The stored surface value and the reduction term both
+// stay finite, and the surface pass replaces the term by its own.
+// ---------------------------------------------------------------------------
+constexpr int RW_X = 62;                              // output columns per wave (lanes 1..62)
+constexpr int RW_Y = kWaves;                          // rows per block, one per wave
+constexpr uint64_t kRowOutLanes = 0x7FFFFFFFFFFFFFFEull;  // lanes that produce outputs
+constexpr int DPP_WSHR1 = 0x138, DPP_WSHL1 = 0x130;   // wave_shr:1 / wave_shl:1, zero fill
+
+template <int CTRL> __device__ __forceinline__ float dpp_wave(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_wave(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Row table of a level for the row waves: entry 1 + (gz ny + gy) tx + t
+// describes node row (gy, gz) as seen by a wave of tile column t (lane j holds x =
+// 62 t - 1 + j): m = used nodes and f = fixed nodes, one bit per lane; base3 = 3
+// times the compact id of the first used node at or after lane 0, clamped to the
+// last node so that an empty row still reads in range; last = the lane count of
+// the row's last used node (0 for an empty row).  A used lane's values start at
+// base3 + 3 mbcnt(m); every other lane reads those of min(mbcnt(m), last).
+// Entry 0 is an empty row, read for rows outside the grid.
+struct alignas(32) RowLane {
+    uint64_t m, f;
+    int32_t base3, last;
+    int32_t pad[2];
+};
+struct RowTable {
+    const RowLane *e;
+    int tx;  // tile columns
+};
+// read through the constant address space, so wave-uniform entries become scalar loads
+typedef const __attribute__((address_space(4))) RowLane *ConstRowLane;
+
+// one thread per (row, tile column); thread 0 also writes the empty entry 0
+__global__ __launch_bounds__(kBlock) void k_row_table(DevLevel L, int tx, RowLane *__restrict__ e)
+{
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) e[0] = RowLane{0, 0, 0, 0, {0, 0}};
+    if (i >= (int64_t)L.ny * L.nz * tx) return;
+    const int t = (int)(i % tx);
+    const int64_t rs = (i / tx) * L.S;
+    const int g0 = t * RW_X - 1;
+    uint64_t m = 0, f = 0;
+    int base;
+    if (g0 < 0) {
+        m = L.nbits[rs] << 1;
+        f = L.fbits[rs] << 1;
+        base = L.nbase[rs];
+    } else {
+        const int sa = g0 >> 6, off = g0 & 63;
+        const uint64_t w0 = L.nbits[rs + sa];
+        base = L.nbase[rs + sa] + __popcll(w0 & ((1ull << off) - 1));
+        m = w0 >> off;
+        f = L.fbits[rs + sa] >> off;
+        if (off && sa + 1 < L.S) {
+            m |= L.nbits[rs + sa + 1] << (64 - off);
+            f |= L.fbits[rs + sa + 1] << (64 - off);
+        }
+    }
+    const int pc = __popcll(m);
+    e[1 + i] = RowLane{m, f & m, 3 * (int)min((int64_t)base, L.nn - 1), pc > 0 ? pc - 1 : 0, {0, 0}};
+}
+
+__device__ __forceinline__ unsigned lane_count(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// this lane's values of a row (cnt = lane_count(m))
+template <typename T>
+__device__ __forceinline__ void row_values(const T *__restrict__ x, int base3, int last, unsigned cnt, T (&v)[3])
+{
+    const T *q = x + (unsigned)base3 + 3u * min(cnt, (unsigned)last);
+    v[0] = q[0];
+    v[1] = q[1];
+    v[2] = q[2];
+}
+
+// words of rows y-1 (a), y (b), y+1 (c) of one plane
+struct PlaneRows {
+    uint64_t ma, mb, mc, fb;
+    int ba, bb, bc;  // base3
+    int la, lb, lc;  // last
+};
+__device__ __forceinline__ PlaneRows plane_rows(ConstRowLane tab, int ia, int ib, int ic)
+{
+    PlaneRows r;
+    r.ma = tab[ia].m;
+    r.ba = tab[ia].base3;
+    r.la = tab[ia].last;
+    r.mb = tab[ib].m;
+    r.fb = tab[ib].f;
+    r.bb = tab[ib].base3;
+    r.lb = tab[ib].last;
+    r.mc = tab[ic].m;
+    r.bc = tab[ic].base3;
+    r.lc = tab[ic].last;
+    return r;
+}
+
+template <typename T, int PERM, int ZC, bool EPI>
+__global__ __launch_bounds__(kBlock) void k_matvec_rows(DevLevel L, StencilTensor<T> K, StencilTiles tiles, RowTable rt,
+                                                         const T *__restrict__ x, T *__restrict__ y, CGState *st,
+                                                         Reduction red, int fin, int honor_done, SmoothEpi<T> ep)
+{
+    constexpr bool CENTRE = sizeof(T) == 4;
+    {
+        const int64_t co = col_offset(3 * L.nn);
+        x += co;
+        y += co;
+        if constexpr (EPI) {  // D^-1 is one array shared by all columns
+            ep.r += co;
+            if (ep.zold) ep.zold += co;
+            ep.z += co;
+        }
+        st = col_state(st);
+        red = col_red(red);
+    }
+    if (honor_done && st->done) return;
+    const uint32_t tl = tiles.tile[remap_block()];
+    if (tl == kNoTile) {  // padding: no work, but the reduction slot is written
+        if (fin >= 0) block_partials(0.0, red);
+        return;
+    }
+    const ConstRowLane tab = (ConstRowLane)rt.e;
+    const int gy = (int)((tl >> 11) & 2047) * RW_Y + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int z0 = (int)(tl >> 22) * ZC, z1 = min(z0 + ZC, L.nz);
+    // table entries of rows y-1, y, y+1 on plane z: 1 + (z ny + y') tx + t, or 0
+    // outside the grid (the y tests are fixed per wave, the z test per plane)
+    const int stride = L.ny * rt.tx;
+    const int e0 = 1 + gy * rt.tx + (int)(tl & 2047);
+    const bool oka = gy >= 1 && gy - 1 < L.ny, okb = gy < L.ny, okc = gy + 1 < L.ny;
+    auto rows = [&](int z) {
+        const bool okz = (unsigned)z < (unsigned)L.nz;
+        const int e = e0 + z * stride;
+        return plane_rows(tab, okz && oka ? e - rt.tx : 0, okz && okb ? e : 0, okz && okc ? e + rt.tx : 0);
+    };
+    // rows of plane p (words and values), words of plane p + 1
+    PlaneRows cur = rows(z0 - 1);
+    unsigned cb = lane_count(cur.mb);
+    T va[3], vb[3], vc[3];
+    row_values(x, cur.ba, cur.la, lane_count(cur.ma), va);
+    row_values(x, cur.bb, cur.lb, cb, vb);
+    row_values(x, cur.bc, cur.lc, lane_count(cur.mc), vc);
+    PlaneRows nxt = rows(z0);
+    // the output row (b of plane p-1)
+    uint64_t mo = 0, fo = 0;
+    int bo = 0;
+    unsigned co_ = 0;
+    // z direction as in k_matvec_sf: sa holds output p-1 but for U_p, wb holds
+    // W_{p-1}, rp the own values of plane p-1
+    T sa[3] = {0, 0, 0}, wb[3] = {0, 0, 0}, rp[3] = {0, 0, 0};
+    double dacc = 0, dacc2 = 0;
+    double er[3] = {0, 0, 0};
+    T ed[3] = {0, 0, 0}, eo[3] = {0, 0, 0};
+    for (int p = z0 - 1; p <= z1; p++) {
+        // values of plane p + 1 (words read one plane ago), then the words of
+        // plane p + 2; both are in flight while plane p is computed
+        const unsigned cbn = lane_count(nxt.mb);
+        T xa[3], xb[3], xc[3];
+        row_values(x, nxt.ba, nxt.la, lane_count(nxt.ma), xa);
+        row_values(x, nxt.bb, nxt.lb, cbn, xb);
+        row_values(x, nxt.bc, nxt.lc, lane_count(nxt.mc), xc);
+        const PlaneRows nx2 = rows(p + 2 <= z1 ? p + 2 : -1);
+        double nr[3] = {0, 0, 0};
+        T nd[3] = {0, 0, 0}, no[3] = {0, 0, 0};
+        if constexpr (EPI) {  // operands of this plane's outputs, used one plane later
+            const int64_t q3 = (int64_t)(unsigned)cur.bb + 3u * min(cb, (unsigned)cur.lb);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                nr[k] = ep.r[q3 + k];
+                nd[k] = ep.dcon ? ep.dc[k] : ep.dinv[q3 + k];
+                if (ep.zold) no[k] = ep.zold[q3 + k];
+            }
+        }
+        // plane p feeds outputs of row b on planes p-1, p, p+1 (an empty plane p
+        // leaves every output next to it a surface node or unused); when none
+        // of them is used, nothing is stored and the z sums restart at zero
+        if (!(mo | cur.mb | nxt.mb)) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sa[k] = wb[k] = T(0);
+        } else {
+            const T *r0 = vb;
+            T U[3], V[3], W[3];
+            T MM[3], KM[3], MK[3], GG[3], GM[3], MG[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                T my, ky, gyv, myl, myr, kyl, kyr;
+                gyv = vc[k] - va[k];
+                if constexpr (CENTRE) {
+                    const T s = (va[k] - r0[k]) + (vc[k] - r0[k]);  // M_y and -K_y of v - r (centre term 0)
+                    const T sl = dpp_wave<DPP_WSHR1>(s), sr = dpp_wave<DPP_WSHL1>(s);
+                    const T rl = dpp_wave<DPP_WSHR1>(r0[k]), rr = dpp_wave<DPP_WSHL1>(r0[k]);
+                    my = s;
+                    ky = -s;
+                    myl = fma(T(6), rl - r0[k], sl);
+                    myr = fma(T(6), rr - r0[k], sr);
+                    kyl = -sl;
+                    kyr = -sr;
+                } else {
+                    const T s = va[k] + vc[k];
+                    my = fma(T(4), r0[k], s);
+                    ky = fma(T(2), r0[k], -s);
+                    myl = dpp_wave<DPP_WSHR1>(my), myr = dpp_wave<DPP_WSHL1>(my);
+                    kyl = dpp_wave<DPP_WSHR1>(ky), kyr = dpp_wave<DPP_WSHL1>(ky);
+                }
+                const T gyl = dpp_wave<DPP_WSHR1>(gyv), gyr = dpp_wave<DPP_WSHL1>(gyv);
+                const T ms = myl + myr;
+                MM[k] = fma(T(4), my, ms);
+                KM[k] = fma(T(2), my, -ms);
+                MK[k] = fma(T(4), ky, kyl + kyr);
+                GG[k] = gyr - gyl;
+                GM[k] = myr - myl;
+                MG[k] = fma(T(4), gyv, gyl + gyr);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                T em = fma(K.k[k][0], KM[k], K.k[k][1] * MK[k]);  // z weight M
+                const T dk = K.k[k][2] * MM[k];                      // z weight K
+                T cg = T(0);                                         // z weight G
+#pragma unroll
+                for (int k2 = 0; k2 < 3; k2++) {
+                    if (k2 == k) continue;
+                    const int pr = sf_pair(st_axis(PERM, k), st_axis(PERM, k2));
+                    const T q = pr == 0 ? GG[k2] : (pr == 1 ? GM[k2] : MG[k2]);
+                    if (pr == 0) em = fma(K.g[k][k2], q, em);
+                    else cg = fma(K.g[k][k2], q, cg);
+                }
+                const T f = em - dk;
+                U[k] = f + cg;
+                V[k] = fma(T(4), em, T(2) * dk);
+                W[k] = f - cg;
+            }
+            T d[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) d[k] = vb[k] - rp[k];
+            const uint64_t sm = mo & kRowOutLanes;  // outputs of plane p-1
+            if (p > z0 && sm) {
+                T out[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    out[k] = sa[k] + U[k];
+                    if constexpr (CENTRE) out[k] = fma(K.t[k], d[k], out[k]);
+                }
+                if (__builtin_amdgcn_inverse_ballot_w64(sm)) {
+                    const bool fx = __builtin_amdgcn_inverse_ballot_w64(fo);
+                    const int64_t q3 = (int64_t)(unsigned)bo + 3u * co_;
+                    T rv[3];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) rv[k] = fx ? T(0) : out[k];
+                    if constexpr (EPI) {
+                        const T om = T(ep.omega);
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const T v = fx ? T(0) : rp[k] + om * ed[k] * (T(er[k]) - rv[k]);
+                            ep.z[q3 + k] = v;
+                            dacc += er[k] * (double)v;
+                            dacc2 += er[k] * (double)eo[k];
+                        }
+                    } else {
+                        T *q = y + q3;
+                        q[0] = rv[0];
+                        q[1] = rv[1];
+                        q[2] = rv[2];
+                        if (fin >= 0) dacc += (double)fma(rp[2], rv[2], fma(rp[1], rv[1], rp[0] * rv[0]));
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sa[k] = wb[k] + V[k];
+                if constexpr (CENTRE) sa[k] = fma(-K.t[k], d[k], sa[k]);
+                wb[k] = W[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            rp[k] = vb[k];
+            va[k] = xa[k];
+            vb[k] = xb[k];
+            vc[k] = xc[k];
+            if constexpr (EPI) {
+                er[k] = nr[k];
+                ed[k] = nd[k];
+                eo[k] = no[k];
+            }
+        }
+        mo = cur.mb;
+        fo = cur.fb;
+        bo = cur.bb;
+        co_ = cb;
+        cur = nxt;
+        cb = cbn;
+        nxt = nx2;
+    }
+    if (fin >= 0) block_partials2(dacc, dacc2, red);
+}
+
 // Surface nodes of the stencil matvec (used, not fixed, some element absent, or
 // on a coarse level some element partial): y_i is recomputed in element form over
 // the present elements around the node (TAB: each with the Galerkin matrix of its
@@ -3565,6 +3889,8 @@ struct LevelStencil {
     StencilTensor<float> t32{};
     StencilTiles tiles{};       // blocks of k_matvec_stencil (16 columns)
     StencilTiles tiles_sf{};    // blocks of k_matvec_sf (14 columns)
+    StencilTiles tiles_rw{};    // blocks of k_matvec_rows (62 columns x 4 rows)
+    RowTable rows{};            // row table of k_matvec_rows
     SurfaceList surf{};         // nodes recomputed by k_stencil_fix
     int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
     bool dcon_interior = false; // every used, non-fixed node outside the surface list has D^-1 = dcon
@@ -3933,6 +4259,20 @@ int build_level_stencil(vfem_system *S, int l)
     if (int rc = upload_tiles(S, l, ST_X, ST_Y, LS.zc, LS.tiles)) return rc;
     if (LS.sf)
         if (int rc = upload_tiles(S, l, SF_X, ST_Y, LS.zc, LS.tiles_sf)) return rc;
+    // row-wave kernel: tile indices in 11 + 11 + 10 bits, 3 ids and the row
+    // table's entry indices in 31 bits
+    const int rtx = (L.nx + RW_X - 1) / RW_X;
+    if (LS.sf && rtx <= 2047 && (L.ny + RW_Y - 1) / RW_Y <= 2047 && 3 * L.nn < INT32_MAX &&
+        (int64_t)L.ny * L.nz * rtx + 1 < INT32_MAX) {
+        const int64_t ne = (int64_t)L.ny * L.nz * rtx;
+        RowLane *e = nullptr;
+        if (int rc = S->alloc(&e, ne + 1, false)) return rc;
+        hipLaunchKernelGGL(k_row_table, dim3((int)((ne + kBlock - 1) / kBlock)), dim3(kBlock), 0, S->stream, S->dl[l], rtx,
+                           e);
+        HIP_TRY(hipGetLastError());
+        LS.rows = {e, rtx};
+        if (int rc = upload_tiles(S, l, RW_X, RW_Y, LS.zc, LS.tiles_rw)) return rc;
+    }
     // surface list on the device: counts per node segment word, their scan, the entries
     const int64_t nseg = (int64_t)L.ny * L.nz * L.S;
     const int grid = (int)((nseg + kWaves - 1) / kWaves);
@@ -4074,7 +4414,8 @@ int alloc_vectors(vfem_system *S, int nb)
                 cap = std::max<int64_t>(cap, (int64_t)G.tx * G.ty * G.tz);
             }
         if (S->st_ok)
-            cap = std::max<int64_t>(cap, (int64_t)std::max(S->stl[0].tiles.n, S->stl[0].tiles_sf.n) + S->stl[0].fix_grid);
+            cap = std::max<int64_t>(cap, (int64_t)std::max({S->stl[0].tiles.n, S->stl[0].tiles_sf.n, S->stl[0].tiles_rw.n}) +
+                                             S->stl[0].fix_grid);
         S->red.cap = (int)cap;
     }
     if (int rc = S->alloc(&S->red.partials, 2 * (size_t)S->red.cap * nb, true, V)) return rc;
@@ -4208,15 +4549,34 @@ template <> const StencilTensor<float> &st_tensor<float>(const LevelStencil &LS)
 // the sum-factorised kernel serves a level with the tensor form unless the
 // structured-stencil kernel is asked for
 bool use_sf(const vfem_system *S, const LevelStencil &LS) { return LS.sf && S->l0_kind != VFEM_L0_STENCIL_DIRECT; }
-const StencilTiles &stencil_tiles(const vfem_system *S, const LevelStencil &LS)
+// FP32 runs the row-wave form of the sum-factorised kernel unless the
+// plane-staged one is asked for.  FP64 keeps the plane-staged kernel, which is
+// faster there (256^3: 174 us against 187 us per launch): a shift of a double
+// is two DPP moves that cannot fold into the FP64 arithmetic.
+bool use_rows(const vfem_system *S, const LevelStencil &LS, bool fp64)
 {
-    return use_sf(S, LS) ? LS.tiles_sf : LS.tiles;
+    return !fp64 && use_sf(S, LS) && S->l0_kind != VFEM_L0_STENCIL_PLANES && LS.tiles_rw.n > 0;
+}
+const StencilTiles &stencil_tiles(const vfem_system *S, const LevelStencil &LS, bool fp64)
+{
+    return use_rows(S, LS, fp64) ? LS.tiles_rw : use_sf(S, LS) ? LS.tiles_sf : LS.tiles;
 }
 
 template <typename T, int ZC, bool EPI>
 void launch_stencil_kernel(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
                            int honor_done, SmoothEpi<T> ep)
 {
+    if constexpr (sizeof(T) == 4) {  // FP64: plane-staged kernel
+        if (use_rows(S, LS, false)) {
+            if (LS.perm == 1)
+                hipLaunchKernelGGL((k_matvec_rows<T, 1, ZC, EPI>), dim3(LS.tiles_rw.n, S->ncols), dim3(kBlock), 0, S->stream,
+                                   L, st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep);
+            else
+                hipLaunchKernelGGL((k_matvec_rows<T, 0, ZC, EPI>), dim3(LS.tiles_rw.n, S->ncols), dim3(kBlock), 0, S->stream,
+                                   L, st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep);
+            return;
+        }
+    }
     if (use_sf(S, LS)) {
         if (LS.perm == 1)
             hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC, EPI>), dim3(LS.tiles_sf.n, S->ncols), dim3(kBlock), 0, S->stream, L,
@@ -4254,7 +4614,7 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
     else launch_stencil_zc<T, kStencilZ>(S, L, LS, x, y, fin, honor_done);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
-        r2.partials += stencil_tiles(S, LS).n;
+        r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
         if (l == 0)
             hipLaunchKernelGGL((k_stencil_fix<T, false, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
                                L, kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{});
@@ -4262,7 +4622,7 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
             hipLaunchKernelGGL((k_stencil_fix<T, true, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
                                L, ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{});
     }
-    enqueue_finalize(S, S->stream, stencil_tiles(S, LS).n + LS.fix_grid, S->ncols, fin, honor_done);
+    enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, honor_done);
 }
 
 // Level-0 operator with the damped cycle's post-smoothing fused in (sum-factorised
@@ -4278,11 +4638,11 @@ void launch_stencil_smooth(vfem_system *S, const T *x, T *z, const double *r, co
     else launch_stencil_zc<T, kStencilZ, true>(S, L, LS, x, (T *)nullptr, fin, 1, ep);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
-        r2.partials += LS.tiles_sf.n;
+        r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
         hipLaunchKernelGGL((k_stencil_fix<T, false, true>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
                            kefine<T>(S), LS.surf, x, (T *)nullptr, S->st, r2, fin, 1, ep);
     }
-    enqueue_finalize(S, S->stream, LS.tiles_sf.n + LS.fix_grid, S->ncols, fin, 1);
+    enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, 1);
 }
 
 // level l uses its stencil form: wherever available under VFEM_L0_STENCIL,
@@ -4291,6 +4651,7 @@ bool level_stencil_on(const vfem_system *S, int l)
 {
     const LevelStencil &LS = S->stl[l];
     return LS.ok && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
+                     S->l0_kind == VFEM_L0_STENCIL_PLANES ||
                      (S->l0_kind == VFEM_L0_AUTO && LS.auto_on));
 }
 bool stencil_on(const vfem_system *S) { return level_stencil_on(S, 0); }
@@ -4467,6 +4828,7 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                            (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
+                                       S->l0_kind == VFEM_L0_STENCIL_PLANES ||
                                        (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
     // the coarser levels run in TN (the level-0 cycle of a reduced-precision plan may
     // keep them in FP64).  The restriction also pre-smooths level l + 1 (z = omega
@@ -5523,11 +5885,11 @@ int vfem_block_config(int32_t index, int32_t *tex, int32_t *tey, int32_t *mz)
 
 int vfem_set_level0_kernel(vfem_system *sys, int32_t kind)
 {
-    if (!sys || kind < VFEM_L0_AUTO || kind > VFEM_L0_STENCIL_DIRECT) {
+    if (!sys || kind < VFEM_L0_AUTO || kind > VFEM_L0_STENCIL_PLANES) {
         g_last_error = "invalid argument";
         return VFEM_ERR_ARGUMENT;
     }
-    if ((kind == VFEM_L0_STENCIL || kind == VFEM_L0_STENCIL_DIRECT) && !sys->st_ok) {
+    if ((kind == VFEM_L0_STENCIL || kind == VFEM_L0_STENCIL_DIRECT || kind == VFEM_L0_STENCIL_PLANES) && !sys->st_ok) {
         g_last_error = "Ke does not assemble to the structured level-0 stencil";
         return VFEM_ERR_UNSUPPORTED;
     }
@@ -5544,7 +5906,8 @@ int32_t vfem_level_operator_form(const vfem_system *sys, int32_t level)
 {
     if (!sys || level < 0 || level >= sys->H.nlev) return -1;
     if (!level_stencil_on(sys, level)) return VFEM_L0_ELEMENT;
-    return use_sf(sys, sys->stl[level]) ? VFEM_L0_STENCIL : VFEM_L0_STENCIL_DIRECT;
+    if (!use_sf(sys, sys->stl[level])) return VFEM_L0_STENCIL_DIRECT;
+    return use_rows(sys, sys->stl[level], false) ? VFEM_L0_STENCIL : VFEM_L0_STENCIL_PLANES;
 }
 
 int32_t vfem_level0_kernel(const vfem_system *sys)
