@@ -4859,14 +4859,17 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     if (pre) S->pre_level = l + 1;
     enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
                                         omega, -1, (const TN *)nullptr);
-    // FP64 cycle, level 0 with the sum-factorised stencil: the prolongation writes
-    // z + P z_c to t and the operator applied to it finishes the cycle with the
+    // Level 0 with the sum-factorised stencil: the prolongation writes z + P z_c
+    // to t and the operator applied to it finishes the cycle with the
     // post-smoothing fused in (z = t + omega D^-1 (r - A t)), saving the separate
-    // smoothing pass (measured at 256^3: 2.78 -> 2.67 ms per FP64 iteration; in
-    // FP32 the fused kernel is slower than the two it replaces, 318 against 294 us)
+    // smoothing pass.  Measured at 256^3: FP64 2.78 -> 2.67 ms per iteration; FP32
+    // with the row-wave kernel 1.54 -> 1.45 ms (with the plane-staged kernel the
+    // FP32 fusion was slower than the two kernels it replaces, 318 against 294 us)
     bool fuse = false;
     if constexpr (std::is_same<TR, double>::value && std::is_same<T, double>::value)
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_sf(S, S->stl[0]);
+    if constexpr (std::is_same<TR, double>::value && std::is_same<T, float>::value)
+        fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_rows(S, S->stl[0], false);
     if (tiled && C.div == 4) {
         hipLaunchKernelGGL((k_prolong4_tile<TN, T, kProlong4Z>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
                            S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
