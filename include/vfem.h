@@ -283,9 +283,10 @@ int vfem_dot_noalloc(const double *a, const double *b, int64_t n, double *out);
  * stencil kernel alone, without the surface pass (FP32 or FP64; fails with
  * VFEM_ERR_UNSUPPORTED when level 0 has no stencil form), 8 = the same kernel as
  * it runs inside CG iterations: `reps` iterations as in 2 (after one untimed),
- * launched eagerly, with each launch of the level-0 stencil kernel (three per
- * iteration: the CG operator and the cycle's two) bracketed by its own events;
- * the mean over those launches.  Kind 7 repeats one launch on the same vectors,
+ * launched eagerly, with each launch of the level-0 stencil kernel without a
+ * fused smoothing epilogue (the CG operator and the cycle's first; the cycle's
+ * second carries the post-smoothing where the stencil form allows it) bracketed
+ * by its own events; the mean over those launches.  Kind 7 repeats one launch on the same vectors,
  * which the last-level cache can partly hold; kind 8 sees the vectors as the
  * solve leaves them; VFEM_ERR_UNSUPPORTED when the plan's iterations launch
  * no level-0 stencil kernel (element form at level 0).  reps is at most 4096. */

@@ -1350,6 +1350,10 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
         red = col_red(red);
     }
     if (honor_done && st->done) return;
+    // the full-element matrix (Ke, or the Galerkin matrix of the full child pattern)
+    __shared__ T kf[576];
+    for (int k = threadIdx.x; k < 576; k += kBlock) kf[k] = ke[(TAB ? 255 * 576 : 0) + k];
+    __syncthreads();
     double dacc = 0;
     for (int64_t base = (int64_t)remap_block() * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
         const int64_t i = base + threadIdx.x;
@@ -1383,28 +1387,42 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
             if (!__any((present >> el) & 1)) continue;
             const int a = el & 1, b = (el >> 1) & 1, c = el >> 2;
             const int Ln = (1 - a) + 2 * (1 - b) + 4 * (1 - c);
-            const T *K = ke + 3 * Ln * 24;
-            if constexpr (TAB) {  // the element's Galerkin matrix (per lane)
-                const unsigned pe = ((el < 4 ? pat.x : pat.y) >> (8 * (el & 3))) & 255u;
-                K += (int64_t)pe * 576;
-            }
-            T s0 = 0, s1 = 0, s2 = 0;
+            // rows of the node's corner of the element matrix times the element's values
+            auto rows3 = [&](const T *K, T &s0, T &s1, T &s2) {
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
+                for (int m = 0; m < 8; m++) {
+                    const int k = ((c + (m >> 2)) * 3 + (b + ((m >> 1) & 1))) * 3 + (a + (m & 1));
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const T v = xv[k][j];
-                    s0 += K[0 * 24 + 3 * m + j] * v;
-                    s1 += K[1 * 24 + 3 * m + j] * v;
-                    s2 += K[2 * 24 + 3 * m + j] * v;
+                    for (int j = 0; j < 3; j++) {
+                        const T v = xv[k][j];
+                        s0 += K[0 * 24 + 3 * m + j] * v;
+                        s1 += K[1 * 24 + 3 * m + j] * v;
+                        s2 += K[2 * 24 + 3 * m + j] * v;
+                    }
                 }
+            };
+            T s0 = 0, s1 = 0, s2 = 0;
+            if constexpr (TAB) {
+                // full elements (pattern 255) share one Galerkin matrix, read from
+                // LDS (one address for the whole wave); a partial element's matrix
+                // is gathered per lane from the pattern table
+                const unsigned pe = ((el < 4 ? pat.x : pat.y) >> (8 * (el & 3))) & 255u;
+                const bool full = pe == 255u, here = (present >> el) & 1;
+                if (__any(full && here))
+                    if (full) rows3(kf + 3 * Ln * 24, s0, s1, s2);
+                if (__any(!full && here))
+                    if (!full) rows3(ke + (int64_t)pe * 576 + 3 * Ln * 24, s0, s1, s2);
+            } else {
+                rows3(kf + 3 * Ln * 24, s0, s1, s2);
             }
             if ((present >> el) & 1) {
                 c0 += s0;
                 c1 += s1;
                 c2 += s2;
             }
+            // keep each element's matrix reads next to its products (hoisting them
+            // all would take every register of the lane)
+            __builtin_amdgcn_sched_barrier(0);
         }
         if (act && EPI) {
             // the smoothed value of a surface node (never fixed), replacing the one
@@ -4598,7 +4616,7 @@ template <typename T, int ZC, bool EPI = false>
 void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
                        int honor_done, SmoothEpi<T> ep = {})
 {
-    const bool probe = &LS == &S->stl[0] && S->probe_n < S->probe_cap;
+    const bool probe = !EPI && &LS == &S->stl[0] && S->probe_n < S->probe_cap;  // the plain operator only
     if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n], S->stream);
     launch_stencil_kernel<T, ZC, EPI>(S, L, LS, x, y, fin, honor_done, ep);
     if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n++ + 1], S->stream);
