@@ -5726,6 +5726,7 @@ int vfem_attach_unet(vfem_system *sys, vfem_unet *net, int32_t level)
         g_last_error = "null system";
         return VFEM_ERR_ARGUMENT;
     }
+    HIP_TRY(hipStreamSynchronize(sys->stream));  // no captured iteration still runs
     for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
     sys->graphs.clear();
     if (!net) {
@@ -5946,6 +5947,7 @@ int vfem_set_block_config(vfem_system *sys, int32_t index)
     }
     if (index == kNumBlockCfgs) sys->batch_l0 = -1;  // the stencil form for the FP32 batch
     else sys->block_cfg = index;
+    HIP_TRY(hipStreamSynchronize(sys->stream));  // no captured iteration still runs
     for (auto &g : sys->graphs) (void)hipGraphExecDestroy(g.second);
     sys->graphs.clear();
     return VFEM_OK;
