@@ -2305,7 +2305,18 @@ template <typename Td> struct DinvRef {
     const Td *d;
     const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
     Td c[3];
+    const Td *cd;          // the same triple in device memory
 };
+// D^-1 at compact DOF k of node nd = k / 3 (flag: the node holds the shared
+// triple).  The triple is read from device memory (cd), so the choice between
+// cd[k - 3 nd] and d[k] is one global load from a chosen address; with the
+// triple as a kernel argument the chosen pointer was a generic one and the load
+// a flat load that waited at once.
+template <typename Td>
+__device__ __forceinline__ Td dinv_at(const DinvRef<Td> &D, bool flag, uint32_t k, uint32_t nd)
+{
+    return flag ? D.cd[k - 3u * nd] : D.d[k];
+}
 
 // The damped cycle's pre-smoothing of the next coarser level, z_c = omega D_c^-1 r_c,
 // written by the restriction that produces r_c (fixed rows: r_c is 0 there, so
@@ -3298,7 +3309,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const uint32_t k = (uint32_t)min(k0 + u * kBlock, R.end - 1), nd = k / 3u;
-            dv[u] = ((cw[u] >> (nd & 31)) & 1) ? D.c[k - 3u * nd] : D.d[k];
+            dv[u] = dinv_at(D, (cw[u] >> (nd & 31)) & 1, k, nd);
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -3447,7 +3458,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const uint32_t i = (uint32_t)min(i0 + u * kBlock, R.end - 1), nd = i / 3u;
-            dv[u] = ((cw[u] >> (nd & 31)) & 1) ? D.c[i - 3u * nd] : D.d[i];
+            dv[u] = dinv_at(D, (cw[u] >> (nd & 31)) & 1, i, nd);
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -3933,6 +3944,8 @@ struct vfem_system {
     // value, dcon; dflat flags (compact-order node bitmask) the nodes that hold it
     uint32_t *dflat[kMaxLevels] = {};
     double dcon[kMaxLevels][3] = {};
+    double *dcon64 = nullptr;  // dcon on the device, [kMaxLevels][3]
+    float *dcon32 = nullptr;
     double *ktab64[kMaxLevels] = {};
     float *ktab32[kMaxLevels] = {};
     double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
@@ -4477,6 +4490,13 @@ int upload_system(vfem_system *S, const DevHierarchy &DH)
             if (int rc = S->upload(&S->ktab32[l], convert<float>(H.ktab[l]))) return rc;
         }
     }
+    {
+        std::vector<double> dc(3 * kMaxLevels);
+        for (int l = 0; l < kMaxLevels; l++)
+            for (int c = 0; c < 3; c++) dc[3 * l + c] = S->dcon[l][c];
+        if (int rc = S->upload(&S->dcon64, dc)) return rc;
+        if (int rc = S->upload(&S->dcon32, convert<float>(dc))) return rc;
+    }
     std::vector<double> ke(H.ke, H.ke + 576);
     if (int rc = S->upload(&S->ke64, ke)) return rc;
     if (int rc = S->upload(&S->ke32, convert<float>(ke))) return rc;
@@ -4513,9 +4533,13 @@ template <> float *level_t<float>(vfem_system *S, int l) { return S->tl32[l]; }
 template <typename T> const T *dinv(vfem_system *S, int l);
 template <> const double *dinv<double>(vfem_system *S, int l) { return S->dinv64[l]; }
 template <> const float *dinv<float>(vfem_system *S, int l) { return S->dinv32[l]; }
+template <typename T> const T *dcon_dev(vfem_system *S, int l);
+template <> const double *dcon_dev<double>(vfem_system *S, int l) { return S->dcon64 + 3 * l; }
+template <> const float *dcon_dev<float>(vfem_system *S, int l) { return S->dcon32 + 3 * l; }
 template <typename T> DinvRef<T> dinv_ref(vfem_system *S, int l)
 {
-    return DinvRef<T>{dinv<T>(S, l), S->dflat[l], {T(S->dcon[l][0]), T(S->dcon[l][1]), T(S->dcon[l][2])}};
+    return DinvRef<T>{dinv<T>(S, l), S->dflat[l], {T(S->dcon[l][0]), T(S->dcon[l][1]), T(S->dcon[l][2])},
+                      dcon_dev<T>(S, l)};
 }
 template <typename T> const T *kefine(vfem_system *S);
 template <> const double *kefine<double>(vfem_system *S) { return S->ke64; }
