@@ -58,6 +58,7 @@ def test_matvec_parity(systems, name):
     say("matvec rel err fp64 %.2e fp32(3-pass bf16 MFMA) %.2e bf16(1 pass) %.2e" % (e64, e32, e16))
     assert e64 < 1e-13
     assert e32 < 1e-4
+    assert e16 < 1e-2  # one BF16 product of element differences (measured 1.6e-3)
     emx = rel(A.matvec(x, V.MIXED), ref)
     say("matvec rel err mixed (6-pass MFMA on FP64 differences) %.2e" % emx)
     assert emx < 1e-6
@@ -551,3 +552,24 @@ def test_changed_geometry_warm_start(ke0):
         B.map_solution_from(V.assemble_system_matrix(vox_a[:, :, :16], ke0, setup(vox_a[:, :, :16])[0], 2))
     A.close()
     B.close()
+
+
+@pytest.mark.parametrize("name", ["ell24", "ell33"])
+def test_bf16_cycle_solve(systems, name):
+    """VFEM_BF16: the damped cycle with one-product BF16 level-0 matvecs under the
+    FP64 outer CG (flexible, true residual) reaches relative residual 1e-10 and
+    the FP64 oracle's displacement to 1e-6, in at most twice the FP32 plan's
+    iterations."""
+    A, ora, vox, fixed = systems[name]
+    ids = problems.compact_ids(vox)
+    top = ids[np.nonzero((ids >= 0).any(axis=(1, 2)))[0].max()]
+    f = np.zeros(A.n)
+    f[3 * top[top >= 0] + 2] = -1.0 / np.count_nonzero(top >= 0)
+    u_ref, _, _ = ora.pcg(f, tol=1e-13, precond=O.DAMPED)
+    u16, s16 = A.solve(f, tol=1e-10, precision=V.BF16, cycle=V.CYCLE_DAMPED)
+    _, s32 = A.solve(f, tol=1e-10, precision=V.FP32, cycle=V.CYCLE_DAMPED)
+    say("bf16 cycle %s: %d iterations (fp32 %d), displacement err %.2e"
+        % (name, s16["iterations"], s32["iterations"], rel(u16, u_ref)))
+    assert s16["converged"] == 1 and s16["rel_residual"] < 1e-10
+    assert rel(u16, u_ref) < 1e-6
+    assert s16["iterations"] <= 2 * s32["iterations"]
