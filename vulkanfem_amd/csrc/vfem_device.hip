@@ -1007,12 +1007,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
 // k_matvec_rows: the sum-factorised stencil of k_matvec_sf with one wavefront
 // per node row.  A block's 4 waves own 4 consecutive y rows of one 62-node x
 // range and march through a chunk of ZC planes; lane j holds x = x0 - 1 + j, so
-// lanes 0 and 63 are halo columns that only feed their neighbours.  The row
-// words of the three rows a wave reads on each plane (y-1, y, y+1) are
-// wave-uniform: scalar loads, shifted into one 64-bit lane mask in scalar
-// registers, and the compact id of a lane is the row's base plus v_mbcnt of that
-// mask, so the values of a row are one contiguous run read from the row base
-// with a 32-bit lane offset.  The y pass takes the three rows from registers and
+// lanes 0 and 63 are halo columns that only feed their neighbours.  What a wave
+// needs of the three rows it reads on each plane (y-1, y, y+1) is wave-uniform
+// and precomputed in the level's row table (RowLane): the used and fixed nodes
+// as 64-bit lane masks and the row's first compact id, read with scalar loads
+// one plane ahead.  A lane's compact id is that base plus v_mbcnt of the mask,
+// so the values of a row are one contiguous run read from the row base with a
+// 32-bit lane offset.  The y pass takes the three rows from registers and
 // the x pass exchanges values across the wave with DPP wave shifts: no LDS and
 // no barrier.  Against k_matvec_sf this drops the staging (LDS writes and reads,
 // the per-lane bit arithmetic of the row words and 64-bit addresses) and the
