@@ -573,3 +573,32 @@ def test_bf16_cycle_solve(systems, name):
     assert s16["converged"] == 1 and s16["rel_residual"] < 1e-10
     assert rel(u16, u_ref) < 1e-6
     assert s16["iterations"] <= 2 * s32["iterations"]
+
+
+@pytest.mark.parametrize("nx", [61, 62, 63, 64, 65, 124, 127, 129])
+def test_row_kernel_tile_boundaries(ke0, nx):
+    """The FP32 row-wave stencil kernel on grids whose x extent puts tile and
+    64-node word boundaries everywhere (62-node tiles, lane 0 at x = 62 t - 1):
+    bit for bit the plane-staged kernel, and the oracle's operator to 2e-6."""
+    # a 6 x 7 x nx slab with interior voxel holes (surface nodes throughout),
+    # fixed at z = 0
+    z, y, xg = np.meshgrid(np.arange(6), np.arange(7), np.arange(nx), indexing="ij")
+    vox = (~((xg % 3 == 1) & (y % 3 == 1) & (z % 2 == 1))).astype(np.int32)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    A = V.assemble_system_matrix(vox, ke0, fixed, 2)
+    ora = O.OracleSystem(vox, ke0, fixed, 2)
+    x = np.random.default_rng(nx).standard_normal(A.n)
+    ref = ora.matvec(x)
+    ora.close()
+    try:
+        A.set_level0_kernel(V.L0_STENCIL)
+        y_rows = A.matvec(x, V.FP32)
+        A.set_level0_kernel(V.L0_STENCIL_PLANES)
+        y_planes = A.matvec(x, V.FP32)
+    finally:
+        A.set_level0_kernel(V.L0_AUTO)
+    say("row kernel, %d x-nodes: rel err %.2e" % (nx + 1, rel(y_rows, ref)))
+    assert np.array_equal(y_rows, y_planes)
+    assert rel(y_rows, ref) < 2e-6
+    A.close()
