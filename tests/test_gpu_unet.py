@@ -96,7 +96,8 @@ def test_unet_learns_coarse_correction_and_solves():
     say("loss first/last epoch %.3f %.3f; validation error untrained %.3f trained %.3f, the cycle's own %.3f"
         % (losses[0], losses[-1], before, after, cycle))
     # the kernels train: the loss on the training pairs falls; the held-out error is
-    # reported next to the plain cycle's (the judge), with no claim on its size here
+    # reported next to the plain cycle's (the yardstick), with no claim on its size:
+    # the network does not beat the cycle's own correction (a measured negative result)
     assert losses[-1] < 0.25 * losses[0]
     assert np.isfinite(after)
     # A/B on a held-out geometry: the plain damped cycle against the learned correction
