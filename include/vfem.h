@@ -98,6 +98,13 @@ typedef struct {
                                 cycle in reduced precision; default 0 (operator in the cycle's precision) */
     int32_t coarse_fp64;     /* VFEM_FP32 / VFEM_BF16 with the damped cycle: 1 = only level 0 of the cycle
                                 in reduced precision, the coarser levels in FP64; default 0 */
+    int32_t cheb_level;      /* damped cycle: levels >= cheb_level (>= 1; the coarsest level keeps its
+                                exact solve) smooth with cheb_degree Chebyshev steps before and after
+                                the coarse correction instead of one damped Jacobi step, fitted to
+                                [lmax / cheb_ratio, lmax] of D^-1 A (vfem_level_lmax); the cycle stays
+                                symmetric positive definite.  0 = damped Jacobi on every level (default) */
+    int32_t cheb_degree;     /* Chebyshev steps, 1..8; default 2 */
+    double cheb_ratio;       /* > 1; default 30 */
 } vfem_options;
 
 typedef struct {
@@ -142,6 +149,12 @@ int vfem_level_extent(const vfem_system *sys, int32_t level, int32_t *nx, int32_
 
 /* Per-level sizes: elements, used nodes, fixed nodes of level `level`. */
 int vfem_level_info(const vfem_system *sys, int32_t level, int64_t *n_elems, int64_t *n_nodes, int64_t *n_fixed);
+
+/* Upper end of the Chebyshev smoother's interval on level `level` (no reference
+ * counterpart: the reference smooths with D^-1 only): a bound of lambda_max(D^-1 A)
+ * that vfem_assemble forms by 30 FP64 power steps from a fixed start, times 1.1.
+ * 0 on level 0 and on the coarsest level, which never use it. */
+int vfem_level_lmax(const vfem_system *sys, int32_t level, double *out);
 
 /* Replaces applyBoundaryConditions (include/FEM.hpp:63-64, src/FEM.cpp:568-584,
  * MATRIX_FREE branch): f[3*node + c] = loads[3*k + c] for node = loaded_nodes[k]. */

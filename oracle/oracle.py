@@ -69,6 +69,8 @@ def lib():
     L.vo_nfree.restype = ctypes.c_int64
     L.vo_get_freedof.argtypes = [vp, _i32p]
     L.vo_get_kc.argtypes = [vp, ctypes.c_int, _f64p]
+    L.vo_set_chebyshev.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _f64p]
+    L.vo_set_chebyshev.restype = ctypes.c_int
     L.vo_set_threads.argtypes = [ctypes.c_int]
     L.vo_get_threads.restype = ctypes.c_int
     _lib = L
@@ -222,6 +224,16 @@ class OracleSystem:
         out = np.zeros(3 * self.nodes(l))
         self._lib.vo_interpolate(self._h, l, xc, out)
         return out
+
+    def set_chebyshev(self, level, degree=2, ratio=30.0, lmax=None):
+        """Damped cycle: Chebyshev smoothing (`degree` steps before and after) on
+        levels >= level, fitted to [lmax[l] / ratio, lmax[l]]; level 0 = damped
+        Jacobi everywhere."""
+        lm = np.zeros(self.num_levels)
+        if lmax is not None:
+            lm[: len(lmax)] = lmax[: self.num_levels]
+        if self._lib.vo_set_chebyshev(self._h, int(level), int(degree), float(ratio), lm) != 0:
+            raise ValueError("invalid Chebyshev smoother settings")
 
     def vcycle(self, r, mode=REF_ADDITIVE, omega=0.6):
         r = np.ascontiguousarray(r, dtype=np.float64)

@@ -75,6 +75,11 @@ typedef struct {
        float, FEMGPU.cpp:226-242) */
     double *rcoef32[VO_MAXL];
     double *invdiag32[VO_MAXL];
+    /* Chebyshev smoothing of the damped cycle on levels >= cheb_level (0 = off):
+       degree cheb_degree, fitted to [lmax/ratio, lmax] of D^-1 A per level */
+    int cheb_level, cheb_degree;
+    double cheb_ratio;
+    double cheb_lmax[VO_MAXL];
 } vo_sys;
 
 /* ------------------------------------------------------------------------- */
@@ -722,19 +727,62 @@ static void mask_fixed(const vo_level *L, double *v)
         if (L->fixed[k]) v[3 * k] = v[3 * k + 1] = v[3 * k + 2] = 0;
 }
 
+/* Chebyshev smoothing of A x = r on level l (Saad, "Iterative Methods for Sparse
+ * Linear Systems", Alg. 12.1, preconditioned by D^-1), `steps` steps from x, or
+ * from x = 0 when zero != 0, on the interval [lmax/ratio, lmax]; t is scratch.
+ * With x = 0 the first step needs no operator product. */
+static void cheb_smooth(const vo_sys *s, int l, const double *r, double *x, double *t, int zero, int steps)
+{
+    const vo_level *L = &s->L[l];
+    int64_t n3 = 3 * L->nn;
+    double hi = s->cheb_lmax[l], lo = hi / s->cheb_ratio;
+    double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta, rho = 1 / sigma;
+    double *d = (double *)malloc(sizeof(double) * n3);
+    for (int k = 0; k < steps; k++) {
+        double a, b;
+        if (k == 0) {
+            a = 0;
+            b = 1 / theta;
+        } else {
+            double rho1 = 1 / (2 * sigma - rho);
+            a = rho1 * rho;
+            b = 2 * rho1 / delta;
+            rho = rho1;
+        }
+        if (k == 0 && zero) {
+            for (int64_t i = 0; i < n3; i++) t[i] = 0;
+        } else if (l == 0) vo_matvec(s, x, t); else vo_level_matvec(s, l, x, t);
+        for (int64_t i = 0; i < n3; i++) {
+            d[i] = (k ? a * d[i] : 0.0) + b * L->invdiag[i] * (r[i] - t[i]);
+            x[i] = (k == 0 && zero ? 0.0 : x[i]) + d[i];
+        }
+        mask_fixed(L, x);
+        for (int64_t i = 0; i < L->nn; i++)
+            if (L->fixed[i]) d[3 * i] = d[3 * i + 1] = d[3 * i + 2] = 0;
+    }
+    free(d);
+}
+
 /* Damped multiplicative V-cycle (no reference counterpart: the configured
- * damped-Jacobi variant with one pre- and one post-smooth).  Level operator:
- * vo_level_matvec; transfers: the reference's restrict/interpolate; coarsest:
- * Kc assembled from the same element matrices.  Fixed DOFs stay zero. */
+ * damped-Jacobi variant with one pre- and one post-smooth; on levels >=
+ * cheb_level, Chebyshev smoothing of cheb_degree steps before and after).
+ * Level operator: vo_level_matvec; transfers: the reference's
+ * restrict/interpolate; coarsest: Kc assembled from the same element matrices.
+ * Fixed DOFs stay zero. */
 static void vcycle_damped_level(const vo_sys *s, int l, const double *r, double *x, double omega)
 {
     const vo_level *L = &s->L[l];
     int64_t n3 = 3 * L->nn;
     if (l == s->nlev - 1) { coarse_solve(s, s->kcg, r, x); return; }
+    const int cheb = s->cheb_level > 0 && l >= s->cheb_level;
     double *t = (double *)malloc(sizeof(double) * n3);
     double *res = (double *)malloc(sizeof(double) * n3);
-    for (int64_t k = 0; k < n3; k++) x[k] = omega * L->invdiag[k] * r[k];
-    mask_fixed(L, x);
+    if (cheb) {
+        cheb_smooth(s, l, r, x, t, 1, s->cheb_degree);
+    } else {
+        for (int64_t k = 0; k < n3; k++) x[k] = omega * L->invdiag[k] * r[k];
+        mask_fixed(L, x);
+    }
     if (l == 0) vo_matvec(s, x, t); else vo_level_matvec(s, l, x, t);
     for (int64_t k = 0; k < n3; k++) res[k] = r[k] - t[k];
     const vo_level *C = &s->L[l + 1];
@@ -746,10 +794,30 @@ static void vcycle_damped_level(const vo_sys *s, int l, const double *r, double 
     interpolate_level(s, l, xc, t, L->rcoef);
     mask_fixed(L, t);
     for (int64_t k = 0; k < n3; k++) x[k] += t[k];
-    if (l == 0) vo_matvec(s, x, t); else vo_level_matvec(s, l, x, t);
-    for (int64_t k = 0; k < n3; k++) x[k] += omega * L->invdiag[k] * (r[k] - t[k]);
-    mask_fixed(L, x);
+    if (cheb) {
+        cheb_smooth(s, l, r, x, t, 0, s->cheb_degree);
+    } else {
+        if (l == 0) vo_matvec(s, x, t); else vo_level_matvec(s, l, x, t);
+        for (int64_t k = 0; k < n3; k++) x[k] += omega * L->invdiag[k] * (r[k] - t[k]);
+        mask_fixed(L, x);
+    }
     free(t); free(res); free(rc); free(xc);
+}
+
+/* Chebyshev smoothing of the damped cycle on levels >= level (0 = off), with
+ * `degree` steps before and after the coarse correction, fitted to
+ * [lmax[l] / ratio, lmax[l]] (lmax: one bound of lambda_max(D^-1 A) per level). */
+int vo_set_chebyshev(vo_sys *s, int level, int degree, double ratio, const double *lmax)
+{
+    if (level < 0 || (level > 0 && (degree < 1 || !(ratio > 1)))) return 1;
+    s->cheb_level = level;
+    s->cheb_degree = degree;
+    s->cheb_ratio = ratio;
+    for (int l = 0; l < s->nlev; l++) s->cheb_lmax[l] = level > 0 ? lmax[l] : 0;
+    if (level > 0)
+        for (int l = level; l < s->nlev - 1; l++)
+            if (!(s->cheb_lmax[l] > 0)) return 1;
+    return 0;
 }
 
 /* mode 0: reference additive cycle, 1: reference additive cycle with FP32 mu/invDiag,

@@ -177,10 +177,34 @@ class Hierarchy:
         """Free-DOF block of the coarsest Galerkin operator (the damped cycle's Kc)."""
         return self.A[-1].toarray()[np.ix_(self.free, self.free)]
 
-    def vcycle_damped(self, r, omega):
+    def chebyshev_error(self, l, e, degree, ratio, lmax):
+        """Error after `degree` Chebyshev smoothing steps on level l, in closed form:
+        e -> T_m((theta I - M) / delta) e / T_m(theta / delta) on the free DOFs,
+        M = D^-1 A, [theta - delta, theta + delta] = [lmax / ratio, lmax] (the
+        error-reduction polynomial of Chebyshev iteration, Saad Sec. 12.1), built
+        with the polynomial's own three-term recurrence T_k+1 = 2 y T_k - T_k-1."""
+        free = ~fixed_dof_mask(len(e) // 3, self.fixed[l])
+        Aff = self.A[l].tocsr()[free][:, free]
+        dff = self.invdiag(l)[free]
+        lo, hi = lmax / ratio, lmax
+        theta, delta = (hi + lo) / 2, (hi - lo) / 2
+        Y = lambda v: (theta * v - dff * (Aff @ v)) / delta  # noqa: E731
+        t0, t1 = e[free], Y(e[free])
+        for _ in range(degree - 1):
+            t0, t1 = t1, 2 * Y(t1) - t0
+        tm = t1 if degree >= 1 else t0
+        out = np.zeros_like(e)
+        out[free] = tm / np.polynomial.chebyshev.chebval(theta / delta, [0] * degree + [1])
+        return out
+
+    def vcycle_damped(self, r, omega, cheb=None):
         """Multiplicative V(1,1) with damped Jacobi written out with explicit
         matrices: x = w D^-1 r; x += P (cycle of P^T (r - A x)); x += w D^-1 (r - A x);
-        fixed DOFs held at zero on every level; exact solve on the coarsest free DOFs."""
+        fixed DOFs held at zero on every level; exact solve on the coarsest free DOFs.
+        cheb = (level, degree, ratio, lmax per level): on levels >= level the
+        smoothing before and after the correction is `degree` Chebyshev steps,
+        applied through their error polynomial (chebyshev_error) and the exact
+        level solution x* = A_ff^-1 r_f."""
         L = self.nlev
 
         def level(l, rl):
@@ -191,14 +215,21 @@ class Hierarchy:
                 return x
             A = self.level_operator(l)
             d = self.invdiag(l)
-            x = omega * d * rl
+            ch = cheb is not None and cheb[0] > 0 and l >= cheb[0]
+            if ch:
+                xs = np.zeros_like(rl)
+                xs[~fm] = np.linalg.solve(self.A[l].toarray()[np.ix_(~fm, ~fm)], rl[~fm])
+                smooth = lambda x0: xs - self.chebyshev_error(l, xs - x0, cheb[1], cheb[2], cheb[3][l])  # noqa: E731
+                x = smooth(np.zeros_like(rl))
+            else:
+                x = omega * d * rl
             x[fm] = 0
             rc = self.P[l].T @ (rl - A @ x)
             rc[fixed_dof_mask(len(rc) // 3, self.fixed[l + 1])] = 0
             t = self.P[l] @ level(l + 1, rc)
             t[fm] = 0
             x = x + t
-            x = x + omega * d * (rl - A @ x)
+            x = smooth(x) if ch else x + omega * d * (rl - A @ x)
             x[fm] = 0
             return x
 

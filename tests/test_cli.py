@@ -71,7 +71,8 @@ def test_cli_reports_missing_and_malformed_files(tmp_path, ke0):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("plan", [(), ("--skip", "1"), ("--precision", "fp32", "--cycle", "damped"),
-                                  ("--skip", "1", "--cycle", "damped")])
+                                  ("--skip", "1", "--cycle", "damped"),
+                                  ("--precision", "fp32", "--cycle", "damped", "--cheb-level", "1")])
 def test_cli_solves_reference_format_model(tmp_path, ke0, plan):
     from oracle import oracle as O
 

@@ -4,8 +4,9 @@ against the CPU oracle on the same seeded inputs:
     ellipsoids128   128^3, 60 ellipsoids, 5 levels, plain FP64 (~2.98M DOF)
     batched128x16   128^3 geometry (seed 3), 16 patch load cases as one block
     ellipsoids256   256^3, seed 2, 120 ellipsoids, 6 levels (~32.6M DOF): the
-                    level-0 operator and one damped cycle against the oracle,
-                    mixed precision against FP64
+                    level-0 operator and one damped cycle (damped Jacobi, and
+                    Chebyshev smoothing from level 2) against the oracle, mixed
+                    precision against FP64
     learned         the same 256^3 solve with the trained U-Net at level 3
 
 Tolerances: FP64 operators to 1e-14 and cycles to 1e-12 relative; every solve
@@ -66,6 +67,30 @@ def test_config2_damped_fp64_matches_oracle(ell128, omega):
         % (omega, st["iterations"], it_ora, st["rel_residual"], err, st["solve_ms"]))
     assert st["converged"] == 1 and st["rel_residual"] < 1e-8
     assert within(st["iterations"], it_ora), (st["iterations"], it_ora)
+    assert err < 1e-6
+
+
+def test_config2_chebyshev_matches_oracle(ell128):
+    """Config 2 with Chebyshev smoothing (degree 2, ratio 30) on levels >= 2 of the
+    damped cycle, FP64: the oracle's twin, given the library's lambda_max bounds,
+    takes the same iteration count to 2 %, fewer than damped Jacobi everywhere,
+    and both reach the oracle's displacement to 1e-6."""
+    p, A, ora, u_ref = ell128
+    f = p.loads[0]
+    lmax = [A.level_lmax(l) for l in range(A.num_levels)]
+    ora.set_chebyshev(2, 2, 30.0, lmax)
+    try:
+        _, it_ora, _ = ora.pcg(f, tol=1e-8, precond=O.DAMPED, omega=0.5)
+    finally:
+        ora.set_chebyshev(0)
+    u, st = A.solve(f, tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.5, cheb_level=2)
+    _, st_j = A.solve(f, tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.5)
+    err = rel(u, u_ref)
+    say("config 2 chebyshev from level 2: GPU %d iterations (damped Jacobi %d), oracle %d, displacement err %.2e, "
+        "%.1f ms" % (st["iterations"], st_j["iterations"], it_ora, err, st["solve_ms"]))
+    assert st["converged"] == 1 and st["rel_residual"] < 1e-8
+    assert within(st["iterations"], it_ora), (st["iterations"], it_ora)
+    assert st["iterations"] < st_j["iterations"]
     assert err < 1e-6
 
 
@@ -137,7 +162,13 @@ def test_config3_operator_and_cycle_match_oracle(ell256, ke0):
     r[np.repeat(ora.fixed(0), 3)] = 0
     y_ref = ora.matvec(x)
     z_ref = ora.vcycle(r, O.DAMPED, 0.5)
+    # the cycle with Chebyshev smoothing on levels >= 2, given the library's bounds
+    ora.set_chebyshev(2, 2, 30.0, [A.level_lmax(l) for l in range(A.num_levels)])
+    zc_ref = ora.vcycle(r, O.DAMPED, 0.5)
     ora.close()
+    ec = rel(V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.5, cheb_level=2).solve(r), zc_ref)
+    say("256^3 damped cycle with Chebyshev smoothing from level 2: %.2e" % ec)
+    assert ec < 1e-12
     try:
         for kind in (V.L0_STENCIL, V.L0_ELEMENT):
             A.set_level0_kernel(kind)
@@ -149,17 +180,21 @@ def test_config3_operator_and_cycle_match_oracle(ell256, ke0):
         A.set_level0_kernel(V.L0_AUTO)
 
 
-def test_config3_mixed_matches_fp64(ell256):
+@pytest.mark.parametrize("cheb_level", [0, 2])
+def test_config3_mixed_matches_fp64(ell256, cheb_level):
     """Config 3: the mixed plan (FP32 damped cycle, FP64 outer CG, true residual
     every 20 iterations) and plain FP64 both reach true relative residual 1e-8 and
-    agree to 1e-6 (the bench reports both against the CPU oracle's solution)."""
+    agree to 1e-6 (the bench reports both against the CPU oracle's solution), with
+    damped Jacobi on every level and with Chebyshev smoothing from level 2 (the
+    measured plan)."""
     p, A, _ = ell256
     f = p.loads[0]
-    u64, s64 = A.solve(f, tol=1e-8, precision=V.FP64, cycle=V.CYCLE_DAMPED, omega=0.5)
-    u32, s32 = A.solve(f, tol=1e-8, precision=V.FP32, cycle=V.CYCLE_DAMPED, omega=0.5, true_res_every=20)
-    say("config 3: fp64 %d iterations %.1f ms, mixed %d iterations %.1f ms, relres %.2e / %.2e, diff %.2e"
-        % (s64["iterations"], s64["solve_ms"], s32["iterations"], s32["solve_ms"], s64["rel_residual"],
-           s32["rel_residual"], rel(u32, u64)))
+    kw = dict(cycle=V.CYCLE_DAMPED, omega=0.5, cheb_level=cheb_level)
+    u64, s64 = A.solve(f, tol=1e-8, precision=V.FP64, **kw)
+    u32, s32 = A.solve(f, tol=1e-8, precision=V.FP32, true_res_every=20, **kw)
+    say("config 3 (cheb_level %d): fp64 %d iterations %.1f ms, mixed %d iterations %.1f ms, relres %.2e / %.2e, "
+        "diff %.2e" % (cheb_level, s64["iterations"], s64["solve_ms"], s32["iterations"], s32["solve_ms"],
+                       s64["rel_residual"], s32["rel_residual"], rel(u32, u64)))
     assert s64["converged"] == 1 and s64["rel_residual"] < 1e-8
     assert s32["converged"] == 1 and s32["rel_residual"] < 1e-8 and s32["true_res_count"] >= 1
     assert rel(u32, u64) < 1e-6

@@ -151,6 +151,56 @@ def test_damped_vcycle_matches_explicit_v11(ke0, damped_cases, omega):
         s.close()
 
 
+def exact_lmax(bf, l):
+    """lambda_max(D^-1 A) on the free DOFs of level l, by a dense symmetric eigensolve
+    of D^-1/2 A D^-1/2."""
+    free = ~BF.fixed_dof_mask(bf.A[l].shape[0] // 3, bf.fixed[l])
+    A = bf.A[l].toarray()[np.ix_(free, free)]
+    s = 1 / np.sqrt(A.diagonal())
+    return float(np.linalg.eigvalsh(s[:, None] * A * s[None, :]).max())
+
+
+@pytest.mark.parametrize("degree,ratio", [(2, 30.0), (3, 8.0)])
+def test_damped_chebyshev_cycle_matches_closed_form(ke0, damped_cases, degree, ratio):
+    """Chebyshev smoothing on the coarse levels of the damped cycle: the
+    oracle's three-term Chebyshev iteration (Saad Alg. 12.1) against the
+    closed-form error polynomial T_m((theta - D^-1 A) / delta) / T_m(theta / delta)
+    applied with explicit matrices (4 levels, Chebyshev on levels 1 and 2).  A
+    wrong recurrence coefficient or a missed first-step shortcut fails here."""
+    rng = np.random.default_rng(14)
+    for vox, fixed in damped_cases:
+        s = O.OracleSystem(vox, ke0, fixed, 4)
+        bf = BF.Hierarchy(vox, ke0, fixed, 4)
+        lmax = [0.0] + [1.1 * exact_lmax(bf, l) for l in (1, 2)] + [0.0]
+        s.set_chebyshev(1, degree, ratio, lmax)
+        r = rng.standard_normal(s.ndof)
+        z = s.vcycle(r, O.DAMPED, 0.5)
+        zb = bf.vcycle_damped(r, 0.5, cheb=(1, degree, ratio, lmax))
+        assert np.abs(z - zb).max() <= 1e-10 * np.abs(zb).max()
+        # the Jacobi cycle is a different preconditioner
+        s.set_chebyshev(0)
+        assert np.abs(s.vcycle(r, O.DAMPED, 0.5) - zb).max() > 1e-3 * np.abs(zb).max()
+        s.close()
+
+
+def test_damped_chebyshev_cycle_is_symmetric_positive(ke0, small_case):
+    """With the same polynomial before and after the correction the cycle stays
+    a symmetric positive definite preconditioner (plain CG needs that)."""
+    vox, fixed = small_case
+    s = O.OracleSystem(vox, ke0, fixed, 3)
+    bf = BF.Hierarchy(vox, ke0, fixed, 3)
+    s.set_chebyshev(1, 2, 30.0, [0.0, 1.1 * exact_lmax(bf, 1), 0.0])
+    rng = np.random.default_rng(15)
+    mask = np.repeat(s.fixed(0), 3)
+    a, b = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    a[mask] = 0
+    b[mask] = 0
+    Ma, Mb = s.vcycle(a, O.DAMPED, 0.5), s.vcycle(b, O.DAMPED, 0.5)
+    assert abs(b @ Ma - a @ Mb) < 1e-10 * abs(a @ Mb)
+    assert a @ Ma > 0 and b @ Mb > 0
+    s.close()
+
+
 def test_damped_skip_level_cycle_matches_brute_force(ke0):
     """The damped cycle over a x4 (skip) level 1 (FEM.cpp:266-272): the oracle's
     level operators, coarsest Galerkin matrix and V(1,1) cycle against the

@@ -41,7 +41,8 @@ class Options(ctypes.Structure):
                 ("check_every", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("warm_start", ctypes.c_int32),
                 ("op_passes", ctypes.c_int32), ("cycle_passes", ctypes.c_int32), ("learned", ctypes.c_int32),
                 ("stall_ratio", ctypes.c_double), ("op_fp64", ctypes.c_int32),
-                ("coarse_fp64", ctypes.c_int32)]
+                ("coarse_fp64", ctypes.c_int32), ("cheb_level", ctypes.c_int32), ("cheb_degree", ctypes.c_int32),
+                ("cheb_ratio", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -89,6 +90,7 @@ def lib():
     L.vfem_num_levels.argtypes = [vp]
     L.vfem_num_levels.restype = ctypes.c_int32
     L.vfem_level_info.argtypes = [vp, ctypes.c_int32] + [ctypes.POINTER(ctypes.c_int64)] * 3
+    L.vfem_level_lmax.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     L.vfem_apply_boundary_conditions.argtypes = [_f64p, ctypes.c_int64, _i64p, _f64p, ctypes.c_int64]
     L.vfem_solve.argtypes = [vp, _f64p, _f64p, ctypes.POINTER(Options), ctypes.POINTER(Stats)]
     L.vfem_solve_batched.argtypes = [vp, _f64p, _f64p, ctypes.c_int32, ctypes.POINTER(Options),
@@ -281,6 +283,12 @@ class MatrixFreeSparse:
         _check(self._lib.vfem_level_data(self._h, int(level), inv.ctypes.data_as(ctypes.c_void_p),
                                          mask.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nfree)))
         return {"invdiag": inv, "fixed": mask.astype(bool), "free_dofs": nfree.value}
+
+    def level_lmax(self, level):
+        """Upper end of the Chebyshev smoother's interval on `level` (vfem_level_lmax)."""
+        out = ctypes.c_double()
+        _check(self._lib.vfem_level_lmax(self._h, int(level), ctypes.byref(out)))
+        return out.value
 
     def level_info(self, level):
         ne, nn, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -8,7 +8,7 @@
  *
  *     vfem_cli <model>.dat [-n levels] [--skip k]
  *              [--precision fp64|fp32|bf16|mixed] [--cycle reference|damped]
- *              [--omega w] [--tol t] [--no-save]
+ *              [--omega w] [--cheb-level l] [--tol t] [--no-save]
  *
  * Files, relative to the working directory as in the reference:
  *   <model>.dat          "X Y Z" then Z lines of X*Y comma-separated integers (x fastest);
@@ -43,6 +43,7 @@ struct Args {
     int precision = VFEM_FP64;
     int cycle = VFEM_CYCLE_REFERENCE;
     double omega = -1;
+    int cheb_level = 0;  // damped cycle: Chebyshev smoothing on levels >= cheb_level (0: damped Jacobi)
     double tol = -1;
     bool save = true;
 };
@@ -147,6 +148,7 @@ Args parse(int argc, char **argv)
             else if (c == "damped") a.cycle = VFEM_CYCLE_DAMPED;
             else fail("unknown cycle " + c);
         } else if (k == "--omega") a.omega = std::strtod(val().c_str(), nullptr);
+        else if (k == "--cheb-level") a.cheb_level = std::atoi(val().c_str());
         else if (k == "--tol") a.tol = std::strtod(val().c_str(), nullptr);
         else if (k == "--no-save") a.save = false;
         else fail("unknown option " + k);
@@ -197,6 +199,7 @@ int main(int argc, char **argv)
         o.precision = a.precision;
         o.cycle = a.cycle;
         if (a.omega > 0) o.omega = a.omega;
+        o.cheb_level = a.cheb_level;
         if (a.tol >= 0) o.tol = a.tol;
         K.setOptions(o);
         vfem::solveWithHIP(K, f, u);

@@ -30,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <type_traits>
 #include <mutex>
 #include <string>
@@ -3331,6 +3332,65 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
     if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
+// One Chebyshev smoothing step on a level, flat over its DOFs: d = a d_prev +
+// b D^-1 (r - t) (t = A x), x += d, where d_prev is d, or x itself when `first`
+// (the step after a start from zero, whose first direction is the x it wrote);
+// d is stored only when another step follows.  Fixed rows stay zero.
+template <typename Tr, typename Td, typename Tz>
+__global__ __launch_bounds__(kBlock) void k_cheb_step(int64_t n3, const uint32_t *__restrict__ fixc,
+                                                       const Tr *__restrict__ r, const Tz *__restrict__ t,
+                                                       DinvRef<Td> D, Tz *__restrict__ x, Tz *__restrict__ d,
+                                                       double a, double b, int first, int write_d, CGState *st)
+{
+    {
+        const int64_t co = col_offset(n3);
+        r += co;
+        t += co;
+        x += co;
+        d += co;
+        st = col_state(st);
+    }
+    if (st->done) return;
+    const Range R = flat_range(n3);
+    const Tz ta = Tz(a), tb = Tz(b);
+    for (int64_t k0 = R.begin + threadIdx.x; k0 < R.end; k0 += kBlock * FLAT_U) {
+        Tr rv[FLAT_U];
+        Td dv[FLAT_U];
+        Tz xv[FLAT_U], tv[FLAT_U], pv[FLAT_U];
+        uint32_t fw[FLAT_U], cw[FLAT_U];
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const uint32_t nd = (uint32_t)min(k0 + u * kBlock, R.end - 1) / 3u;
+            fw[u] = fixc[nd >> 5];
+            cw[u] = D.flat ? D.flat[nd >> 5] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t k = min(k0 + u * kBlock, R.end - 1);
+            rv[u] = r[k];
+            tv[u] = t[k];
+            xv[u] = x[k];
+            pv[u] = (a != 0 && !first) ? d[k] : Tz(0);
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const uint32_t k = (uint32_t)min(k0 + u * kBlock, R.end - 1), nd = k / 3u;
+            dv[u] = dinv_at(D, (cw[u] >> (nd & 31)) & 1, k, nd);
+        }
+#pragma unroll
+        for (int u = 0; u < FLAT_U; u++) {
+            const int64_t k = k0 + u * kBlock;
+            if (k >= R.end) break;
+            const bool fx = (fw[u] >> (((uint32_t)k / 3u) & 31)) & 1;
+            const Tz dp = first ? xv[u] : pv[u];
+            Tz dn = ta * dp + tb * Tz(dv[u]) * (Tz(rv[u]) - tv[u]);
+            if (fx) dn = 0;
+            x[k] = xv[u] + dn;
+            if (write_d) d[k] = dn;
+        }
+    }
+}
+
 // Coarsest level: z[freedof[i]] = sum_k Kc^-1[i][k] r[freedof[k]] (replaces the
 // host SimplicialLDLT solve of FEMGPU.cpp:509-521).  One wavefront per row.
 template <typename Tk, typename Tv>
@@ -3927,7 +3987,7 @@ struct LevelStencil {
 };
 
 // captured-iteration cache key: the plan's discrete choices and omega's bits
-using GraphKey = std::pair<int64_t, uint64_t>;
+using GraphKey = std::tuple<int64_t, uint64_t, uint64_t>;
 
 struct vfem_system {
     Hierarchy H;
@@ -3959,6 +4019,14 @@ struct vfem_system {
     // coarse-level vectors (index = level), t = operator scratch of the damped cycle
     double *rl64[kMaxLevels] = {}, *zl64[kMaxLevels] = {}, *tl64[kMaxLevels] = {};
     float *rl32[kMaxLevels] = {}, *zl32[kMaxLevels] = {}, *tl32[kMaxLevels] = {};
+    // Chebyshev smoothing: the step direction per coarse level, the bound of
+    // lambda_max(D^-1 A) per level (from vfem_assemble), and the settings of the
+    // plan being enqueued (cheb_level 0: damped Jacobi everywhere)
+    double *dl64[kMaxLevels] = {};
+    float *dl32[kMaxLevels] = {};
+    double lmax[kMaxLevels] = {};
+    int cheb_level = 0, cheb_degree = 2;
+    double cheb_ratio = 30.0;
     float *t0_32 = nullptr;
     CGState *st = nullptr;
     Reduction red{};
@@ -4435,6 +4503,10 @@ int alloc_vectors(vfem_system *S, int nb)
         if (int rc = S->alloc(&S->rl32[l], m, true, V)) return rc;
         if (int rc = S->alloc(&S->zl32[l], m, true, V)) return rc;
         if (int rc = S->alloc(&S->tl32[l], m, true, V)) return rc;
+        if (l + 1 < H.nlev) {
+            if (int rc = S->alloc(&S->dl64[l], m, true, V)) return rc;
+            if (int rc = S->alloc(&S->dl32[l], m, true, V)) return rc;
+        }
     }
     if (int rc = S->alloc(&S->st, nb, true, V)) return rc;
     {
@@ -4531,6 +4603,39 @@ template <> float *level_z<float>(vfem_system *S, int l) { return S->zl32[l]; }
 template <typename T> T *level_t(vfem_system *S, int l);
 template <> double *level_t<double>(vfem_system *S, int l) { return S->tl64[l]; }
 template <> float *level_t<float>(vfem_system *S, int l) { return S->tl32[l]; }
+template <typename T> T *level_d(vfem_system *S, int l);
+template <> double *level_d<double>(vfem_system *S, int l) { return S->dl64[l]; }
+template <> float *level_d<float>(vfem_system *S, int l) { return S->dl32[l]; }
+
+// Chebyshev smoothing (Saad, "Iterative Methods for Sparse Linear Systems", Alg.
+// 12.1, preconditioned by D^-1) on [lmax / ratio, lmax]: step k updates the
+// direction d_k = a_k d_k-1 + b_k D^-1 (r - A x_k) and x_k+1 = x_k + d_k, with
+// a_0 = 0, b_0 = 1 / theta and, for k >= 1, rho_k = 1 / (2 sigma - rho_k-1),
+// a_k = rho_k rho_k-1, b_k = 2 rho_k / delta (theta, delta: centre and half
+// width of the interval, sigma = theta / delta, rho_0 = 1 / sigma).
+struct ChebCoef {
+    double a, b;
+};
+ChebCoef cheb_coef(double lmax, double ratio, int k)
+{
+    const double hi = lmax, lo = lmax / ratio, theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    if (k == 0) return {0.0, 1.0 / theta};
+    double rho = 1.0 / sigma, rho1 = rho;
+    for (int i = 1; i <= k; i++) {
+        rho1 = 1.0 / (2.0 * sigma - rho);
+        if (i < k) rho = rho1;
+    }
+    return {rho1 * rho, 2.0 * rho1 / delta};
+}
+// level l of the cycle being enqueued smooths with the Chebyshev polynomial
+bool cheb_on(const vfem_system *S, int l) { return S->cheb_level >= 1 && l >= S->cheb_level && l + 1 < S->H.nlev; }
+// weight of the smoothing that starts from x = 0 on level l: omega D^-1 r, or the
+// Chebyshev first step D^-1 r / theta
+double pre_weight(const vfem_system *S, int l, double omega)
+{
+    return cheb_on(S, l) ? cheb_coef(S->lmax[l], S->cheb_ratio, 0).b : omega;
+}
 template <typename T> const T *dinv(vfem_system *S, int l);
 template <> const double *dinv<double>(vfem_system *S, int l) { return S->dinv64[l]; }
 template <> const float *dinv<float>(vfem_system *S, int l) { return S->dinv32[l]; }
@@ -4861,15 +4966,25 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
     const int64_t n3l = 3 * L.nn;
+    const bool cheb = cheb_on(S, l);
     if (l == 0 && S->presmoothed)
         S->presmoothed = false;  // z = omega D^-1 r was written by k_cg_update_smooth
     else if (l > 0 && S->pre_level == l)
         S->pre_level = -1;  // written by the restriction that produced r_l
     else
         hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
-                           S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_INIT, S->st, S->red, -1,
-                           (const T *)nullptr);
+                           S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, pre_weight(S, l, omega), (int)SMOOTH_INIT,
+                           S->st, S->red, -1, (const T *)nullptr);
     launch_level_matvec<T>(S, l, zl, tl);
+    // Chebyshev: the remaining pre-smoothing steps (the first, from zero, wrote zl),
+    // each followed by the product the next step or the restriction needs
+    for (int k = 1; cheb && k < S->cheb_degree; k++) {
+        const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
+        hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
+                           S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b, k == 1 ? 1 : 0,
+                           k + 1 < S->cheb_degree ? 1 : 0, S->st);
+        launch_level_matvec<T>(S, l, zl, tl);
+    }
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
                                        S->l0_kind == VFEM_L0_STENCIL_PLANES ||
                                        (S->l0_kind == VFEM_L0_AUTO && S->tiled_auto[l]));
@@ -4878,7 +4993,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     // D^-1 r) when that level runs a damped cycle of its own (not the coarse solve,
     // not the learned correction), saving the separate smoothing launch.
     const bool pre = l + 2 < nl && !(S->unet_on && l + 1 == S->unet_level);
-    const PreSmooth<TN> ps{pre ? level_z<TN>(S, l + 1) : (TN *)nullptr, dinv_ref<TN>(S, l + 1), omega};
+    const PreSmooth<TN> ps{pre ? level_z<TN>(S, l + 1) : (TN *)nullptr, dinv_ref<TN>(S, l + 1),
+                           pre_weight(S, l + 1, omega)};
     if (tiled && C.div == 4) {
         hipLaunchKernelGGL((k_restrict4_tile<TR, T, TN, kRestrict4Z>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
                            S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
@@ -4937,6 +5053,16 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
             return;
         }
     }
+    if (cheb) {  // post-smoothing: all steps from the corrected x (a coarse level: no reduction)
+        for (int k = 0; k < S->cheb_degree; k++) {
+            const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
+            launch_level_matvec<T>(S, l, zl, tl);
+            hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream,
+                               n3l, S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b, 0,
+                               k + 1 < S->cheb_degree ? 1 : 0, S->st);
+        }
+        return;
+    }
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
@@ -4955,6 +5081,21 @@ struct Plan {
     bool force_flex;   // flexible CG although the cycle is fixed (after a learned phase)
     bool op64;         // reduced-precision cycle with the CG operator (and p) in FP64
     bool coarse64;     // reduced-precision cycle on level 0 only, the coarser levels in FP64
+    int cheb_level;    // damped cycle: Chebyshev smoothing on levels >= cheb_level (0: damped Jacobi)
+    int cheb_degree;   // its steps before and after the coarse correction
+    double cheb_ratio; // its interval [lmax / ratio, lmax]
+};
+
+// While alive: the damped cycle smooths with the plan's Chebyshev settings.
+struct ChebMode {
+    vfem_system *S;
+    ChebMode(vfem_system *s, const Plan &P) : S(s)
+    {
+        S->cheb_level = P.cycle == VFEM_CYCLE_DAMPED ? P.cheb_level : 0;
+        S->cheb_degree = P.cheb_degree;
+        S->cheb_ratio = P.cheb_ratio;
+    }
+    ~ChebMode() { S->cheb_level = 0; }
 };
 
 // While alive: level-0 FP64 matvecs of the plan go to the matrix cores (VFEM_MIXED).
@@ -5006,6 +5147,7 @@ void enqueue_rzx(vfem_system *S, const Plan &P)
 void enqueue_precond(vfem_system *S, const Plan &P, const double *r, int fin)
 {
     LearnedMode learned(S, P);
+    ChebMode cheb(S, P);
     if (S->H.nlev == 1) {
         // single level: the cycle is the exact coarse solve, z = Kc^-1 r (FP64), then r.z
         launch_coarse_solve<double>(S, r, S->z64, P.cycle == VFEM_CYCLE_DAMPED);
@@ -5168,6 +5310,18 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     }
     P.cycle = o.cycle;
     P.omega = o.omega;
+    P.cheb_level = o.cheb_level;
+    P.cheb_degree = o.cheb_degree ? o.cheb_degree : 2;
+    P.cheb_ratio = o.cheb_ratio != 0 ? o.cheb_ratio : 30.0;
+    if (P.cheb_level < 0 || P.cheb_degree < 1 || P.cheb_degree > 8 || !(P.cheb_ratio > 1)) {
+        g_last_error = "cheb_level must be >= 0, cheb_degree 1..8 and cheb_ratio > 1";
+        return VFEM_ERR_ARGUMENT;
+    }
+    if (P.cheb_level > 0 && P.cycle != VFEM_CYCLE_DAMPED) {
+        g_last_error = "Chebyshev smoothing needs the damped cycle";
+        return VFEM_ERR_UNSUPPORTED;
+    }
+    if (P.cheb_level > 15) P.cheb_level = 15;  // past the coarsest level: no level smooths with it
     P.op_passes = o.op_passes ? o.op_passes : 6;
     P.cycle_passes = o.cycle_passes ? o.cycle_passes : (P.bf16 ? 1 : 6);
     if ((P.op_passes != 3 && P.op_passes != 6) ||
@@ -5188,10 +5342,14 @@ int get_iteration_graph(vfem_system *S, const Plan &P, bool with_true, hipGraphE
     key += (int64_t)(P.force_flex ? 1 : 0) << 52;
     key += (int64_t)(P.op64 ? 1 : 0) << 53;
     key += (int64_t)(P.coarse64 ? 1 : 0) << 54;
-    // omega is baked into the graph's kernel arguments: the cache key holds its bit pattern
-    uint64_t obits;
+    key += (int64_t)P.cheb_level << 55;
+    key += (int64_t)P.cheb_degree << 59;
+    // omega and the Chebyshev interval are baked into the graph's kernel
+    // arguments: the cache key holds their bit patterns
+    uint64_t obits, cbits;
     std::memcpy(&obits, &P.omega, sizeof obits);
-    const GraphKey gk{key, obits};
+    std::memcpy(&cbits, &P.cheb_ratio, sizeof cbits);
+    const GraphKey gk{key, obits, cbits};
     auto it = S->graphs.find(gk);
     if (it != S->graphs.end()) {
         *out = it->second;
@@ -5435,6 +5593,72 @@ int noalloc_scratch_ready()
 
 extern "C" {
 
+// FP64 reduction of a . b over one level's 3 nl DOFs into the host (setup only)
+int level_dot(vfem_system *S, int64_t n3, const double *a, const double *b, double *out)
+{
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, S->stream, n3, a, b, S->st,
+                       S->red, (int)FIN_STORE, 0);
+    enqueue_finalize(S, S->stream, grid_for_flat(n3), 1, FIN_STORE, 0);
+    CGState h{};
+    if (int rc = read_state(S, h)) return rc;
+    *out = h.result;
+    return 0;
+}
+
+// Bound of lambda_max(D^-1 A) on every level between 1 and the coarsest (the
+// interval end of the Chebyshev smoother): kLmaxIters power steps v <- D^-1 A v
+// from a fixed pseudo-random start (fixed DOFs zero), in FP64, the estimate
+// |v_k+1| / |v_k| times kLmaxSafety (a power estimate approaches lambda_max from
+// below).  Deterministic for a given system.  Level scratch vectors r, z, t hold
+// the iterates; no plan is active.
+constexpr int kLmaxIters = 30;
+constexpr double kLmaxSafety = 1.1;
+int estimate_lmax(vfem_system *S)
+{
+    const int nl = S->H.nlev;
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1);
+    for (int l = 1; l + 1 < nl; l++) {
+        const int64_t n3 = 3 * S->H.L[l].nn;
+        const int gf = grid_for_flat(n3);
+        double *v = S->rl64[l], *t = S->tl64[l], *w = S->zl64[l];
+        hipLaunchKernelGGL(k_fill_hash, dim3(2048), dim3(kBlock), 0, S->stream, n3, S->tl32[l], 2024u + (uint32_t)l);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->tl32[l], w);
+        hipLaunchKernelGGL((k_smooth<double, double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->fixc[l],
+                           (const double *)w, (const double *)nullptr, dinv_ref<double>(S, l), v, 1.0, (int)SMOOTH_INIT,
+                           S->st, S->red, -1, (const double *)nullptr);  // v = D^-1 (start), fixed rows zero
+        HIP_TRY(hipGetLastError());
+        double vv = 0, ww = 0;
+        for (int k = 0; k < kLmaxIters; k++) {
+            launch_level_matvec<double>(S, l, v, t);
+            hipLaunchKernelGGL((k_smooth<double, double, double>), dim3(gf), dim3(kBlock), 0, S->stream, n3, S->fixc[l],
+                               (const double *)t, (const double *)nullptr, dinv_ref<double>(S, l), w, 1.0,
+                               (int)SMOOTH_INIT, S->st, S->red, -1, (const double *)nullptr);  // w = D^-1 A v
+            HIP_TRY(hipGetLastError());
+            if (k + 1 < kLmaxIters) {
+                std::swap(v, w);
+                continue;
+            }
+            if (int rc = level_dot(S, n3, v, v, &vv)) return rc;
+            if (int rc = level_dot(S, n3, w, w, &ww)) return rc;
+        }
+        S->lmax[l] = vv > 0 ? kLmaxSafety * std::sqrt(ww / vv) : 0.0;
+        if (!(S->lmax[l] > 0) || !std::isfinite(S->lmax[l])) S->lmax[l] = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+int vfem_level_lmax(const vfem_system *sys, int32_t level, double *out)
+{
+    if (!sys || !out || level < 0 || level >= sys->H.nlev) {
+        g_last_error = "invalid argument";
+        return VFEM_ERR_ARGUMENT;
+    }
+    *out = sys->lmax[level];
+    return VFEM_OK;
+}
+
 void vfem_default_options(vfem_options *o)
 {
     if (!o) return;
@@ -5456,6 +5680,9 @@ void vfem_default_options(vfem_options *o)
     o->stall_ratio = 0.9;
     o->op_fp64 = 0;
     o->coarse_fp64 = 0;
+    o->cheb_level = 0;
+    o->cheb_degree = 2;
+    o->cheb_ratio = 30.0;
 }
 
 const char *vfem_status_string(int status)
@@ -5501,6 +5728,7 @@ int vfem_assemble(const int32_t *voxels, int32_t nx, int32_t ny, int32_t nz, con
         return rc;
     }
     rc = upload_system(S, DH);
+    if (!rc) rc = estimate_lmax(S);
     if (rc) {
         delete S;
         return rc;
