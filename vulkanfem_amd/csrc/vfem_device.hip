@@ -2186,6 +2186,39 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
 // they touch, and the four partial products meet in LDS in a fixed order.  The
 // levels this kernel serves are small and latency-bound: four short dependency
 // chains per segment instead of one long one.
+template <typename Td> struct DinvRef {
+    const Td *d;
+    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
+    Td c[3];
+    const Td *cd;          // the same triple in device memory
+};
+// D^-1 at compact DOF k of node nd = k / 3 (flag: the node holds the shared
+// triple).  The triple is read from device memory (cd), so the choice between
+// cd[k - 3 nd] and d[k] is one global load from a chosen address; with the
+// triple as a kernel argument the chosen pointer was a generic one and the load
+// a flat load that waited at once.
+template <typename Td>
+__device__ __forceinline__ Td dinv_at(const DinvRef<Td> &D, bool flag, uint32_t k, uint32_t nd)
+{
+    return flag ? D.cd[k - 3u * nd] : D.d[k];
+}
+
+// One Chebyshev smoothing step fused into the level product that feeds it (the
+// epilogue of k_matvec_coarse): with t = (A x)_i at the node,
+// d = a d_prev + b D^-1 (r - t) (d_prev = x when first), x_out = x + d, d written
+// when dout is set.  x_out is another buffer than the product's input x, which
+// other blocks still read; d is per node, so it is updated in place.  Fixed
+// rows: d = 0.  Inactive when xout is null (the plain product y = A x).
+template <typename T> struct ChebEpi {
+    const T *r;
+    DinvRef<T> D;
+    const T *dprev;
+    T *dout;
+    T *xout;
+    double a, b;
+    int first;
+};
+
 template <typename T, int B, int C>
 __device__ __forceinline__ void coarse_pair(const DevLevel &L, const T *__restrict__ tab, const T *__restrict__ x,
                                             int s, int yy, int zz, int lane, T (&acc)[3])
@@ -2256,12 +2289,18 @@ __device__ __forceinline__ void coarse_pair(const DevLevel &L, const T *__restri
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *__restrict__ tab,
                                                            const T *__restrict__ x, T *__restrict__ y,
-                                                           const CGState *st)
+                                                           const CGState *st, ChebEpi<T> ep)
 {
     {
         const int64_t co = col_offset(3 * L.nn);
         x += co;
-        y += co;
+        if (y) y += co;
+        if (ep.xout) {  // D^-1 is shared by all columns
+            ep.r += co;
+            if (ep.dprev) ep.dprev += co;
+            if (ep.dout) ep.dout += co;
+            ep.xout += co;
+        }
         st = col_state(st);
     }
     if (st->done) return;
@@ -2291,10 +2330,27 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
 #pragma unroll
             for (int c = 0; c < 3; c++) r[c] = ((part[0][c][lane] + part[1][c][lane]) + part[2][c][lane]) + part[3][c][lane];
             const int id = L.nbase[it] + __popcll(wself & ((1ull << lane) - 1));
-            if ((L.fbits[it] >> lane) & 1) r[0] = r[1] = r[2] = 0;
-            y[3 * id + 0] = r[0];
-            y[3 * id + 1] = r[1];
-            y[3 * id + 2] = r[2];
+            const bool fx = (L.fbits[it] >> lane) & 1;
+            if (fx) r[0] = r[1] = r[2] = 0;
+            if (ep.xout) {
+                const uint32_t nd = (uint32_t)id;
+                const bool flat = ep.D.flat && ((ep.D.flat[nd >> 5] >> (nd & 31)) & 1);
+                const T ta = T(ep.a), tb = T(ep.b);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const uint32_t k = 3u * nd + c;
+                    const T xv = x[k];
+                    const T dp = ep.first ? xv : (ep.a != 0 ? ep.dprev[k] : T(0));
+                    T dn = ta * dp + tb * dinv_at(ep.D, flat, k, nd) * (ep.r[k] - r[c]);
+                    if (fx) dn = 0;
+                    ep.xout[k] = xv + dn;
+                    if (ep.dout) ep.dout[k] = dn;
+                }
+            } else {
+                y[3 * id + 0] = r[0];
+                y[3 * id + 1] = r[1];
+                y[3 * id + 2] = r[2];
+            }
         }
         __syncthreads();
     }
@@ -2303,23 +2359,6 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
 // D^-1 of a level as the flat kernels read it: the stored array, except at the
 // nodes flagged in `flat`, which hold the level's common value c (the interior:
 // no load there, saving 4 or 8 bytes per DOF on most of the grid).
-template <typename Td> struct DinvRef {
-    const Td *d;
-    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
-    Td c[3];
-    const Td *cd;          // the same triple in device memory
-};
-// D^-1 at compact DOF k of node nd = k / 3 (flag: the node holds the shared
-// triple).  The triple is read from device memory (cd), so the choice between
-// cd[k - 3 nd] and d[k] is one global load from a chosen address; with the
-// triple as a kernel argument the chosen pointer was a generic one and the load
-// a flat load that waited at once.
-template <typename Td>
-__device__ __forceinline__ Td dinv_at(const DinvRef<Td> &D, bool flag, uint32_t k, uint32_t nd)
-{
-    return flag ? D.cd[k - 3u * nd] : D.d[k];
-}
-
 // The damped cycle's pre-smoothing of the next coarser level, z_c = omega D_c^-1 r_c,
 // written by the restriction that produces r_c (fixed rows: r_c is 0 there, so
 // z_c is too).  z null: none.
@@ -2511,7 +2550,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict2(DevLevel F, DevLevel C, co
 // Fine-node kernel of the upward sweep.
 //   mode PROLONG_SMOOTH (reference additive cycle, Interpolate + PreSmoothAndAXPY + Smooth):
 //        z_f = P z_c + 2 D^-1 r_f
-//   mode PROLONG_ADD (damped cycle):  z_f += P z_c
+//   mode PROLONG_ADD (damped cycle):  z_f = z_in + P z_c (z_in null: z_f)
 // maskfixed zeroes fixed rows; fin >= 0 reduces r.z.
 enum { PROLONG_SMOOTH = 0, PROLONG_ADD = 1 };
 
@@ -2519,13 +2558,15 @@ template <typename Tc, typename Tr, typename Td, typename Tz>
 __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, const Tc *__restrict__ zc,
                                                      const Tr *__restrict__ rf, const Td *__restrict__ dinv,
                                                      Tz *__restrict__ zf, int mode, int maskfixed, CGState *st,
-                                                     Reduction red, int fin, const Tz *__restrict__ zold)
+                                                     Reduction red, int fin, const Tz *__restrict__ zold,
+                                                     const Tz *__restrict__ zin)
 {
     {
         const int64_t cf = col_offset(3 * F.nn);
         zc += col_offset(3 * C.nn);
         rf += cf;
         zf += cf;
+        if (zin) zin += cf;
         if (zold) zold += cf;
         st = col_state(st);
         red = col_red(red);
@@ -2600,7 +2641,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(DevLevel F, DevLevel C, cons
                 const Tz dr = Tz(dinv[3 * id + c]) * Tz(rf[3 * id + c]);
                 v = (o[c] + dr) + dr;
             } else {
-                v = zf[3 * id + c] + o[c];
+                v = (zin ? zin[3 * id + c] : zf[3 * id + c]) + o[c];
             }
             if (fx) v = 0;
             zf[3 * id + c] = v;
@@ -4895,7 +4936,22 @@ template <typename T> void launch_level_matvec(vfem_system *S, int l, const T *x
     const DevLevel &L = S->dl[l];
     const int segs = (int)std::min<int64_t>((int64_t)L.nz * L.ny * L.S, 8192);  // one row segment per block
     hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x, y,
-                       S->st);
+                       S->st, ChebEpi<T>{});
+}
+
+// a level served by k_matvec_coarse (element form, row segments) takes its
+// Chebyshev steps fused into the product
+bool cheb_fused(const vfem_system *S, int l) { return l > 0 && !level_stencil_on(S, l); }
+
+// one Chebyshev step of level l fused into its product: xout = x + d (ChebEpi)
+template <typename T>
+void launch_coarse_cheb(vfem_system *S, int l, const T *x, T *xout, const T *r, const T *dprev, T *dout, double a,
+                        double b, int first)
+{
+    const DevLevel &L = S->dl[l];
+    const int segs = (int)std::min<int64_t>((int64_t)L.nz * L.ny * L.S, 8192);
+    hipLaunchKernelGGL(k_matvec_coarse<T>, dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, ktab<T>(S, l), x,
+                       (T *)nullptr, S->st, ChebEpi<T>{r, dinv_ref<T>(S, l), dprev, dout, xout, a, b, first});
 }
 
 // The explicit inverse is always applied in FP64: its entries carry the coarse
@@ -4931,12 +4987,13 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
         const DevLevel &F = S->dl[l], &C = S->dl[l + 1];
         hipLaunchKernelGGL((k_prolong<T, T, T, T>), dim3(items_grid(F), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                            (const T *)level_z<T>(S, l + 1), (const T *)level_r<T>(S, l), dinv<T>(S, l),
-                           level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1, (const T *)nullptr);
+                           level_z<T>(S, l), (int)PROLONG_SMOOTH, 0, S->st, S->red, -1, (const T *)nullptr,
+                           (const T *)nullptr);
     }
     const DevLevel &F = S->dl[0], &C = S->dl[1];
     hipLaunchKernelGGL((k_prolong<T, double, T, T>), dim3(items_grid(F), S->ncols), dim3(kBlock), 0, S->stream, F, C,
                        (const T *)level_z<T>(S, 1), r, dinv<T>(S, 0), z, (int)PROLONG_SMOOTH, 1, S->st, S->red, fin,
-                       zold);
+                       zold, (const T *)nullptr);
     enqueue_finalize(S, S->stream, items_grid(F), S->ncols, fin, 1);
 }
 
@@ -4975,15 +5032,36 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
                            S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, pre_weight(S, l, omega), (int)SMOOTH_INIT,
                            S->st, S->red, -1, (const T *)nullptr);
-    launch_level_matvec<T>(S, l, zl, tl);
     // Chebyshev: the remaining pre-smoothing steps (the first, from zero, wrote zl),
-    // each followed by the product the next step or the restriction needs
-    for (int k = 1; cheb && k < S->cheb_degree; k++) {
-        const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
-        hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
-                           S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b, k == 1 ? 1 : 0,
-                           k + 1 < S->cheb_degree ? 1 : 0, S->st);
+    // each after the product it needs, then the product the restriction needs.
+    // On an element-form coarse level each step is fused into its product, and x
+    // alternates between zl and tl (the product's input is read by other blocks):
+    // xcur holds the smoothed x, tcur the restriction's A x.
+    T *xcur = zl, *tcur = tl;
+    bool fz = false;
+    if constexpr (std::is_same<T, TR>::value) fz = cheb && cheb_fused(S, l);
+    if (fz) {
+        if constexpr (std::is_same<T, TR>::value) {
+            T *xa = zl, *xb = tl;
+            for (int k = 1; k < S->cheb_degree; k++) {
+                const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
+                launch_coarse_cheb<T>(S, l, xa, xb, rl, k == 1 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
+                                      k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, k == 1 ? 1 : 0);
+                std::swap(xa, xb);
+            }
+            launch_level_matvec<T>(S, l, xa, xb);
+            xcur = xa;
+            tcur = xb;
+        }
+    } else {
         launch_level_matvec<T>(S, l, zl, tl);
+        for (int k = 1; cheb && k < S->cheb_degree; k++) {
+            const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
+            hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream,
+                               n3l, S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b,
+                               k == 1 ? 1 : 0, k + 1 < S->cheb_degree ? 1 : 0, S->st);
+            launch_level_matvec<T>(S, l, zl, tl);
+        }
     }
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
                                        S->l0_kind == VFEM_L0_STENCIL_PLANES ||
@@ -4997,23 +5075,23 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                            pre_weight(S, l + 1, omega)};
     if (tiled && C.div == 4) {
         hipLaunchKernelGGL((k_restrict4_tile<TR, T, TN, kRestrict4Z>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+                           S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     } else if (tiled) {
         if (S->short_z[l])
             hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZShort>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock),
-                               0, S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+                               0, S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
         else
             hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, rl, (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+                               S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     } else if (C.div == 2) {
         // one coarse row segment per block
         const int segs = (int)std::min<int64_t>((int64_t)C.nz * C.ny * C.S, 8192);
         hipLaunchKernelGGL((k_restrict2<TR, T, TN>), dim3(segs, S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
-                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
+                           (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, ps);
     } else {
         // the skip level (div 4, RestrictDouble.comp): the general row-segment gather
         hipLaunchKernelGGL((k_restrict<TR, T, TN>), dim3(items_grid(C), S->ncols), dim3(kBlock), 0, S->stream, L, C, rl,
-                           (const T *)tl, level_r<TN>(S, l + 1), 1, S->st, ps);
+                           (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, ps);
     }
     if (pre) S->pre_level = l + 1;
     enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
@@ -5031,27 +5109,42 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_rows(S, S->stl[0], false);
     if (tiled && C.div == 4) {
         hipLaunchKernelGGL((k_prolong4_tile<TN, T, kProlong4Z>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
+                           S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1, S->st,
                            S->ptiles[l]);
     } else if (tiled) {
         if (S->short_z[l])
             hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZShort>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
-                               S->ptiles[l]);
+                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1,
+                               S->st, S->ptiles[l]);
         else
             hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)zl, fuse ? tl : zl, 1, S->st,
-                               S->ptiles[l]);
+                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1,
+                               S->st, S->ptiles[l]);
     } else {
         hipLaunchKernelGGL((k_prolong<TN, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
                            (const TN *)level_z<TN>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
-                           -1, (const T *)nullptr);
+                           -1, (const T *)nullptr, xcur != zl ? (const T *)xcur : (const T *)nullptr);
     }
     if constexpr (std::is_same<TR, double>::value) {
         if (fuse) {
             launch_stencil_smooth<T>(S, tl, zl, (const double *)rl, zold, omega, fin);
             return;
         }
+    }
+    if (fz) {  // post-smoothing fused into the products, x alternating between zl and tl
+        if constexpr (std::is_same<T, TR>::value) {
+            T *xa = zl, *xb = tl;
+            for (int k = 0; k < S->cheb_degree; k++) {
+                const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
+                launch_coarse_cheb<T>(S, l, xa, xb, rl, k == 0 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
+                                      k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, 0);
+                std::swap(xa, xb);
+            }
+            if (xa != zl)  // an odd number of steps ends in tl
+                hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
+                                   (const T *)xa, zl);
+        }
+        return;
     }
     if (cheb) {  // post-smoothing: all steps from the corrected x (a coarse level: no reduction)
         for (int k = 0; k < S->cheb_degree; k++) {
