@@ -96,10 +96,10 @@ def test_stencil_form_matches_element_form(systems, name):
 
 
 def test_large_level_kernels_match_element_form(ke0):
-    """A level 0 of at least 4M nodes, as at 256^3, runs the 8-plane stencil
-    kernels and the long transfer chunks (smaller levels run 4 planes and short
-    chunks): its operator and damped cycle against the element form on the same
-    system, which the smaller cases pin to the oracle."""
+    """A level 0 of at least 4M nodes, as at 256^3, runs the 12-plane stencil
+    kernels (smaller levels run 4 planes): its operator and damped cycle against
+    the element form on the same system, which the smaller cases pin to the
+    oracle."""
     p = problems.cantilever_box(160, 160, 160)
     A = V.assemble_system_matrix(p.voxels, ke0, p.fixed, 6)
     assert A.n // 3 >= 4_000_000

@@ -149,8 +149,8 @@ def ell256(ke0):
 
 
 def test_config3_operator_and_cycle_match_oracle(ell256, ke0):
-    """The 256^3 rough geometry (10.9M nodes: the 12-plane stencil chunks and the
-    long transfer chunks) against the oracle: the level-0 operator in stencil and
+    """The 256^3 rough geometry (10.9M nodes: the 12-plane stencil chunks) against
+    the oracle: the level-0 operator in stencil and
     element form (FP64 to 1e-14, FP32 to 2e-6) and one damped V(1,1) cycle in
     both forms (FP64 to 1e-12)."""
     p, A, nl = ell256

@@ -3519,12 +3519,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int64_t n, double *__restr
 // are not touched here.  With `rzx`, the z it overwrites (z_k, the last
 // preconditioned residual) is first dotted with the new residual: the
 // r_{k+1}.z_k of the flexible beta, so no copy of z_k has to be kept.
-template <typename Ta, typename Tz>
+template <typename Ta, typename Tz, bool RZX>
 __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *__restrict__ r,
                                                               const Ta *__restrict__ ap,
                                                               const uint32_t *__restrict__ fixc,
                                                               DinvRef<Tz> D, Tz *__restrict__ z,
-                                                              double omega, CGState *st, Reduction red, int rzx)
+                                                              double omega, CGState *st, Reduction red)
 {
     {
         const int64_t co = col_offset(n);
@@ -3538,6 +3538,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
     const double alpha = st->alpha;
     const Tz om = Tz(omega);
     const Range R = flat_range(n);
+    const uint32_t *flat = D.flat ? D.flat : fixc, flat_keep = D.flat ? ~0u : 0u;
     double acc = 0, acc2 = 0;
     for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
         double rv[FLAT_U];
@@ -3548,14 +3549,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
         for (int u = 0; u < FLAT_U; u++) {  // mask words first: the D^-1 loads wait on them only
             const uint32_t nd = (uint32_t)min(i0 + u * kBlock, R.end - 1) / 3u;
             fw[u] = fixc[nd >> 5];
-            cw[u] = D.flat ? D.flat[nd >> 5] : 0u;
+            cw[u] = flat[nd >> 5] & flat_keep;
         }
+        // every load is unconditional: a load under a branch ends in a wait for all
+        // loads in flight where the branch joins
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
             rv[u] = r[i];
             av[u] = ap[i];
-            zo[u] = rzx ? z[i] : Tz(0);
+            if constexpr (RZX) zo[u] = z[i];
+            else zo[u] = Tz(0);
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -3571,7 +3575,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
             acc += rn * rn;
             const bool fx = (fw[u] >> (((uint32_t)i / 3u) & 31)) & 1;
             z[i] = fx ? Tz(0) : om * dv[u] * Tz(rn);
-            acc2 += rn * (double)zo[u];
+            if constexpr (RZX) acc2 += rn * (double)zo[u];
         }
     }
     block_partials2(acc, acc2, red);
@@ -4000,12 +4004,12 @@ BlockGrid block_grid(const DevLevel &L, const BlockCfg &c)
 
 constexpr int64_t kStencilMinNodes = 100000;  // smallest coarse level given the stencil form
 constexpr int64_t kTiledMinNodes = 500000;     // smallest fine level given the tiled transfers
-constexpr int kRestrictZ = 8;                  // coarse planes per k_restrict_tile block
-constexpr int kProlongZ = 16;                  // fine planes per k_prolong_tile block
-// transfers from fine levels below kShortZNodes nodes use 4x shorter chunks: their
-// grids are otherwise too small to fill the GPU (level 1 at 256^3: 176 blocks)
-constexpr int kRestrictZShort = 2, kProlongZShort = 4;
-constexpr int64_t kShortZNodes = 4000000;
+// planes per block of the tiled transfers: short chunks give every level enough
+// blocks (level 1 at 256^3: 704 restriction blocks) and, measured at 256^3, were
+// no slower on level 0 than 4x longer ones (mixed solve 58.8 against 59.4 ms with
+// 8 coarse / 16 fine planes)
+constexpr int kRestrictZ = 2;                  // coarse planes per k_restrict_tile block
+constexpr int kProlongZ = 4;                   // fine planes per k_prolong_tile block
 
 // Stencil form of one level's operator (see k_matvec_stencil).
 struct LevelStencil {
@@ -4099,7 +4103,6 @@ struct vfem_system {
     // tiled transfers between level l and l + 1 (div 2, large fine level)
     bool tiled[kMaxLevels] = {}, tiled_auto[kMaxLevels] = {};
     StencilTiles rtiles[kMaxLevels] = {}, ptiles[kMaxLevels] = {};
-    bool short_z[kMaxLevels] = {};  // the tiles of level l use kRestrictZShort / kProlongZShort
     bool presmoothed = false;   // the level-0 pre-smoothing was fused into the CG update just enqueued
     bool lazy_x = false;        // that update left x += alpha p to the p update
     int pre_level = -1;         // coarse level whose pre-smoothing the restriction just enqueued wrote
@@ -4484,20 +4487,17 @@ int build_stencil(vfem_system *S)
                 continue;
             if (int rc = upload_tiles(S, l + 1, R4_T, R4_T, kRestrict4Z, S->rtiles[l])) return rc;
             if (int rc = upload_tiles(S, l, 32, 32, kProlong4Z, S->ptiles[l])) return rc;
-            S->short_z[l] = false;
             S->tiled[l] = true;
             S->tiled_auto[l] = F.nn >= kTiledMinNodes;
             continue;
         }
         if (C.div != 2) continue;
-        const bool sz = F.nn < kShortZNodes;
-        const int rz = sz ? kRestrictZShort : kRestrictZ, pz = sz ? kProlongZShort : kProlongZ;
+        const int rz = kRestrictZ, pz = kProlongZ;
         if ((C.nx + RT_X - 1) / RT_X > 2047 || (C.ny + RT_Y - 1) / RT_Y > 2047 || (C.nz + rz - 1) / rz > 1023 ||
             (F.nz + pz - 1) / pz > 1023)
             continue;
         if (int rc = upload_tiles(S, l + 1, RT_X, RT_Y, rz, S->rtiles[l])) return rc;
         if (int rc = upload_tiles(S, l, 2 * RT_X, 2 * RT_Y, pz, S->ptiles[l])) return rc;
-        S->short_z[l] = sz;
         S->tiled[l] = true;
         S->tiled_auto[l] = F.nn >= kTiledMinNodes;  // small levels: the row-segment kernels are faster
     }
@@ -5077,12 +5077,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         hipLaunchKernelGGL((k_restrict4_tile<TR, T, TN, kRestrict4Z>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
                            S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     } else if (tiled) {
-        if (S->short_z[l])
-            hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZShort>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock),
-                               0, S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
-        else
-            hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
+        hipLaunchKernelGGL((k_restrict_tile<TR, T, TN, kRestrictZ>), dim3(S->rtiles[l].n, S->ncols), dim3(kBlock), 0,
+                           S->stream, L, C, rl, (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, S->rtiles[l], ps);
     } else if (C.div == 2) {
         // one coarse row segment per block
         const int segs = (int)std::min<int64_t>((int64_t)C.nz * C.ny * C.S, 8192);
@@ -5112,14 +5108,9 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                            S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1, S->st,
                            S->ptiles[l]);
     } else if (tiled) {
-        if (S->short_z[l])
-            hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZShort>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1,
-                               S->st, S->ptiles[l]);
-        else
-            hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                               S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1,
-                               S->st, S->ptiles[l]);
+        hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream,
+                           L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1, S->st,
+                           S->ptiles[l]);
     } else {
         hipLaunchKernelGGL((k_prolong<TN, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
                            (const TN *)level_z<TN>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
@@ -5330,6 +5321,21 @@ void enqueue_x_flush(vfem_system *S, const Plan &P, bool clear = true)
     if (clear) enqueue_x_lag_clear(S);
 }
 
+// residual update fused with the level-0 pre-smoothing (k_cg_update_smooth);
+// rzx: also reduce r_k+1 . z_k from the z it overwrites (flexible beta)
+template <typename Ta, typename Tz>
+void launch_update_smooth(vfem_system *S, const Ta *ap, DinvRef<Tz> D, Tz *z, double omega, int rzx)
+{
+    const int64_t n3 = S->n3;
+    const dim3 grid(grid_for_flat(n3), S->ncols);
+    if (rzx)
+        hipLaunchKernelGGL((k_cg_update_smooth<Ta, Tz, true>), grid, dim3(kBlock), 0, S->stream, n3, S->r, ap, S->fixc[0],
+                           D, z, omega, S->st, S->red);
+    else
+        hipLaunchKernelGGL((k_cg_update_smooth<Ta, Tz, false>), grid, dim3(kBlock), 0, S->stream, n3, S->r, ap,
+                           S->fixc[0], D, z, omega, S->st, S->red);
+}
+
 // one CG iteration (FEMGPU.cpp:471-541 / Eigen ConjugateGradient loop body).
 // with_true: residual replacement -- after the x/r update, r = f - A x in FP64,
 // and that true residual decides convergence (mixed precision).
@@ -5348,11 +5354,9 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
         L0Mode mode(S, P);
         launch_matvec_fine<double>(S, S->p64, S->ap64, FIN_PAP, 1, P.op_passes, P.op64);
         if (fuse && P.precision == VFEM_FP64)
-            hipLaunchKernelGGL((k_cg_update_smooth<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->r, S->ap64, S->fixc[0], dinv_ref<double>(S, 0), S->z64, om, S->st, S->red, rzx);
+            launch_update_smooth<double, double>(S, S->ap64, dinv_ref<double>(S, 0), S->z64, om, rzx);
         else if (fuse)
-            hipLaunchKernelGGL((k_cg_update_smooth<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->r, S->ap64, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red, rzx);
+            launch_update_smooth<double, float>(S, S->ap64, dinv_ref<float>(S, 0), S->z32, om, rzx);
         else
             hipLaunchKernelGGL((k_cg_update<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x,
                                S->r, S->p64, S->ap64, S->st, S->red);
@@ -5360,8 +5364,7 @@ void enqueue_iteration(vfem_system *S, const Plan &P, bool with_true)
     } else {
         launch_matvec_fine<float>(S, S->p32, S->ap32, FIN_PAP, 1, P.op_passes);
         if (fuse)
-            hipLaunchKernelGGL((k_cg_update_smooth<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3,
-                               S->r, S->ap32, S->fixc[0], dinv_ref<float>(S, 0), S->z32, om, S->st, S->red, rzx);
+            launch_update_smooth<float, float>(S, S->ap32, dinv_ref<float>(S, 0), S->z32, om, rzx);
         else
             hipLaunchKernelGGL((k_cg_update<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->x, S->r,
                                S->p32, S->ap32, S->st, S->red);
