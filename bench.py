@@ -29,7 +29,7 @@ PEAK_HBM_GBS = 8000.0
 # the bound, while 0.6 exceeds it and makes the iteration counts erratic.
 OMEGA = 0.5
 # The measured plans smooth the coarse levels >= 2 with two Chebyshev steps
-# (interval [lmax / 30, lmax] of D^-1 A) before and after the coarse correction
+# (interval [lmax / 60, lmax] of D^-1 A) before and after the coarse correction
 # instead of one damped Jacobi step: 46 -> 37 iterations at 256^3 for about 8%
 # more time per iteration (levels >= 2 are small).  Level 0 and 1 keep damped
 # Jacobi.  CHEB_LEVEL = 0 gives damped Jacobi everywhere.
@@ -176,7 +176,7 @@ def cpu_baseline(p, nlev, threads, lmax):
     ke = O.ke0_closed_form(1.0, 0.3)
     t0 = time.perf_counter()
     s = O.OracleSystem(p.voxels, ke, p.fixed, nlev)
-    s.set_chebyshev(CHEB_LEVEL, 2, 30.0, lmax)
+    s.set_chebyshev(CHEB_LEVEL, 2, 60.0, lmax)
     t_setup = time.perf_counter() - t0
     f = p.loads[0]
     t0 = time.perf_counter()

@@ -104,7 +104,7 @@ typedef struct {
                                 [lmax / cheb_ratio, lmax] of D^-1 A (vfem_level_lmax); the cycle stays
                                 symmetric positive definite.  0 = damped Jacobi on every level (default) */
     int32_t cheb_degree;     /* Chebyshev steps, 1..8; default 2 */
-    double cheb_ratio;       /* > 1; default 30 */
+    double cheb_ratio;       /* > 1; default 60 */
 } vfem_options;
 
 typedef struct {

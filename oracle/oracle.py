@@ -225,7 +225,7 @@ class OracleSystem:
         self._lib.vo_interpolate(self._h, l, xc, out)
         return out
 
-    def set_chebyshev(self, level, degree=2, ratio=30.0, lmax=None):
+    def set_chebyshev(self, level, degree=2, ratio=60.0, lmax=None):
         """Damped cycle: Chebyshev smoothing (`degree` steps before and after) on
         levels >= level, fitted to [lmax[l] / ratio, lmax[l]]; level 0 = damped
         Jacobi everywhere."""

@@ -96,7 +96,7 @@ def test_chebyshev_solve_matches_oracle(system):
     A, ora, f = system
     lmax = [A.level_lmax(l) for l in range(A.num_levels)]
     for level in (1, 2):
-        ora.set_chebyshev(level, 2, 30.0, lmax)
+        ora.set_chebyshev(level, 2, 60.0, lmax)
         try:
             u_ref, it_ref, _ = ora.pcg(f, tol=1e-8, precond=O.DAMPED, omega=0.5)
             u_tight, _, _ = ora.pcg(f, tol=1e-13, precond=O.DAMPED, omega=0.5)

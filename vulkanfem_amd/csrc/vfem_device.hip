@@ -4071,7 +4071,7 @@ struct vfem_system {
     float *dl32[kMaxLevels] = {};
     double lmax[kMaxLevels] = {};
     int cheb_level = 0, cheb_degree = 2;
-    double cheb_ratio = 30.0;
+    double cheb_ratio = 60.0;
     float *t0_32 = nullptr;
     CGState *st = nullptr;
     Reduction red{};
@@ -5408,7 +5408,7 @@ int check_plan(vfem_system *S, const vfem_options &o, Plan &P)
     P.omega = o.omega;
     P.cheb_level = o.cheb_level;
     P.cheb_degree = o.cheb_degree ? o.cheb_degree : 2;
-    P.cheb_ratio = o.cheb_ratio != 0 ? o.cheb_ratio : 30.0;
+    P.cheb_ratio = o.cheb_ratio != 0 ? o.cheb_ratio : 60.0;
     if (P.cheb_level < 0 || P.cheb_degree < 1 || P.cheb_degree > 8 || !(P.cheb_ratio > 1)) {
         g_last_error = "cheb_level must be >= 0, cheb_degree 1..8 and cheb_ratio > 1";
         return VFEM_ERR_ARGUMENT;
@@ -5778,7 +5778,7 @@ void vfem_default_options(vfem_options *o)
     o->coarse_fp64 = 0;
     o->cheb_level = 0;
     o->cheb_degree = 2;
-    o->cheb_ratio = 30.0;
+    o->cheb_ratio = 60.0;
 }
 
 const char *vfem_status_string(int status)
