@@ -127,3 +127,64 @@ def test_chebyshev_options_checked(system):
     for bad in (dict(cheb_degree=9), dict(cheb_ratio=0.5), dict(cheb_level=-1)):
         with pytest.raises(V.VfemError):
             A.solve(f, **{"cycle": V.CYCLE_DAMPED, "cheb_level": 2, **bad})
+
+
+def test_chebyshev_fused_and_separate_steps_agree(system):
+    """The Chebyshev steps run fused into the element-form product (L0_AUTO on
+    these small coarse levels) or as separate k_cheb_step launches after the
+    stencil-form product (L0_STENCIL): both against the oracle's twin, for an
+    even and an odd number of steps (an odd count ends in the level's t buffer)."""
+    A, ora, _ = system
+    lmax = [A.level_lmax(l) for l in range(A.num_levels)]
+    r = np.random.default_rng(23).standard_normal(A.n)
+    r[np.repeat(ora.fixed(0), 3)] = 0
+    for degree in (2, 3):
+        ora.set_chebyshev(1, degree, 60.0, lmax)
+        try:
+            z_ref = ora.vcycle(r, O.DAMPED, 0.5)
+        finally:
+            ora.set_chebyshev(0)
+        out = {}
+        try:
+            for kind in (V.L0_AUTO, V.L0_STENCIL):
+                A.set_level0_kernel(kind)
+                forms = [A.level_operator_form(l) for l in range(1, A.num_levels - 1)]
+                out[kind] = V.MultigridPreconditioner(A, cycle=V.CYCLE_DAMPED, omega=0.5, cheb_level=1,
+                                                      cheb_degree=degree).solve(r)
+                say("degree %d, form %d (coarse forms %s): %.2e" % (degree, kind, forms, rel(out[kind], z_ref)))
+                assert rel(out[kind], z_ref) < 1e-12
+        finally:
+            A.set_level0_kernel(V.L0_AUTO)
+        assert rel(out[V.L0_AUTO], out[V.L0_STENCIL]) < 1e-12
+
+
+def test_chebyshev_with_skip_level_and_fp64_parts(ke0):
+    """Chebyshev smoothing on the x4 skip level (element-table operator) and the
+    levels below it matches the oracle's cycle; the FP32 cycle with FP64 coarse
+    levels or an FP64 CG operator reaches the oracle's displacement."""
+    vox, _ = problems.ellipsoid_union(32, 10, 3.0, 10.0, 5)
+    ids = problems.compact_ids(vox)
+    fixed = np.sort(ids[0][ids[0] >= 0])
+    A = V.assemble_system_matrix(vox, ke0, fixed, 4, skip_levels=1)
+    ora = O.OracleSystem(vox, ke0, fixed, 4, skip=1)
+    lmax = [A.level_lmax(l) for l in range(A.num_levels)]
+    assert A.num_levels == 4 and all(lm > 0 for lm in lmax[1:3])
+    ora.set_chebyshev(1, 2, 60.0, lmax)
+    r = np.random.default_rng(24).standard_normal(A.n)
+    r[np.repeat(ora.fixed(0), 3)] = 0
+    z_ref = ora.vcycle(r, O.DAMPED, 0.5)
+    f = np.zeros(A.n)
+    f[2::3] = -1.0 / (A.n // 3)
+    f[np.repeat(ora.fixed(0), 3)] = 0
+    u_ref, it_ref, _ = ora.pcg(f, tol=1e-12, precond=O.DAMPED, omega=0.5)
+    ora.close()
+    kw = dict(cycle=V.CYCLE_DAMPED, omega=0.5, cheb_level=1)
+    z = V.MultigridPreconditioner(A, **kw).solve(r)
+    say("skip level with Chebyshev from level 1: cycle %.2e" % rel(z, z_ref))
+    assert rel(z, z_ref) < 1e-12
+    for extra in (dict(coarse_fp64=1), dict(op_fp64=1)):
+        u, st = A.solve(f, tol=1e-10, precision=V.FP32, **kw, **extra)
+        say("skip level, Chebyshev, FP32 cycle", extra, st["iterations"], "iterations, err %.2e" % rel(u, u_ref))
+        assert st["converged"] == 1 and st["rel_residual"] < 1e-10
+        assert rel(u, u_ref) < 1e-6
+    A.close()
