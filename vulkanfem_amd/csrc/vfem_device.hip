@@ -4450,7 +4450,9 @@ int build_level_stencil(vfem_system *S, int l)
     HIP_TRY(hipStreamSynchronize(S->stream));
     LS.surf.n = nsurf;
     LS.dcon_interior = hbad == 0;
-    LS.fix_grid = (int)std::min<int64_t>(512, (LS.surf.n + kBlock - 1) / kBlock);
+    // one surface node per thread: the pass is bound by its per-lane gathers, and
+    // more waves in flight hide them better than a grid-stride loop over fewer
+    LS.fix_grid = (int)std::min<int64_t>(kMaxPartials, (LS.surf.n + kBlock - 1) / kBlock);
     int4 *ds;
     int32_t *drb;
     uint2 *dp;
