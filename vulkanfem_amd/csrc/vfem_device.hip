@@ -1331,19 +1331,23 @@ __global__ __launch_bounds__(kBlock) void k_matvec_rows(DevLevel L, StencilTenso
 
 // Surface nodes of the stencil matvec (used, not fixed, some element absent, or
 // on a coarse level some element partial): y_i is recomputed in element form over
-// the present elements around the node (TAB: each with the Galerkin matrix of its
-// child pattern, tab[pattern * 576], pattern 0 = absent),
-// on differences to x_i (Ke annihilates rigid translations), one list entry per
-// thread, and the reduction gets x_i . (y_new - y_stencil).
+// the present elements around the node, on differences to x_i (Ke annihilates
+// rigid translations), one list entry per thread, and the reduction gets
+// x_i . (y_new - y_stencil).  TAB (coarse levels): a full element uses the
+// Galerkin matrix of the full child pattern, tab[255 * 576]; a partial element's
+// three rows times the element's values were formed by k_partial_elems, which
+// left them in ys[slot][entry] (ysz values per column).
 template <typename T, bool TAB, bool EPI>
 __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__restrict__ ke, SurfaceList sl,
                                                          const T *__restrict__ x, T *__restrict__ y, CGState *st,
-                                                         Reduction red, int fin, int honor_done, SmoothEpi<T> ep)
+                                                         Reduction red, int fin, int honor_done, SmoothEpi<T> ep,
+                                                         const T *__restrict__ ys, int64_t ysz)
 {
     {
         const int64_t co = col_offset(3 * L.nn);
         x += co;
         y += co;
+        if constexpr (TAB) ys += (int64_t)blockIdx.y * ysz;
         if constexpr (EPI) {  // D^-1 is one array shared by all columns
             ep.r += co;
             ep.z += co;
@@ -1364,7 +1368,22 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
         const int4 e = sl.e[ic];
         const unsigned present = act ? (unsigned)e.y : 0u, used = (unsigned)e.z;
         uint2 pat = make_uint2(0, 0);
+        // every slot's triple, used where the slot holds a partial element
+        // (unconditional loads in range; the others are discarded).  FP32 loads
+        // them all here; FP64, whose 27 neighbour triples already take most of
+        // the registers, loads each at its slot
+        constexpr bool HOIST = TAB && sizeof(T) == 4;
+        T yp[HOIST ? 8 : 1][3];
         if constexpr (TAB) pat = sl.pat[ic];
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int el = 0; el < 8; el++) {
+                const T *q = ys + 3 * ((int64_t)el * sl.n + ic);
+                yp[el][0] = q[0];
+                yp[el][1] = q[1];
+                yp[el][2] = q[2];
+            }
+        }
         T xv[27][3];
 #pragma unroll
         for (int r = 0; r < 9; r++) {
@@ -1406,14 +1425,28 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
             T s0 = 0, s1 = 0, s2 = 0;
             if constexpr (TAB) {
                 // full elements (pattern 255) share one Galerkin matrix, read from
-                // LDS (one address for the whole wave); a partial element's matrix
-                // is gathered per lane from the pattern table
+                // LDS (one address for the whole wave); a partial element's rows
+                // were multiplied out by the element pass
                 const unsigned pe = ((el < 4 ? pat.x : pat.y) >> (8 * (el & 3))) & 255u;
                 const bool full = pe == 255u, here = (present >> el) & 1;
+                T q0, q1, q2;
+                if constexpr (HOIST) {
+                    q0 = yp[el][0];
+                    q1 = yp[el][1];
+                    q2 = yp[el][2];
+                } else {
+                    const T *q = ys + 3 * ((int64_t)el * sl.n + ic);
+                    q0 = q[0];
+                    q1 = q[1];
+                    q2 = q[2];
+                }
                 if (__any(full && here))
                     if (full) rows3(kf + 3 * Ln * 24, s0, s1, s2);
-                if (__any(!full && here))
-                    if (!full) rows3(ke + (int64_t)pe * 576 + 3 * Ln * 24, s0, s1, s2);
+                if (!full) {
+                    s0 = q0;
+                    s1 = q1;
+                    s2 = q2;
+                }
             } else {
                 rows3(kf + 3 * Ln * 24, s0, s1, s2);
             }
@@ -3895,6 +3928,120 @@ __global__ __launch_bounds__(kBlock) void k_surface(DevLevel L, int coarse, cons
     if (coarse) pat[k] = make_uint2(pw[0], pw[1]);
 }
 
+// Partial elements of a coarse stencil-form level (child pattern neither 0 nor
+// 255), for k_partial_elems: corner n of element e is node node[8e + n]; dst[8e + n]
+// is (7 - n) * nsurf + its surface-list entry (the element is slot 7 - n of that
+// node), or -1 for a fixed corner, which the operator does not recompute.
+struct PartialList {
+    int64_t n;
+    const int32_t *node;
+    const int32_t *dst;
+    const uint8_t *pat;
+};
+
+// map[node] = surface-list entry of the node (map preset to -1)
+__global__ __launch_bounds__(kBlock) void k_surface_index(SurfaceList sl, int32_t *__restrict__ map)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < sl.n) map[sl.e[k].x] = (int32_t)k;
+}
+
+// OUT = false: cnt[w] = partial elements among elements 64 w .. 64 w + 63 (linear
+// element order); OUT = true: the list entries, at off[w] onwards in that order
+template <bool OUT>
+__global__ __launch_bounds__(kBlock) void k_partial_list(DevLevel L, int32_t *__restrict__ cnt,
+                                                          const int32_t *__restrict__ off, const int32_t *__restrict__ map,
+                                                          int64_t nsurf, int32_t *__restrict__ node,
+                                                          int32_t *__restrict__ dst, uint8_t *__restrict__ pat)
+{
+    const int64_t ne = (int64_t)L.ex * L.ey * L.ez;
+    const int64_t wid = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    if (wid * 64 >= ne) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int64_t e = wid * 64 + lane;
+    const unsigned p = e < ne ? L.epat[e] : 0u;
+    const bool part = p != 0u && p != 255u;
+    const uint64_t b = __ballot(part);
+    if constexpr (!OUT) {
+        if (lane == 0) cnt[wid] = __popcll(b);
+        return;
+    }
+    if (!part) return;
+    const int64_t k = off[wid] + __popcll(b & ((1ull << lane) - 1));
+    const int ex = (int)(e % L.ex), ey = (int)((e / L.ex) % L.ey), ez = (int)(e / ((int64_t)L.ex * L.ey));
+    pat[k] = (uint8_t)p;
+    for (int n = 0; n < 8; n++) {
+        bool used, fixed;
+        const int id = node_info(L, ex + (n & 1), ey + ((n >> 1) & 1), ez + (n >> 2), used, fixed);
+        const int32_t sidx = used && !fixed ? map[id] : -1;
+        node[8 * k + n] = used ? id : 0;
+        dst[8 * k + n] = sidx >= 0 ? (int32_t)((7 - n) * nsurf + sidx) : -1;
+    }
+}
+
+// Products of the partial elements for the surface pass (k_stencil_fix, TAB):
+// lane 8e + n forms rows 3n .. 3n+2 of element e's Galerkin matrix times the
+// element's values on differences to corner n, with the operations and order the
+// surface pass uses for a full element, and stores the three sums at dst.  The
+// matrix comes from an interleaved copy of the pattern table, tabi: value t of a
+// corner's 72 (row-major, 24 per row) sits at pattern * 576 + (t / W) 8W + n W +
+// t % W with W values per 16 bytes, so the 8 lanes of an element read 128
+// adjacent bytes per load.  The lanes of an element read the same 8 nodes.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_partial_elems(PartialList pl, const T *__restrict__ tabi,
+                                                           const T *__restrict__ x, int64_t nn, T *__restrict__ ys,
+                                                           int64_t ysz, const CGState *st, int honor_done)
+{
+    x += col_offset(3 * nn);
+    ys += (int64_t)blockIdx.y * ysz;
+    st = col_state(st);
+    if (honor_done && st->done) return;
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= 8 * pl.n) return;
+    const int64_t e = g >> 3;
+    const int n = (int)(g & 7);
+    const int32_t d = pl.dst[g];
+    const unsigned p = pl.pat[e];
+    T xv[8][3];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const T *q = x + 3 * (int64_t)pl.node[8 * e + m];
+        xv[m][0] = q[0];
+        xv[m][1] = q[1];
+        xv[m][2] = q[2];
+    }
+    T ctr[3];
+    {
+        const T *q = x + 3 * (int64_t)pl.node[g];
+        ctr[0] = q[0];
+        ctr[1] = q[1];
+        ctr[2] = q[2];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[m][j] -= ctr[j];
+    constexpr int W = 16 / (int)sizeof(T);
+    const T *K = tabi + (int64_t)p * 576 + n * W;
+    auto kt = [&](int t) { return K[(t / W) * 8 * W + t % W]; };
+    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const T v = xv[m][j];
+            s0 += kt(3 * m + j) * v;
+            s1 += kt(24 + 3 * m + j) * v;
+            s2 += kt(48 + 3 * m + j) * v;
+        }
+    if (d >= 0) {
+        T *q = ys + 3 * (int64_t)d;
+        q[0] = s0;
+        q[1] = s1;
+        q[2] = s2;
+    }
+}
+
 // busy[tile] = 1 for the bx x by x bz node tiles of level L that hold a used node
 __global__ __launch_bounds__(kBlock) void k_tiles_busy(DevLevel L, int bx, int by, int bz, int tx, int ty,
                                                         uint8_t *__restrict__ busy)
@@ -4028,6 +4175,7 @@ struct LevelStencil {
     RowTable rows{};            // row table of k_matvec_rows
     SurfaceList surf{};         // nodes recomputed by k_stencil_fix
     int fix_grid = 0;           // blocks of k_stencil_fix (0: no surface node)
+    PartialList part{};         // coarse levels: partial elements for k_partial_elems
     bool dcon_interior = false; // every used, non-fixed node outside the surface list has D^-1 = dcon
 };
 
@@ -4054,6 +4202,13 @@ struct vfem_system {
     float *dcon32 = nullptr;
     double *ktab64[kMaxLevels] = {};
     float *ktab32[kMaxLevels] = {};
+    // the pattern tables interleaved for k_partial_elems, and its products (one
+    // triple per surface entry and slot, ysz values per column)
+    double *tabi64[kMaxLevels] = {};
+    float *tabi32[kMaxLevels] = {};
+    double *ys64[kMaxLevels] = {};
+    float *ys32[kMaxLevels] = {};
+    int64_t ysz[kMaxLevels] = {};
     double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
     int32_t *freedof = nullptr;
     int64_t n3 = 0;
@@ -4143,6 +4298,18 @@ template <typename T> std::vector<T> convert(const std::vector<double> &v)
 {
     std::vector<T> o(v.size());
     for (size_t i = 0; i < v.size(); i++) o[i] = (T)v[i];
+    return o;
+}
+
+// pattern table (256 x 576, row-major) in k_partial_elems's order: value t of the
+// 72 that rows 3n .. 3n+2 hold, at (t / W) 8W + n W + t % W, W values per 16 bytes
+template <typename T> std::vector<T> interleave_table(const std::vector<T> &tab)
+{
+    constexpr int W = 16 / (int)sizeof(T);
+    std::vector<T> o(tab.size());
+    for (size_t p = 0; p + 576 <= tab.size(); p += 576)
+        for (int n = 0; n < 8; n++)
+            for (int t = 0; t < 72; t++) o[p + (t / W) * 8 * W + n * W + t % W] = tab[p + 72 * n + t];
     return o;
 }
 
@@ -4382,6 +4549,57 @@ bool stencil_tensor(const double *ke, int perm, LevelStencil &LS)
 // of their 27 neighbours and, on coarse levels, the 8 elements' child patterns.
 // Leaves ok false when the level has no such form (its element-form kernel then
 // serves it).
+// partial elements of coarse level l with their corners' nodes and surface-list
+// slots (PartialList), built on the device once the surface list exists
+int build_partial_list(vfem_system *S, int l)
+{
+    const HostLevel &L = S->H.L[l];
+    LevelStencil &LS = S->stl[l];
+    const DevLevel &DL = S->dl[l];
+    const int64_t ne = (int64_t)L.ex * L.ey * L.ez, nw = (ne + 63) / 64;
+    if (8 * LS.surf.n >= INT32_MAX) {  // dst holds slot * entries + entry in 31 bits
+        g_last_error = "too many surface nodes on a coarse level";
+        return VFEM_ERR_UNSUPPORTED;
+    }
+    struct Free {
+        void *p;
+        ~Free() { if (p) (void)hipFree(p); }
+    };
+    int32_t *cnt = nullptr, *off = nullptr, *map = nullptr;
+    HIP_TRY(hipMalloc(&cnt, (nw + 1) * sizeof(int32_t)));
+    Free f1{cnt};
+    HIP_TRY(hipMalloc(&off, (nw + 1) * sizeof(int32_t)));
+    Free f2{off};
+    HIP_TRY(hipMalloc(&map, std::max<int64_t>(L.nn, 1) * sizeof(int32_t)));
+    Free f3{map};
+    HIP_TRY(hipMemsetAsync(cnt, 0, (nw + 1) * sizeof(int32_t), S->stream));
+    HIP_TRY(hipMemsetAsync(map, 0xFF, std::max<int64_t>(L.nn, 1) * sizeof(int32_t), S->stream));
+    hipLaunchKernelGGL(k_surface_index, dim3((int)((LS.surf.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, S->stream,
+                       LS.surf, map);
+    const int grid = (int)((nw + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL(k_partial_list<false>, dim3(grid), dim3(kBlock), 0, S->stream, DL, cnt, (const int32_t *)nullptr,
+                       (const int32_t *)nullptr, (int64_t)0, (int32_t *)nullptr, (int32_t *)nullptr, (uint8_t *)nullptr);
+    HIP_TRY(hipGetLastError());
+    int64_t np = 0;
+    std::string err;
+    if (int rc = device_exclusive_scan(cnt, nw, off, &np, S->stream, err)) {
+        g_last_error = err;
+        return rc;
+    }
+    int32_t *node, *dst;
+    uint8_t *pat;
+    if (int rc = S->alloc(&node, 8 * std::max<int64_t>(np, 1), false)) return rc;
+    if (int rc = S->alloc(&dst, 8 * std::max<int64_t>(np, 1), false)) return rc;
+    if (int rc = S->alloc(&pat, std::max<int64_t>(np, 1), false)) return rc;
+    if (np > 0)
+        hipLaunchKernelGGL(k_partial_list<true>, dim3(grid), dim3(kBlock), 0, S->stream, DL, (int32_t *)nullptr,
+                           (const int32_t *)off, (const int32_t *)map, LS.surf.n, node, dst, pat);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(S->stream));
+    LS.part = PartialList{np, node, dst, pat};
+    return 0;
+}
+
 int build_level_stencil(vfem_system *S, int l)
 {
     const Hierarchy &H = S->H;
@@ -4469,6 +4687,8 @@ int build_level_stencil(vfem_system *S, int l)
     LS.surf.e = ds;
     LS.surf.rb = drb;
     LS.surf.pat = dp;
+    if (coarse && nsurf > 0)
+        if (int rc = build_partial_list(S, l)) return rc;
     LS.ok = true;
     return 0;
 }
@@ -4552,6 +4772,14 @@ int alloc_vectors(vfem_system *S, int nb)
         }
     }
     if (int rc = S->alloc(&S->st, nb, true, V)) return rc;
+    // the surface pass of a coarse level reads every slot's triple (in range)
+    // whether or not the level has partial elements: sized by its surface list
+    for (int l = 1; l < H.nlev; l++)
+        if (S->stl[l].ok && S->stl[l].surf.n > 0) {
+            S->ysz[l] = 24 * S->stl[l].surf.n;
+            if (int rc = S->alloc(&S->ys64[l], nb * S->ysz[l], false, V)) return rc;
+            if (int rc = S->alloc(&S->ys32[l], nb * S->ysz[l], false, V)) return rc;
+        }
     {
         const MfmaTiles t = mfma_tiles(S->dl[0]);
         int64_t cap = std::max<int64_t>(kMaxPartials, (int64_t)t.tx * t.ty * t.tz);
@@ -4604,6 +4832,8 @@ int upload_system(vfem_system *S, const DevHierarchy &DH)
         if (l >= 1 && l < (int)H.ktab.size() && !H.ktab[l].empty()) {
             if (int rc = S->upload(&S->ktab64[l], H.ktab[l])) return rc;
             if (int rc = S->upload(&S->ktab32[l], convert<float>(H.ktab[l]))) return rc;
+            if (int rc = S->upload(&S->tabi64[l], interleave_table(H.ktab[l]))) return rc;
+            if (int rc = S->upload(&S->tabi32[l], interleave_table(convert<float>(H.ktab[l])))) return rc;
         }
     }
     {
@@ -4696,6 +4926,12 @@ template <> const float *kefine<float>(vfem_system *S) { return S->ke32; }
 template <typename T> const T *ktab(vfem_system *S, int l);
 template <> const double *ktab<double>(vfem_system *S, int l) { return S->ktab64[l]; }
 template <> const float *ktab<float>(vfem_system *S, int l) { return S->ktab32[l]; }
+template <typename T> const T *tabi(vfem_system *S, int l);
+template <> const double *tabi<double>(vfem_system *S, int l) { return S->tabi64[l]; }
+template <> const float *tabi<float>(vfem_system *S, int l) { return S->tabi32[l]; }
+template <typename T> T *ysbuf(vfem_system *S, int l);
+template <> double *ysbuf<double>(vfem_system *S, int l) { return S->ys64[l]; }
+template <> float *ysbuf<float>(vfem_system *S, int l) { return S->ys32[l]; }
 const double *kinv(vfem_system *S, bool gal) { return gal ? S->kinv_gal64 : S->kinv_ref64; }
 
 // level-0 matvec.  FP64: node-centric VALU kernel.  FP32: matrix cores with
@@ -4806,12 +5042,20 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
         r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
-        if (l == 0)
+        if (l == 0) {
             hipLaunchKernelGGL((k_stencil_fix<T, false, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
-                               L, kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{});
-        else
+                               L, kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{},
+                               (const T *)nullptr, (int64_t)0);
+        } else {
+            // the partial elements' rows first, then the surface pass that adds them
+            if (LS.part.n > 0)
+                hipLaunchKernelGGL(k_partial_elems<T>, dim3((int)((8 * LS.part.n + kBlock - 1) / kBlock), S->ncols),
+                                   dim3(kBlock), 0, S->stream, LS.part, tabi<T>(S, l), x, L.nn, ysbuf<T>(S, l),
+                                   S->ysz[l], S->st, honor_done);
             hipLaunchKernelGGL((k_stencil_fix<T, true, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
-                               L, ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{});
+                               L, ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{},
+                               (const T *)ysbuf<T>(S, l), S->ysz[l]);
+        }
     }
     enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, honor_done);
 }
@@ -4831,7 +5075,7 @@ void launch_stencil_smooth(vfem_system *S, const T *x, T *z, const double *r, co
         Reduction r2 = S->red;
         r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
         hipLaunchKernelGGL((k_stencil_fix<T, false, true>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
-                           kefine<T>(S), LS.surf, x, (T *)nullptr, S->st, r2, fin, 1, ep);
+                           kefine<T>(S), LS.surf, x, (T *)nullptr, S->st, r2, fin, 1, ep, (const T *)nullptr, (int64_t)0);
     }
     enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, 1);
 }
