@@ -521,6 +521,94 @@ struct SurfaceList {
     const uint2 *pat;  // coarse levels: child pattern of the 8 elements (byte a + 2b + 4c)
 };
 
+// Partial elements of a coarse stencil-form level (child pattern neither 0 nor
+// 255), for k_partial_elems: corner n of element e is node node[8e + n]; dst[8e + n]
+// is (7 - n) * nsurf + its surface-list entry (the element is slot 7 - n of that
+// node), or -1 for a fixed corner, which the operator does not recompute.
+struct PartialList {
+    int64_t n;
+    const int32_t *node;
+    const int32_t *dst;
+    const uint8_t *pat;
+};
+
+// The partial-element pass of a level: its list, the interleaved pattern table
+// and the products' buffer (ysz values per column).  It runs as k_partial_elems
+// or as extra blocks of the level's stencil kernel, which reads the same x.
+template <typename T> struct PartialJob {
+    PartialList pl;
+    const T *tabi;
+    T *ys;
+    int64_t ysz;
+};
+
+// Lane g = 8e + n of the partial-element pass (x and ys offset to the column):
+// rows 3n .. 3n+2 of element e's Galerkin matrix times the element's values on
+// differences to corner n, with the operations and order the surface pass
+// (k_stencil_fix) uses for a full element, stored at dst for the surface node
+// at that corner.  The matrix comes from an interleaved copy of the pattern
+// table: value t of a corner's 72 (row-major, 24 per row) sits at pattern * 576 +
+// (t / W) 8W + n W + t % W with W values per 16 bytes, so the 8 lanes of an
+// element read 128 adjacent bytes per load.  The lanes of an element read the
+// same 8 nodes.
+template <typename T>
+__device__ __forceinline__ void partial_elem_lane(const PartialList &pl, const T *__restrict__ tabi,
+                                                  const T *__restrict__ x, T *__restrict__ ys, int64_t g)
+{
+    const int64_t e = g >> 3;
+    const int n = (int)(g & 7);
+    const int32_t d = pl.dst[g];
+    const unsigned p = pl.pat[e];
+    T xv[8][3];
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const T *q = x + 3 * (int64_t)pl.node[8 * e + m];
+        xv[m][0] = q[0];
+        xv[m][1] = q[1];
+        xv[m][2] = q[2];
+    }
+    T ctr[3];
+    {
+        const T *q = x + 3 * (int64_t)pl.node[g];
+        ctr[0] = q[0];
+        ctr[1] = q[1];
+        ctr[2] = q[2];
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) xv[m][j] -= ctr[j];
+    constexpr int W = 16 / (int)sizeof(T);
+    const T *K = tabi + (int64_t)p * 576 + n * W;
+    auto kt = [&](int t) { return K[(t / W) * 8 * W + t % W]; };
+    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const T v = xv[m][j];
+            s0 += kt(3 * m + j) * v;
+            s1 += kt(24 + 3 * m + j) * v;
+            s2 += kt(48 + 3 * m + j) * v;
+        }
+    if (d >= 0) {
+        T *q = ys + 3 * (int64_t)d;
+        q[0] = s0;
+        q[1] = s1;
+        q[2] = s2;
+    }
+}
+
+// the extra blocks of a stencil kernel launched with a partial-element job:
+// remapped block index rb at or past the tile count (blocks past the job's
+// lanes do nothing)
+template <typename T>
+__device__ __forceinline__ void partial_elem_block(const PartialJob<T> &pj, const T *__restrict__ x, int rb, int ntiles)
+{
+    const int64_t g = (int64_t)(rb - ntiles) * kBlock + threadIdx.x;
+    if (g < 8 * pj.pl.n) partial_elem_lane(pj.pl, pj.tabi, x, pj.ys + (int64_t)blockIdx.y * pj.ysz, g);
+}
+
 // A staged node: its three values and id << 2 | used << 1 | fixed (bit pattern).
 template <typename T> struct alignas(4 * sizeof(T)) StNode {
     T v[3];
@@ -795,10 +883,11 @@ template <int CTRL> __device__ __forceinline__ double dpp_row(double v)
 // 1 = GM ({x,z}, z weight G), 2 = MG ({y,z}, z weight G)
 __host__ __device__ constexpr int sf_pair(int a, int b) { return (a + b == 1) ? 0 : (a + b == 2 ? 1 : 2); }
 
-template <typename T, int PERM, int ZC, bool EPI>
+template <typename T, int PERM, int ZC, bool EPI, bool PART = false>
 __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<T> K, StencilTiles tiles,
                                                        const T *__restrict__ x, T *__restrict__ y, CGState *st,
-                                                       Reduction red, int fin, int honor_done, SmoothEpi<T> ep)
+                                                       Reduction red, int fin, int honor_done, SmoothEpi<T> ep,
+                                                       PartialJob<T> pj)
 {
     constexpr bool CENTRE = sizeof(T) == 4;
     constexpr int NSLOT = SF_HN;
@@ -816,7 +905,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sf(DevLevel L, StencilTensor<
     }
     if (honor_done && st->done) return;
     __shared__ StNode<T> pl[2][SF_HN];
-    const uint32_t tl = tiles.tile[remap_block()];
+    const int rb = remap_block();
+    if constexpr (PART)
+        if (rb >= tiles.n) {  // extra blocks: the level's partial elements (no reduction)
+            partial_elem_block(pj, x, rb, tiles.n);
+            return;
+        }
+    const uint32_t tl = tiles.tile[rb];
     if (tl == kNoTile) {  // padding: no work, but the reduction slot is written
         if (fin >= 0) block_partials(0.0, red);
         return;
@@ -1131,10 +1226,11 @@ __device__ __forceinline__ PlaneRows plane_rows(ConstRowLane tab, int ia, int ib
     return r;
 }
 
-template <typename T, int PERM, int ZC, bool EPI>
+template <typename T, int PERM, int ZC, bool EPI, bool PART = false>
 __global__ __launch_bounds__(kBlock) void k_matvec_rows(DevLevel L, StencilTensor<T> K, StencilTiles tiles, RowTable rt,
                                                          const T *__restrict__ x, T *__restrict__ y, CGState *st,
-                                                         Reduction red, int fin, int honor_done, SmoothEpi<T> ep)
+                                                         Reduction red, int fin, int honor_done, SmoothEpi<T> ep,
+                                                         PartialJob<T> pj)
 {
     constexpr bool CENTRE = sizeof(T) == 4;
     {
@@ -1150,7 +1246,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_rows(DevLevel L, StencilTenso
         red = col_red(red);
     }
     if (honor_done && st->done) return;
-    const uint32_t tl = tiles.tile[remap_block()];
+    const int rb = remap_block();
+    if constexpr (PART)
+        if (rb >= tiles.n) {  // extra blocks: the level's partial elements (no reduction)
+            partial_elem_block(pj, x, rb, tiles.n);
+            return;
+        }
+    const uint32_t tl = tiles.tile[rb];
     if (tl == kNoTile) {  // padding: no work, but the reduction slot is written
         if (fin >= 0) block_partials(0.0, red);
         return;
@@ -3928,16 +4030,16 @@ __global__ __launch_bounds__(kBlock) void k_surface(DevLevel L, int coarse, cons
     if (coarse) pat[k] = make_uint2(pw[0], pw[1]);
 }
 
-// Partial elements of a coarse stencil-form level (child pattern neither 0 nor
-// 255), for k_partial_elems: corner n of element e is node node[8e + n]; dst[8e + n]
-// is (7 - n) * nsurf + its surface-list entry (the element is slot 7 - n of that
-// node), or -1 for a fixed corner, which the operator does not recompute.
-struct PartialList {
-    int64_t n;
-    const int32_t *node;
-    const int32_t *dst;
-    const uint8_t *pat;
-};
+// the partial-element pass on its own (FP64 levels, fused reductions)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_partial_elems(PartialJob<T> pj, const T *__restrict__ x, int64_t nn,
+                                                           const CGState *st, int honor_done)
+{
+    x += col_offset(3 * nn);
+    st = col_state(st);
+    if (honor_done && st->done) return;
+    partial_elem_block(pj, x, blockIdx.x, 0);
+}
 
 // map[node] = surface-list entry of the node (map preset to -1)
 __global__ __launch_bounds__(kBlock) void k_surface_index(SurfaceList sl, int32_t *__restrict__ map)
@@ -3976,69 +4078,6 @@ __global__ __launch_bounds__(kBlock) void k_partial_list(DevLevel L, int32_t *__
         const int32_t sidx = used && !fixed ? map[id] : -1;
         node[8 * k + n] = used ? id : 0;
         dst[8 * k + n] = sidx >= 0 ? (int32_t)((7 - n) * nsurf + sidx) : -1;
-    }
-}
-
-// Products of the partial elements for the surface pass (k_stencil_fix, TAB):
-// lane 8e + n forms rows 3n .. 3n+2 of element e's Galerkin matrix times the
-// element's values on differences to corner n, with the operations and order the
-// surface pass uses for a full element, and stores the three sums at dst.  The
-// matrix comes from an interleaved copy of the pattern table, tabi: value t of a
-// corner's 72 (row-major, 24 per row) sits at pattern * 576 + (t / W) 8W + n W +
-// t % W with W values per 16 bytes, so the 8 lanes of an element read 128
-// adjacent bytes per load.  The lanes of an element read the same 8 nodes.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_partial_elems(PartialList pl, const T *__restrict__ tabi,
-                                                           const T *__restrict__ x, int64_t nn, T *__restrict__ ys,
-                                                           int64_t ysz, const CGState *st, int honor_done)
-{
-    x += col_offset(3 * nn);
-    ys += (int64_t)blockIdx.y * ysz;
-    st = col_state(st);
-    if (honor_done && st->done) return;
-    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= 8 * pl.n) return;
-    const int64_t e = g >> 3;
-    const int n = (int)(g & 7);
-    const int32_t d = pl.dst[g];
-    const unsigned p = pl.pat[e];
-    T xv[8][3];
-#pragma unroll
-    for (int m = 0; m < 8; m++) {
-        const T *q = x + 3 * (int64_t)pl.node[8 * e + m];
-        xv[m][0] = q[0];
-        xv[m][1] = q[1];
-        xv[m][2] = q[2];
-    }
-    T ctr[3];
-    {
-        const T *q = x + 3 * (int64_t)pl.node[g];
-        ctr[0] = q[0];
-        ctr[1] = q[1];
-        ctr[2] = q[2];
-    }
-#pragma unroll
-    for (int m = 0; m < 8; m++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) xv[m][j] -= ctr[j];
-    constexpr int W = 16 / (int)sizeof(T);
-    const T *K = tabi + (int64_t)p * 576 + n * W;
-    auto kt = [&](int t) { return K[(t / W) * 8 * W + t % W]; };
-    T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-    for (int m = 0; m < 8; m++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const T v = xv[m][j];
-            s0 += kt(3 * m + j) * v;
-            s1 += kt(24 + 3 * m + j) * v;
-            s2 += kt(48 + 3 * m + j) * v;
-        }
-    if (d >= 0) {
-        T *q = ys + 3 * (int64_t)d;
-        q[0] = s0;
-        q[1] = s1;
-        q[2] = s2;
     }
 }
 
@@ -4989,29 +5028,48 @@ const StencilTiles &stencil_tiles(const vfem_system *S, const LevelStencil &LS, 
     return use_rows(S, LS, fp64) ? LS.tiles_rw : use_sf(S, LS) ? LS.tiles_sf : LS.tiles;
 }
 
+// pj (a level's partial-element pass, or null): run it as extra blocks of the
+// row or plane-staged kernel, padded to a multiple of 8 so that the tiles keep
+// their XCD mapping; returns whether it ran (other kernels leave it to the caller)
 template <typename T, int ZC, bool EPI>
-void launch_stencil_kernel(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
-                           int honor_done, SmoothEpi<T> ep)
+bool launch_stencil_kernel(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
+                           int honor_done, SmoothEpi<T> ep, const PartialJob<T> *pj = nullptr)
 {
+    const int pb = pj ? (int)((8 * pj->pl.n + 8 * kBlock - 1) / (8 * kBlock)) * 8 : 0;
+    const PartialJob<T> job = pj ? *pj : PartialJob<T>{};
     if constexpr (sizeof(T) == 4) {  // FP64: plane-staged kernel
         if (use_rows(S, LS, false)) {
-            if (LS.perm == 1)
-                hipLaunchKernelGGL((k_matvec_rows<T, 1, ZC, EPI>), dim3(LS.tiles_rw.n, S->ncols), dim3(kBlock), 0, S->stream,
-                                   L, st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep);
+            const dim3 g(LS.tiles_rw.n + pb, S->ncols);
+            if (pj && LS.perm == 1)
+                hipLaunchKernelGGL((k_matvec_rows<T, 1, ZC, EPI, true>), g, dim3(kBlock), 0, S->stream, L,
+                                   st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep, job);
+            else if (pj)
+                hipLaunchKernelGGL((k_matvec_rows<T, 0, ZC, EPI, true>), g, dim3(kBlock), 0, S->stream, L,
+                                   st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep, job);
+            else if (LS.perm == 1)
+                hipLaunchKernelGGL((k_matvec_rows<T, 1, ZC, EPI>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                                   LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep, job);
             else
-                hipLaunchKernelGGL((k_matvec_rows<T, 0, ZC, EPI>), dim3(LS.tiles_rw.n, S->ncols), dim3(kBlock), 0, S->stream,
-                                   L, st_tensor<T>(LS), LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep);
-            return;
+                hipLaunchKernelGGL((k_matvec_rows<T, 0, ZC, EPI>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                                   LS.tiles_rw, LS.rows, x, y, S->st, S->red, fin, honor_done, ep, job);
+            return pj != nullptr;
         }
     }
     if (use_sf(S, LS)) {
-        if (LS.perm == 1)
-            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC, EPI>), dim3(LS.tiles_sf.n, S->ncols), dim3(kBlock), 0, S->stream, L,
-                               st_tensor<T>(LS), LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep);
+        const dim3 g(LS.tiles_sf.n + pb, S->ncols);
+        if (pj && LS.perm == 1)
+            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC, EPI, true>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                               LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep, job);
+        else if (pj)
+            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC, EPI, true>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                               LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep, job);
+        else if (LS.perm == 1)
+            hipLaunchKernelGGL((k_matvec_sf<T, 1, ZC, EPI>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                               LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep, job);
         else
-            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC, EPI>), dim3(LS.tiles_sf.n, S->ncols), dim3(kBlock), 0, S->stream, L,
-                               st_tensor<T>(LS), LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep);
-        return;
+            hipLaunchKernelGGL((k_matvec_sf<T, 0, ZC, EPI>), g, dim3(kBlock), 0, S->stream, L, st_tensor<T>(LS),
+                               LS.tiles_sf, x, y, S->st, S->red, fin, honor_done, ep, job);
+        return pj != nullptr;
     }
     if (LS.perm == 1)
         hipLaunchKernelGGL((k_matvec_stencil<T, 1, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
@@ -5019,16 +5077,18 @@ void launch_stencil_kernel(vfem_system *S, const DevLevel &L, const LevelStencil
     else
         hipLaunchKernelGGL((k_matvec_stencil<T, 0, ZC>), dim3(LS.tiles.n, S->ncols), dim3(kBlock), 0, S->stream, L,
                            st_coef<T>(LS), LS.tiles, x, y, S->st, S->red, fin, honor_done);
+    return false;
 }
 
 template <typename T, int ZC, bool EPI = false>
-void launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
-                       int honor_done, SmoothEpi<T> ep = {})
+bool launch_stencil_zc(vfem_system *S, const DevLevel &L, const LevelStencil &LS, const T *x, T *y, int fin,
+                       int honor_done, SmoothEpi<T> ep = {}, const PartialJob<T> *pj = nullptr)
 {
     const bool probe = !EPI && &LS == &S->stl[0] && S->probe_n < S->probe_cap;  // the plain operator only
     if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n], S->stream);
-    launch_stencil_kernel<T, ZC, EPI>(S, L, LS, x, y, fin, honor_done, ep);
+    const bool ran = launch_stencil_kernel<T, ZC, EPI>(S, L, LS, x, y, fin, honor_done, ep, pj);
     if (probe) (void)hipEventRecord(S->probe_ev[2 * S->probe_n++ + 1], S->stream);
+    return ran;
 }
 
 // stencil matvec of level l plus its surface recomputation; the reduction (level
@@ -5037,8 +5097,14 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
 {
     const DevLevel &L = S->dl[l];
     const LevelStencil &LS = S->stl[l];
-    if (LS.zc == kStencilZShort) launch_stencil_zc<T, kStencilZShort>(S, L, LS, x, y, fin, honor_done);
-    else launch_stencil_zc<T, kStencilZ>(S, L, LS, x, y, fin, honor_done);
+    // a coarse level's partial elements: extra blocks of the stencil kernel (which
+    // reads the same x) where the level has no reduction, else a launch of their own
+    const PartialJob<T> job{LS.part, tabi<T>(S, l), ysbuf<T>(S, l), S->ysz[l]};
+    const PartialJob<T> *pj = l > 0 && LS.part.n > 0 && fin < 0 ? &job : nullptr;
+    bool part_done;
+    if (LS.zc == kStencilZShort)
+        part_done = launch_stencil_zc<T, kStencilZShort>(S, L, LS, x, y, fin, honor_done, SmoothEpi<T>{}, pj);
+    else part_done = launch_stencil_zc<T, kStencilZ>(S, L, LS, x, y, fin, honor_done, SmoothEpi<T>{}, pj);
     if (LS.fix_grid > 0) {
         Reduction r2 = S->red;
         r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
@@ -5048,10 +5114,9 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
                                (const T *)nullptr, (int64_t)0);
         } else {
             // the partial elements' rows first, then the surface pass that adds them
-            if (LS.part.n > 0)
+            if (LS.part.n > 0 && !part_done)
                 hipLaunchKernelGGL(k_partial_elems<T>, dim3((int)((8 * LS.part.n + kBlock - 1) / kBlock), S->ncols),
-                                   dim3(kBlock), 0, S->stream, LS.part, tabi<T>(S, l), x, L.nn, ysbuf<T>(S, l),
-                                   S->ysz[l], S->st, honor_done);
+                                   dim3(kBlock), 0, S->stream, job, x, L.nn, S->st, honor_done);
             hipLaunchKernelGGL((k_stencil_fix<T, true, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
                                L, ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{},
                                (const T *)ysbuf<T>(S, l), S->ysz[l]);
