@@ -235,7 +235,7 @@ def cpu_reference_cycle(threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ellipsoids256")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16"])
