@@ -131,9 +131,10 @@ def test_chebyshev_options_checked(system):
 
 def test_chebyshev_fused_and_separate_steps_agree(system):
     """The Chebyshev steps run fused into the element-form product (L0_AUTO on
-    these small coarse levels) or as separate k_cheb_step launches after the
-    stencil-form product (L0_STENCIL): both against the oracle's twin, for an
-    even and an odd number of steps (an odd count ends in the level's t buffer)."""
+    these small coarse levels) or into the stencil form's surface-pass launch
+    (L0_STENCIL: the surface nodes in its epilogue, the other DOFs in extra
+    blocks): both against the oracle's twin, for an even and an odd number of
+    steps (an odd count ends in the level's t buffer)."""
     A, ora, _ = system
     lmax = [A.level_lmax(l) for l in range(A.num_levels)]
     r = np.random.default_rng(23).standard_normal(A.n)

@@ -1431,25 +1431,96 @@ __global__ __launch_bounds__(kBlock) void k_matvec_rows(DevLevel L, StencilTenso
     if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
+template <typename Td> struct DinvRef {
+    const Td *d;
+    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
+    Td c[3];
+    const Td *cd;          // the same triple in device memory
+};
+// D^-1 at compact DOF k of node nd = k / 3 (flag: the node holds the shared
+// triple).  The triple is read from device memory (cd), so the choice between
+// cd[k - 3 nd] and d[k] is one global load from a chosen address; with the
+// triple as a kernel argument the chosen pointer was a generic one and the load
+// a flat load that waited at once.
+template <typename Td>
+__device__ __forceinline__ Td dinv_at(const DinvRef<Td> &D, bool flag, uint32_t k, uint32_t nd)
+{
+    return flag ? D.cd[k - 3u * nd] : D.d[k];
+}
+
+// One Chebyshev smoothing step fused into the level product that feeds it (the
+// epilogue of k_matvec_coarse, or of a stencil level's surface pass with the
+// other nodes in extra blocks of that launch): with t = (A x)_i at the node,
+// d = a d_prev + b D^-1 (r - t) (d_prev = x when first), x_out = x + d, d written
+// when dout is set.  x_out is another buffer than the product's input x, which
+// other blocks still read; d is per node, so it is updated in place.  Fixed
+// rows: d = 0.  Inactive when xout is null (the plain product y = A x).
+template <typename T> struct ChebEpi {
+    const T *r;
+    DinvRef<T> D;
+    const T *dprev;
+    T *dout;
+    T *xout;
+    double a, b;
+    int first;
+};
+
+// The Chebyshev step of a stencil level at every DOF outside the surface list
+// (smask: compact-order bitmask of the surface nodes, whose step the surface
+// pass takes), after the row kernel wrote t = A x into cep.xout: the same
+// operations as k_cheb_step, x_out = x + d written over t at the same index.
+// Blocks b of G, flat over the level's n3 DOFs.
+template <typename T>
+__device__ __forceinline__ void cheb_rest(const ChebEpi<T> &cep, const T *__restrict__ x,
+                                          const uint32_t *__restrict__ fixc, const uint32_t *__restrict__ smask,
+                                          int64_t n3, int b, int G)
+{
+    const T ta = T(cep.a), tb = T(cep.b);
+    for (int64_t k = (int64_t)b * kBlock + threadIdx.x; k < n3; k += (int64_t)G * kBlock) {
+        const uint32_t nd = (uint32_t)(k / 3);
+        const uint32_t sw = smask[nd >> 5], fw = fixc[nd >> 5];
+        const uint32_t cw = cep.D.flat ? cep.D.flat[nd >> 5] : 0u;
+        const T t = cep.xout[k], xv = x[k], rv = cep.r[k];
+        const T dp = cep.first ? xv : (cep.a != 0 ? cep.dprev[k] : T(0));
+        T dn = ta * dp + tb * dinv_at(cep.D, (cw >> (nd & 31)) & 1, (uint32_t)k, nd) * (rv - t);
+        if ((fw >> (nd & 31)) & 1) dn = 0;
+        if (!((sw >> (nd & 31)) & 1)) {
+            cep.xout[k] = xv + dn;
+            if (cep.dout) cep.dout[k] = dn;
+        }
+    }
+}
+
 // Surface nodes of the stencil matvec (used, not fixed, some element absent, or
 // on a coarse level some element partial): y_i is recomputed in element form over
 // the present elements around the node, on differences to x_i (Ke annihilates
 // rigid translations), one list entry per thread, and the reduction gets
 // x_i . (y_new - y_stencil).  TAB (coarse levels): a full element uses the
 // Galerkin matrix of the full child pattern, tab[255 * 576]; a partial element's
-// three rows times the element's values were formed by k_partial_elems, which
-// left them in ys[slot][entry] (ysz values per column).
-template <typename T, bool TAB, bool EPI>
+// three rows times the element's values were formed by the partial-element
+// pass, which left them in ys[slot][entry] (ysz values per column).  CHEB: a
+// Chebyshev step follows the product (ChebEpi, y unused): the surface entries
+// take it in their epilogue, and blocks sgrid and up take it at every other DOF
+// (cheb_rest).
+template <typename T, bool TAB, bool EPI, bool CHEB = false>
 __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__restrict__ ke, SurfaceList sl,
                                                          const T *__restrict__ x, T *__restrict__ y, CGState *st,
                                                          Reduction red, int fin, int honor_done, SmoothEpi<T> ep,
-                                                         const T *__restrict__ ys, int64_t ysz)
+                                                         const T *__restrict__ ys, int64_t ysz, ChebEpi<T> cep,
+                                                         const uint32_t *__restrict__ fixc,
+                                                         const uint32_t *__restrict__ smask, int sgrid)
 {
     {
         const int64_t co = col_offset(3 * L.nn);
         x += co;
         y += co;
         if constexpr (TAB) ys += (int64_t)blockIdx.y * ysz;
+        if constexpr (CHEB) {
+            cep.r += co;
+            if (cep.dprev) cep.dprev += co;
+            if (cep.dout) cep.dout += co;
+            cep.xout += co;
+        }
         if constexpr (EPI) {  // D^-1 is one array shared by all columns
             ep.r += co;
             ep.z += co;
@@ -1458,12 +1529,18 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
         red = col_red(red);
     }
     if (honor_done && st->done) return;
+    if constexpr (CHEB)
+        if ((int)blockIdx.x >= sgrid) {
+            cheb_rest(cep, x, fixc, smask, 3 * L.nn, (int)blockIdx.x - sgrid, (int)gridDim.x - sgrid);
+            return;
+        }
     // the full-element matrix (Ke, or the Galerkin matrix of the full child pattern)
     __shared__ T kf[576];
     for (int k = threadIdx.x; k < 576; k += kBlock) kf[k] = ke[(TAB ? 255 * 576 : 0) + k];
     __syncthreads();
     double dacc = 0;
-    for (int64_t base = (int64_t)remap_block() * kBlock; base < sl.n; base += (int64_t)gridDim.x * kBlock) {
+    const int64_t sg = CHEB ? sgrid : gridDim.x, sb0 = CHEB ? blockIdx.x : remap_block();
+    for (int64_t base = sb0 * kBlock; base < sl.n; base += sg * kBlock) {
         const int64_t i = base + threadIdx.x;
         const bool act = i < sl.n;
         const int64_t ic = act ? i : 0;
@@ -1561,7 +1638,21 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
             // all would take every register of the lane)
             __builtin_amdgcn_sched_barrier(0);
         }
-        if (act && EPI) {
+        if (CHEB && act) {
+            // the Chebyshev step at a surface node (never fixed) from its product
+            const int64_t q3 = 3 * (int64_t)e.x;
+            const uint32_t nd = (uint32_t)e.x;
+            const bool fl = cep.D.flat && ((cep.D.flat[nd >> 5] >> (nd & 31)) & 1);
+            const T ta = T(cep.a), tb = T(cep.b), cc[3] = {c0, c1, c2};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const int64_t k = q3 + c;
+                const T dp = cep.first ? ctr[c] : (cep.a != 0 ? cep.dprev[k] : T(0));
+                const T dn = ta * dp + tb * dinv_at(cep.D, fl, (uint32_t)k, nd) * (cep.r[k] - cc[c]);
+                cep.xout[k] = ctr[c] + dn;
+                if (cep.dout) cep.dout[k] = dn;
+            }
+        } else if (act && EPI) {
             // the smoothed value of a surface node (never fixed), replacing the one
             // the stencil kernel formed from its zero-extended product
             const int64_t q3 = 3 * (int64_t)e.x;
@@ -2321,38 +2412,6 @@ __global__ __launch_bounds__(kBlock) void k_matvec_block(DevLevel L, const float
 // they touch, and the four partial products meet in LDS in a fixed order.  The
 // levels this kernel serves are small and latency-bound: four short dependency
 // chains per segment instead of one long one.
-template <typename Td> struct DinvRef {
-    const Td *d;
-    const uint32_t *flat;  // compact-order node bitmask; null: read d everywhere
-    Td c[3];
-    const Td *cd;          // the same triple in device memory
-};
-// D^-1 at compact DOF k of node nd = k / 3 (flag: the node holds the shared
-// triple).  The triple is read from device memory (cd), so the choice between
-// cd[k - 3 nd] and d[k] is one global load from a chosen address; with the
-// triple as a kernel argument the chosen pointer was a generic one and the load
-// a flat load that waited at once.
-template <typename Td>
-__device__ __forceinline__ Td dinv_at(const DinvRef<Td> &D, bool flag, uint32_t k, uint32_t nd)
-{
-    return flag ? D.cd[k - 3u * nd] : D.d[k];
-}
-
-// One Chebyshev smoothing step fused into the level product that feeds it (the
-// epilogue of k_matvec_coarse): with t = (A x)_i at the node,
-// d = a d_prev + b D^-1 (r - t) (d_prev = x when first), x_out = x + d, d written
-// when dout is set.  x_out is another buffer than the product's input x, which
-// other blocks still read; d is per node, so it is updated in place.  Fixed
-// rows: d = 0.  Inactive when xout is null (the plain product y = A x).
-template <typename T> struct ChebEpi {
-    const T *r;
-    DinvRef<T> D;
-    const T *dprev;
-    T *dout;
-    T *xout;
-    double a, b;
-    int first;
-};
 
 template <typename T, int B, int C>
 __device__ __forceinline__ void coarse_pair(const DevLevel &L, const T *__restrict__ tab, const T *__restrict__ x,
@@ -4041,6 +4100,16 @@ __global__ __launch_bounds__(kBlock) void k_partial_elems(PartialJob<T> pj, cons
     partial_elem_block(pj, x, blockIdx.x, 0);
 }
 
+// the surface nodes' bits in a compact-order node mask (preset to 0)
+__global__ __launch_bounds__(kBlock) void k_surface_mask(SurfaceList sl, uint32_t *__restrict__ mask)
+{
+    const int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (k < sl.n) {
+        const uint32_t nd = (uint32_t)sl.e[k].x;
+        atomicOr(&mask[nd >> 5], 1u << (nd & 31));
+    }
+}
+
 // map[node] = surface-list entry of the node (map preset to -1)
 __global__ __launch_bounds__(kBlock) void k_surface_index(SurfaceList sl, int32_t *__restrict__ map)
 {
@@ -4248,6 +4317,8 @@ struct vfem_system {
     double *ys64[kMaxLevels] = {};
     float *ys32[kMaxLevels] = {};
     int64_t ysz[kMaxLevels] = {};
+    // coarse stencil levels: compact-order bitmask of the surface nodes
+    uint32_t *smask[kMaxLevels] = {};
     double *kinv_ref64 = nullptr, *kinv_gal64 = nullptr;
     int32_t *freedof = nullptr;
     int64_t n3 = 0;
@@ -4728,6 +4799,14 @@ int build_level_stencil(vfem_system *S, int l)
     LS.surf.pat = dp;
     if (coarse && nsurf > 0)
         if (int rc = build_partial_list(S, l)) return rc;
+    if (coarse) {  // for the Chebyshev steps fused into the surface pass
+        if (int rc = S->alloc(&S->smask[l], (L.nn + 31) / 32)) return rc;
+        if (nsurf > 0)
+            hipLaunchKernelGGL(k_surface_mask, dim3((int)((nsurf + kBlock - 1) / kBlock)), dim3(kBlock), 0, S->stream,
+                               LS.surf, S->smask[l]);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(S->stream));
+    }
     LS.ok = true;
     return 0;
 }
@@ -5111,7 +5190,8 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
         if (l == 0) {
             hipLaunchKernelGGL((k_stencil_fix<T, false, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
                                L, kefine<T>(S), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{},
-                               (const T *)nullptr, (int64_t)0);
+                               (const T *)nullptr, (int64_t)0, ChebEpi<T>{}, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, 0);
         } else {
             // the partial elements' rows first, then the surface pass that adds them
             if (LS.part.n > 0 && !part_done)
@@ -5119,7 +5199,8 @@ template <typename T> void launch_matvec_stencil(vfem_system *S, int l, const T 
                                    dim3(kBlock), 0, S->stream, job, x, L.nn, S->st, honor_done);
             hipLaunchKernelGGL((k_stencil_fix<T, true, false>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream,
                                L, ktab<T>(S, l), LS.surf, x, y, S->st, r2, fin, honor_done, SmoothEpi<T>{},
-                               (const T *)ysbuf<T>(S, l), S->ysz[l]);
+                               (const T *)ysbuf<T>(S, l), S->ysz[l], ChebEpi<T>{}, (const uint32_t *)nullptr,
+                               (const uint32_t *)nullptr, 0);
         }
     }
     enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, honor_done);
@@ -5140,7 +5221,8 @@ void launch_stencil_smooth(vfem_system *S, const T *x, T *z, const double *r, co
         Reduction r2 = S->red;
         r2.partials += stencil_tiles(S, LS, sizeof(T) == 8).n;
         hipLaunchKernelGGL((k_stencil_fix<T, false, true>), dim3(LS.fix_grid, S->ncols), dim3(kBlock), 0, S->stream, L,
-                           kefine<T>(S), LS.surf, x, (T *)nullptr, S->st, r2, fin, 1, ep, (const T *)nullptr, (int64_t)0);
+                           kefine<T>(S), LS.surf, x, (T *)nullptr, S->st, r2, fin, 1, ep, (const T *)nullptr, (int64_t)0,
+                           ChebEpi<T>{}, (const uint32_t *)nullptr, (const uint32_t *)nullptr, 0);
     }
     enqueue_finalize(S, S->stream, stencil_tiles(S, LS, sizeof(T) == 8).n + LS.fix_grid, S->ncols, fin, 1);
 }
@@ -5265,6 +5347,34 @@ void launch_coarse_cheb(vfem_system *S, int l, const T *x, T *xout, const T *r, 
                        (T *)nullptr, S->st, ChebEpi<T>{r, dinv_ref<T>(S, l), dprev, dout, xout, a, b, first});
 }
 
+// One Chebyshev step on a stencil-form coarse level: the row kernel (with the
+// partial-element blocks) writes A x into xout, then the surface pass takes
+// the step at its nodes and, in extra blocks, at every other DOF, writing
+// x + d over A x (ChebEpi semantics as in launch_coarse_cheb)
+template <typename T>
+void launch_stencil_cheb(vfem_system *S, int l, const T *x, T *xout, const T *r, const T *dprev, T *dout, double a,
+                         double b, int first)
+{
+    const DevLevel &L = S->dl[l];
+    const LevelStencil &LS = S->stl[l];
+    const PartialJob<T> job{LS.part, tabi<T>(S, l), ysbuf<T>(S, l), S->ysz[l]};
+    const PartialJob<T> *pj = LS.part.n > 0 ? &job : nullptr;
+    bool part_done;
+    if (LS.zc == kStencilZShort)
+        part_done = launch_stencil_zc<T, kStencilZShort>(S, L, LS, x, xout, -1, 1, SmoothEpi<T>{}, pj);
+    else part_done = launch_stencil_zc<T, kStencilZ>(S, L, LS, x, xout, -1, 1, SmoothEpi<T>{}, pj);
+    if (LS.part.n > 0 && !part_done)
+        hipLaunchKernelGGL(k_partial_elems<T>, dim3((int)((8 * LS.part.n + kBlock - 1) / kBlock), S->ncols), dim3(kBlock),
+                           0, S->stream, job, x, L.nn, S->st, 1);
+    const int64_t n3 = 3 * L.nn;
+    const int cg = (int)std::min<int64_t>(4096, (n3 + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_stencil_fix<T, true, false, true>), dim3(LS.fix_grid + cg, S->ncols), dim3(kBlock), 0,
+                       S->stream, L, ktab<T>(S, l), LS.surf, x, xout, S->st, S->red, -1, 1, SmoothEpi<T>{},
+                       (const T *)ysbuf<T>(S, l), S->ysz[l],
+                       ChebEpi<T>{r, dinv_ref<T>(S, l), dprev, dout, xout, a, b, first}, (const uint32_t *)S->fixc[l],
+                       (const uint32_t *)S->smask[l], LS.fix_grid);
+}
+
 // The explicit inverse is always applied in FP64: its entries carry the coarse
 // matrix's condition number, so an FP32 copy would perturb every reduced-precision
 // cycle by ~cond(Kc) * 2^-24 (the vectors may be FP32; products accumulate in FP64).
@@ -5350,14 +5460,22 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     // xcur holds the smoothed x, tcur the restriction's A x.
     T *xcur = zl, *tcur = tl;
     bool fz = false;
-    if constexpr (std::is_same<T, TR>::value) fz = cheb && cheb_fused(S, l);
+    // a stencil-form coarse level takes the step in its surface pass's launch
+    const bool sten = l > 0 && level_stencil_on(S, l) && S->smask[l];
+    if constexpr (std::is_same<T, TR>::value) fz = cheb && (cheb_fused(S, l) || sten);
+    auto cheb_step = [&](const T *xx, T *xo, const T *dprev, T *dout, double a, double b, int first) {
+        if constexpr (std::is_same<T, TR>::value) {
+            if (sten) launch_stencil_cheb<T>(S, l, xx, xo, rl, dprev, dout, a, b, first);
+            else launch_coarse_cheb<T>(S, l, xx, xo, rl, dprev, dout, a, b, first);
+        }
+    };
     if (fz) {
         if constexpr (std::is_same<T, TR>::value) {
             T *xa = zl, *xb = tl;
             for (int k = 1; k < S->cheb_degree; k++) {
                 const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
-                launch_coarse_cheb<T>(S, l, xa, xb, rl, k == 1 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
-                                      k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, k == 1 ? 1 : 0);
+                cheb_step(xa, xb, k == 1 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
+                          k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, k == 1 ? 1 : 0);
                 std::swap(xa, xb);
             }
             launch_level_matvec<T>(S, l, xa, xb);
@@ -5438,8 +5556,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
             T *xa = zl, *xb = tl;
             for (int k = 0; k < S->cheb_degree; k++) {
                 const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
-                launch_coarse_cheb<T>(S, l, xa, xb, rl, k == 0 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
-                                      k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, 0);
+                cheb_step(xa, xb, k == 0 ? (const T *)nullptr : (const T *)level_d<T>(S, l),
+                          k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, 0);
                 std::swap(xa, xb);
             }
             if (xa != zl)  // an odd number of steps ends in tl
