@@ -1468,7 +1468,8 @@ template <typename T> struct ChebEpi {
 // The Chebyshev step of a stencil level at every DOF outside the surface list
 // (smask: compact-order bitmask of the surface nodes, whose step the surface
 // pass takes), after the row kernel wrote t = A x into cep.xout: the same
-// operations as k_cheb_step, x_out = x + d written over t at the same index.
+// operations as the element-form epilogue (k_matvec_coarse), x_out = x + d
+// written over t at the same index.
 // Blocks b of G, flat over the level's n3 DOFs.
 template <typename T>
 __device__ __forceinline__ void cheb_rest(const ChebEpi<T> &cep, const T *__restrict__ x,
@@ -3567,65 +3568,6 @@ __global__ __launch_bounds__(kBlock) void k_smooth(int64_t n3, const uint32_t *_
     if (fin >= 0) block_partials2(dacc, dacc2, red);
 }
 
-// One Chebyshev smoothing step on a level, flat over its DOFs: d = a d_prev +
-// b D^-1 (r - t) (t = A x), x += d, where d_prev is d, or x itself when `first`
-// (the step after a start from zero, whose first direction is the x it wrote);
-// d is stored only when another step follows.  Fixed rows stay zero.
-template <typename Tr, typename Td, typename Tz>
-__global__ __launch_bounds__(kBlock) void k_cheb_step(int64_t n3, const uint32_t *__restrict__ fixc,
-                                                       const Tr *__restrict__ r, const Tz *__restrict__ t,
-                                                       DinvRef<Td> D, Tz *__restrict__ x, Tz *__restrict__ d,
-                                                       double a, double b, int first, int write_d, CGState *st)
-{
-    {
-        const int64_t co = col_offset(n3);
-        r += co;
-        t += co;
-        x += co;
-        d += co;
-        st = col_state(st);
-    }
-    if (st->done) return;
-    const Range R = flat_range(n3);
-    const Tz ta = Tz(a), tb = Tz(b);
-    for (int64_t k0 = R.begin + threadIdx.x; k0 < R.end; k0 += kBlock * FLAT_U) {
-        Tr rv[FLAT_U];
-        Td dv[FLAT_U];
-        Tz xv[FLAT_U], tv[FLAT_U], pv[FLAT_U];
-        uint32_t fw[FLAT_U], cw[FLAT_U];
-#pragma unroll
-        for (int u = 0; u < FLAT_U; u++) {
-            const uint32_t nd = (uint32_t)min(k0 + u * kBlock, R.end - 1) / 3u;
-            fw[u] = fixc[nd >> 5];
-            cw[u] = D.flat ? D.flat[nd >> 5] : 0u;
-        }
-#pragma unroll
-        for (int u = 0; u < FLAT_U; u++) {
-            const int64_t k = min(k0 + u * kBlock, R.end - 1);
-            rv[u] = r[k];
-            tv[u] = t[k];
-            xv[u] = x[k];
-            pv[u] = (a != 0 && !first) ? d[k] : Tz(0);
-        }
-#pragma unroll
-        for (int u = 0; u < FLAT_U; u++) {
-            const uint32_t k = (uint32_t)min(k0 + u * kBlock, R.end - 1), nd = k / 3u;
-            dv[u] = dinv_at(D, (cw[u] >> (nd & 31)) & 1, k, nd);
-        }
-#pragma unroll
-        for (int u = 0; u < FLAT_U; u++) {
-            const int64_t k = k0 + u * kBlock;
-            if (k >= R.end) break;
-            const bool fx = (fw[u] >> (((uint32_t)k / 3u) & 31)) & 1;
-            const Tz dp = first ? xv[u] : pv[u];
-            Tz dn = ta * dp + tb * Tz(dv[u]) * (Tz(rv[u]) - tv[u]);
-            if (fx) dn = 0;
-            x[k] = xv[u] + dn;
-            if (write_d) d[k] = dn;
-        }
-    }
-}
-
 // Coarsest level: z[freedof[i]] = sum_k Kc^-1[i][k] r[freedof[k]] (replaces the
 // host SimplicialLDLT solve of FEMGPU.cpp:509-521).  One wavefront per row.
 template <typename Tk, typename Tv>
@@ -5461,6 +5403,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
     T *xcur = zl, *tcur = tl;
     bool fz = false;
     // a stencil-form coarse level takes the step in its surface pass's launch
+    // (Chebyshev smoothing runs only on coarse levels, whose vectors and residual
+    // share one type, and every coarse level takes one of the two fused forms)
     const bool sten = l > 0 && level_stencil_on(S, l) && S->smask[l];
     if constexpr (std::is_same<T, TR>::value) fz = cheb && (cheb_fused(S, l) || sten);
     auto cheb_step = [&](const T *xx, T *xo, const T *dprev, T *dout, double a, double b, int first) {
@@ -5484,13 +5428,6 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         }
     } else {
         launch_level_matvec<T>(S, l, zl, tl);
-        for (int k = 1; cheb && k < S->cheb_degree; k++) {
-            const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
-            hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream,
-                               n3l, S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b,
-                               k == 1 ? 1 : 0, k + 1 < S->cheb_degree ? 1 : 0, S->st);
-            launch_level_matvec<T>(S, l, zl, tl);
-        }
     }
     const bool tiled = S->tiled[l] && (S->l0_kind == VFEM_L0_STENCIL || S->l0_kind == VFEM_L0_STENCIL_DIRECT ||
                                        S->l0_kind == VFEM_L0_STENCIL_PLANES ||
@@ -5563,16 +5500,6 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
             if (xa != zl)  // an odd number of steps ends in tl
                 hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
                                    (const T *)xa, zl);
-        }
-        return;
-    }
-    if (cheb) {  // post-smoothing: all steps from the corrected x (a coarse level: no reduction)
-        for (int k = 0; k < S->cheb_degree; k++) {
-            const ChebCoef c = cheb_coef(S->lmax[l], S->cheb_ratio, k);
-            launch_level_matvec<T>(S, l, zl, tl);
-            hipLaunchKernelGGL((k_cheb_step<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream,
-                               n3l, S->fixc[l], rl, (const T *)tl, dinv_ref<T>(S, l), zl, level_d<T>(S, l), c.a, c.b, 0,
-                               k + 1 < S->cheb_degree ? 1 : 0, S->st);
         }
         return;
     }
