@@ -1465,6 +1465,25 @@ template <typename T> struct ChebEpi {
     int first;
 };
 
+// x_out at one DOF (dv = D^-1, t = (A x) there, fx: fixed row), d into dn.  A
+// plain damped-Jacobi step (a = 0, not first, no d kept) is evaluated as
+// k_smooth evaluates z + omega D^-1 (r - t), fixed rows 0, so the fused and the
+// separate post-smoothing give the same bits.
+template <typename T>
+__device__ __forceinline__ T cheb_xout(const ChebEpi<T> &ep, T xv, T dp, T dv, T rv, T t, bool fx, T &dn)
+{
+    const T ta = T(ep.a), tb = T(ep.b);
+    if (ep.a == 0 && !ep.first && !ep.dout) {
+        T v = xv + tb * dv * (rv - t);
+        if (fx) v = 0;
+        dn = 0;
+        return v;
+    }
+    dn = ta * dp + tb * dv * (rv - t);
+    if (fx) dn = 0;
+    return xv + dn;
+}
+
 // The Chebyshev step of a stencil level at every DOF outside the surface list
 // (smask: compact-order bitmask of the surface nodes, whose step the surface
 // pass takes), after the row kernel wrote t = A x into cep.xout: the same
@@ -1476,17 +1495,17 @@ __device__ __forceinline__ void cheb_rest(const ChebEpi<T> &cep, const T *__rest
                                           const uint32_t *__restrict__ fixc, const uint32_t *__restrict__ smask,
                                           int64_t n3, int b, int G)
 {
-    const T ta = T(cep.a), tb = T(cep.b);
     for (int64_t k = (int64_t)b * kBlock + threadIdx.x; k < n3; k += (int64_t)G * kBlock) {
         const uint32_t nd = (uint32_t)(k / 3);
         const uint32_t sw = smask[nd >> 5], fw = fixc[nd >> 5];
         const uint32_t cw = cep.D.flat ? cep.D.flat[nd >> 5] : 0u;
         const T t = cep.xout[k], xv = x[k], rv = cep.r[k];
         const T dp = cep.first ? xv : (cep.a != 0 ? cep.dprev[k] : T(0));
-        T dn = ta * dp + tb * dinv_at(cep.D, (cw >> (nd & 31)) & 1, (uint32_t)k, nd) * (rv - t);
-        if ((fw >> (nd & 31)) & 1) dn = 0;
+        T dn;
+        const T v = cheb_xout(cep, xv, dp, dinv_at(cep.D, (cw >> (nd & 31)) & 1, (uint32_t)k, nd), rv, t,
+                              (fw >> (nd & 31)) & 1, dn);
         if (!((sw >> (nd & 31)) & 1)) {
-            cep.xout[k] = xv + dn;
+            cep.xout[k] = v;
             if (cep.dout) cep.dout[k] = dn;
         }
     }
@@ -1644,13 +1663,13 @@ __global__ __launch_bounds__(kBlock) void k_stencil_fix(DevLevel L, const T *__r
             const int64_t q3 = 3 * (int64_t)e.x;
             const uint32_t nd = (uint32_t)e.x;
             const bool fl = cep.D.flat && ((cep.D.flat[nd >> 5] >> (nd & 31)) & 1);
-            const T ta = T(cep.a), tb = T(cep.b), cc[3] = {c0, c1, c2};
+            const T cc[3] = {c0, c1, c2};
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const int64_t k = q3 + c;
                 const T dp = cep.first ? ctr[c] : (cep.a != 0 ? cep.dprev[k] : T(0));
-                const T dn = ta * dp + tb * dinv_at(cep.D, fl, (uint32_t)k, nd) * (cep.r[k] - cc[c]);
-                cep.xout[k] = ctr[c] + dn;
+                T dn;
+                cep.xout[k] = cheb_xout(cep, ctr[c], dp, dinv_at(cep.D, fl, (uint32_t)k, nd), cep.r[k], cc[c], false, dn);
                 if (cep.dout) cep.dout[k] = dn;
             }
         } else if (act && EPI) {
@@ -2530,15 +2549,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_coarse(DevLevel L, const T *_
             if (ep.xout) {
                 const uint32_t nd = (uint32_t)id;
                 const bool flat = ep.D.flat && ((ep.D.flat[nd >> 5] >> (nd & 31)) & 1);
-                const T ta = T(ep.a), tb = T(ep.b);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
                     const uint32_t k = 3u * nd + c;
                     const T xv = x[k];
                     const T dp = ep.first ? xv : (ep.a != 0 ? ep.dprev[k] : T(0));
-                    T dn = ta * dp + tb * dinv_at(ep.D, flat, k, nd) * (ep.r[k] - r[c]);
-                    if (fx) dn = 0;
-                    ep.xout[k] = xv + dn;
+                    T dn;
+                    ep.xout[k] = cheb_xout(ep, xv, dp, dinv_at(ep.D, flat, k, nd), ep.r[k], r[c], fx, dn);
                     if (ep.dout) ep.dout[k] = dn;
                 }
             } else {
@@ -4201,6 +4218,7 @@ BlockGrid block_grid(const DevLevel &L, const BlockCfg &c)
 
 constexpr int64_t kStencilMinNodes = 100000;  // smallest coarse level given the stencil form
 constexpr int64_t kTiledMinNodes = 500000;     // smallest fine level given the tiled transfers
+constexpr int64_t kFusedJacobiMaxDofs = 2000000;  // largest stencil level whose Jacobi step is fused
 // planes per block of the tiled transfers: short chunks give every level enough
 // blocks (level 1 at 256^3: 704 restriction blocks) and, measured at 256^3, were
 // no slower on level 0 than 4x longer ones (mixed solve 58.8 against 59.4 ms with
@@ -4231,6 +4249,15 @@ struct LevelStencil {
 
 // captured-iteration cache key: the plan's discrete choices and omega's bits
 using GraphKey = std::tuple<int64_t, uint64_t, uint64_t>;
+
+// host-to-device copy ordered on `s` and waited for: the library's streams are
+// non-blocking, so a copy on the null stream (hipMemcpy) is not ordered before
+// their kernels
+inline hipError_t copy_to_device(hipStream_t s, void *dst, const void *src, size_t bytes)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
 
 struct vfem_system {
     Hierarchy H;
@@ -4317,12 +4344,20 @@ struct vfem_system {
     int ncols = 1;              // columns of the launches being enqueued (1, or a batch)
     std::vector<void *> vec_allocs;
 
+    // zero: cleared on the library stream and waited for.  `stream` is a
+    // non-blocking stream, which a fill on the null stream (hipMemset) does not
+    // order: a kernel on `stream` could otherwise run before the fill ends and
+    // have its writes cleared.
     template <typename T> int alloc(T **p, size_t n, bool zero = true, std::vector<void *> *list = nullptr)
     {
         void *q = nullptr;
-        HIP_TRY(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        HIP_TRY(hipMalloc(&q, bytes));
         (list ? list : &allocs)->push_back(q);
-        if (zero) HIP_TRY(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(T)));
+        if (zero) {
+            HIP_TRY(hipMemsetAsync(q, 0, bytes, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
         *p = (T *)q;
         return 0;
     }
@@ -4330,7 +4365,7 @@ struct vfem_system {
     {
         int rc = alloc(p, v.size(), false);
         if (rc) return rc;
-        if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        if (!v.empty()) HIP_TRY(copy_to_device(stream, *p, v.data(), v.size() * sizeof(T)));
         return 0;
     }
     ~vfem_system()
@@ -5361,15 +5396,18 @@ template <typename T> void enqueue_vcycle_ref(vfem_system *S, const double *r, T
 }
 
 // Damped multiplicative V(1,1) cycle, level l >= 0.  rl: level residual
-// (level 0: FP64 CG residual), zl: output, tl: operator scratch.
+// (level 0: FP64 CG residual), zl and tl: the level's two vectors.  Returns the
+// one that holds the level's correction: zl on level 0; on a coarse level the
+// smoothing steps fused into the level products alternate between zl and tl,
+// and the caller's prolongation reads the result where it ended.
 template <typename T, typename TR, typename TN = T>
-void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin,
-                                 const T *zold)
+const T *enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *tl, double omega, int fin,
+                                     const T *zold)
 {
     const int nl = S->H.nlev;
     if (l == nl - 1) {
         launch_coarse_solve<T>(S, (const T *)rl, zl, true);
-        return;
+        return zl;
     }
     if (S->unet_on && l == S->unet_level) {
         // learned coarse correction: z_l = s N(r_l / s) in place of the cycle from level l down
@@ -5382,7 +5420,7 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         unet_forward_enqueue(S->unet, S->stream);
         hipLaunchKernelGGL((k_unet_out<T>), dim3((V + kBlock - 1) / kBlock, S->ncols), dim3(kBlock), 0, S->stream, L,
                            unet_output(S->unet), S->unet_scale, zl, S->st);
-        return;
+        return zl;
     }
     const DevLevel &L = S->dl[l], &C = S->dl[l + 1];
     const int64_t n3l = 3 * L.nn;
@@ -5456,8 +5494,8 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                            (const T *)tcur, level_r<TN>(S, l + 1), 1, S->st, ps);
     }
     if (pre) S->pre_level = l + 1;
-    enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1), level_t<TN>(S, l + 1),
-                                        omega, -1, (const TN *)nullptr);
+    const TN *zc = enqueue_vcycle_damped_level<TN, TN>(S, l + 1, level_r<TN>(S, l + 1), level_z<TN>(S, l + 1),
+                                                       level_t<TN>(S, l + 1), omega, -1, (const TN *)nullptr);
     // Level 0 with the sum-factorised stencil: the prolongation writes z + P z_c
     // to t and the operator applied to it finishes the cycle with the
     // post-smoothing fused in (z = t + omega D^-1 (r - A t)), saving the separate
@@ -5471,21 +5509,19 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
         fuse = l == 0 && tiled && level_stencil_on(S, 0) && use_rows(S, S->stl[0], false);
     if (tiled && C.div == 4) {
         hipLaunchKernelGGL((k_prolong4_tile<TN, T, kProlong4Z>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0,
-                           S->stream, L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1, S->st,
-                           S->ptiles[l]);
+                           S->stream, L, C, zc, (const T *)xcur, fuse ? tl : zl, 1, S->st, S->ptiles[l]);
     } else if (tiled) {
         hipLaunchKernelGGL((k_prolong_tile<TN, T, kProlongZ>), dim3(S->ptiles[l].n, S->ncols), dim3(kBlock), 0, S->stream,
-                           L, C, (const TN *)level_z<TN>(S, l + 1), (const T *)xcur, fuse ? tl : zl, 1, S->st,
-                           S->ptiles[l]);
+                           L, C, zc, (const T *)xcur, fuse ? tl : zl, 1, S->st, S->ptiles[l]);
     } else {
         hipLaunchKernelGGL((k_prolong<TN, TR, T, T>), dim3(items_grid(L), S->ncols), dim3(kBlock), 0, S->stream, L, C,
-                           (const TN *)level_z<TN>(S, l + 1), rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red,
-                           -1, (const T *)nullptr, xcur != zl ? (const T *)xcur : (const T *)nullptr);
+                           zc, rl, dinv<T>(S, l), zl, (int)PROLONG_ADD, 1, S->st, S->red, -1, (const T *)nullptr,
+                           xcur != zl ? (const T *)xcur : (const T *)nullptr);
     }
     if constexpr (std::is_same<TR, double>::value) {
         if (fuse) {
             launch_stencil_smooth<T>(S, tl, zl, (const double *)rl, zold, omega, fin);
-            return;
+            return zl;
         }
     }
     if (fz) {  // post-smoothing fused into the products, x alternating between zl and tl
@@ -5497,16 +5533,26 @@ void enqueue_vcycle_damped_level(vfem_system *S, int l, const TR *rl, T *zl, T *
                           k + 1 < S->cheb_degree ? level_d<T>(S, l) : nullptr, c.a, c.b, 0);
                 std::swap(xa, xb);
             }
-            if (xa != zl)  // an odd number of steps ends in tl
-                hipLaunchKernelGGL((k_convert<T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l,
-                                   (const T *)xa, zl);
+            return xa;  // tl after an odd number of steps
         }
-        return;
+        return zl;
+    }
+    // a coarse level's damped-Jacobi post-smoothing z + omega D^-1 (r - A z): the
+    // Chebyshev form with a = 0, b = omega, fused the same way, ending in tl.  On
+    // a large stencil level the separate streaming k_smooth is faster than the
+    // step in the surface pass's extra blocks (256^3 level 1, 4.3M DOFs: the
+    // mixed solve 55.2 against 55.5 ms), so there it stays separate.
+    if constexpr (std::is_same<T, TR>::value) {
+        if (l > 0 && fin < 0 && (!sten || n3l <= kFusedJacobiMaxDofs)) {
+            cheb_step(zl, tl, (const T *)nullptr, (T *)nullptr, 0.0, omega, 0);
+            return tl;
+        }
     }
     launch_level_matvec<T>(S, l, zl, tl);
     hipLaunchKernelGGL((k_smooth<TR, T, T>), dim3(grid_for_flat(n3l), S->ncols), dim3(kBlock), 0, S->stream, n3l, S->fixc[l], rl,
                        (const T *)tl, dinv_ref<T>(S, l), zl, omega, (int)SMOOTH_POST, S->st, S->red, fin, zold);
     enqueue_finalize(S, S->stream, grid_for_flat(n3l), S->ncols, fin, 1);
+    return zl;
 }
 
 struct Plan {
@@ -6284,7 +6330,7 @@ int vfem_set_rhs(vfem_system *sys, const double *f)
         g_last_error = "null pointer";
         return VFEM_ERR_ARGUMENT;
     }
-    HIP_TRY(hipMemcpy(sys->f, f, sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->f, f, sys->n3 * sizeof(double)));
     return VFEM_OK;
 }
 
@@ -6341,7 +6387,7 @@ int vfem_solve(vfem_system *sys, const double *f, double *u, const vfem_options 
     vfem_default_options(&o);
     if (opts) o = *opts;
     if (int rc = vfem_set_rhs(sys, f)) return rc;
-    if (o.warm_start) HIP_TRY(hipMemcpy(sys->x, u, sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    if (o.warm_start) HIP_TRY(copy_to_device(sys->stream, sys->x, u, sys->n3 * sizeof(double)));
     int rc = solve_resident(sys, o, stats, o.warm_start != 0);
     if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED) return rc;
     int rc2 = vfem_get_solution(sys, u);
@@ -6356,7 +6402,7 @@ int vfem_set_rhs_block(vfem_system *sys, const double *f, int32_t nrhs)
     }
     if (nrhs > sys->nb)
         if (int rc = alloc_vectors(sys, nrhs)) return rc;
-    HIP_TRY(hipMemcpy(sys->f, f, (size_t)nrhs * sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->f, f, (size_t)nrhs * sys->n3 * sizeof(double)));
     return VFEM_OK;
 }
 
@@ -6404,7 +6450,7 @@ int vfem_solve_batched(vfem_system *sys, const double *f, double *u, int32_t nrh
         const int64_t off = (int64_t)c0 * sys->n3;
         if (int rc = vfem_set_rhs_block(sys, f + off, nc)) return rc;
         if (o.warm_start)
-            HIP_TRY(hipMemcpy(sys->x, u + off, (size_t)nc * sys->n3 * sizeof(double), hipMemcpyHostToDevice));
+            HIP_TRY(copy_to_device(sys->stream, sys->x, u + off, (size_t)nc * sys->n3 * sizeof(double)));
         int rc = vfem_solve_resident_block(sys, nc, &o, stats ? stats + c0 : nullptr);
         if (rc != VFEM_OK && rc != VFEM_ERR_NOT_CONVERGED && rc != VFEM_ERR_BREAKDOWN) return rc;
         if (rc != VFEM_OK && (worst == VFEM_OK || rc == VFEM_ERR_BREAKDOWN)) worst = rc;
@@ -6501,7 +6547,7 @@ int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, 
     if (int rc = check_plan(sys, o, P)) return rc;
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
-    HIP_TRY(hipMemcpy(sys->f, f, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->f, f, n3 * sizeof(double)));
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, n_iter + 1, 0.0, 0, 1);
     hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->f, (const double *)nullptr,
                        sys->r, sys->x, sys->st, sys->red, (int)FIN_RHS);
@@ -6521,10 +6567,11 @@ int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, 
         if (h.done) break;
         // the residual the last cycle restricted to `level`, and the cycle's own correction there
         HIP_TRY(hipMemcpy(r.data(), sys->rl64[level], n3l * sizeof(double), hipMemcpyDeviceToHost));
-        enqueue_vcycle_damped_level<double, double>(sys, level, sys->rl64[level], sys->zl64[level],
-                                                    sys->tl64[level], o.omega, -1, (const double *)nullptr);
+        const double *zl = enqueue_vcycle_damped_level<double, double>(sys, level, sys->rl64[level], sys->zl64[level],
+                                                                       sys->tl64[level], o.omega, -1,
+                                                                       (const double *)nullptr);
         HIP_TRY(hipStreamSynchronize(sys->stream));
-        HIP_TRY(hipMemcpy(zc.data(), sys->zl64[level], n3l * sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(zc.data(), zl, n3l * sizeof(double), hipMemcpyDeviceToHost));
         double s2 = 0;
         for (int64_t i = 0; i < nf; i++) s2 += r[freedof[i]] * r[freedof[i]];
         if (!(s2 > 0)) continue;
@@ -6573,7 +6620,7 @@ int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs
     } guard{sys};
     sys->ncols = nrhs;
     const int gf = grid_for_flat(n3);
-    HIP_TRY(hipMemcpy(sys->t64, x, nall * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->t64, x, nall * sizeof(double)));
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs);
     if (precision == VFEM_FP64) {
         launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
@@ -6751,7 +6798,7 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
     }
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
-    HIP_TRY(hipMemcpy(sys->t64, x, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->t64, x, n3 * sizeof(double)));
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
     if (precision == VFEM_FP64 || precision == VFEM_MIXED) {
         sys->l0_mfma = precision == VFEM_MIXED;
@@ -6782,7 +6829,7 @@ int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_o
     if (int rc = check_plan(sys, o, P)) return rc;
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
-    HIP_TRY(hipMemcpy(sys->r, r, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->r, r, n3 * sizeof(double)));
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
     enqueue_precond(sys, P, sys->r, FIN_STORE);
     if (P.precision != VFEM_FP64 && sys->H.nlev > 1)
@@ -6801,8 +6848,8 @@ int vfem_dot(vfem_system *sys, const double *a, const double *b, double *out)
         return VFEM_ERR_ARGUMENT;
     }
     const int64_t n3 = sys->n3;
-    HIP_TRY(hipMemcpy(sys->t64, a, n3 * sizeof(double), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(sys->ap64, b, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(copy_to_device(sys->stream, sys->t64, a, n3 * sizeof(double)));
+    HIP_TRY(copy_to_device(sys->stream, sys->ap64, b, n3 * sizeof(double)));
     hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, sys->stream, n3, sys->t64,
                        sys->ap64, sys->st, sys->red, (int)FIN_STORE, 0);
     enqueue_finalize(sys, sys->stream, grid_for_flat(n3), 1, FIN_STORE, 0);

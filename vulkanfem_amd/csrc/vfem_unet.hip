@@ -613,6 +613,15 @@ int64_t vfem_unet_num_params(void)
     return s;
 }
 
+// host-to-device copy ordered on the network's stream and waited for: that
+// stream is non-blocking, so a copy on the null stream would not be ordered
+// before its kernels
+static hipError_t to_device(hipStream_t s, void *dst, const void *src, size_t bytes)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 int vfem_unet_create(uint64_t seed, vfem_unet **out)
 {
     if (!out) return VFEM_ERR_ARGUMENT;
@@ -636,8 +645,11 @@ int vfem_unet_create(uint64_t seed, vfem_unet **out)
         std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / ((d.cin0 + d.cin1) * d.k * d.k * d.k)));
         std::vector<float> h(N->n[l]);
         for (auto &x : h) x = nd(rng);
-        if (hipMemcpy(N->w[l], h.data(), b4, hipMemcpyHostToDevice) != hipSuccess) return fail(VFEM_ERR_DEVICE);
-        if (hipMemset(N->m[l], 0, b4) != hipSuccess || hipMemset(N->v[l], 0, b4) != hipSuccess)
+        if (to_device(N->stream, N->w[l], h.data(), b4) != hipSuccess) return fail(VFEM_ERR_DEVICE);
+        // on the network's own (non-blocking) stream, which a null-stream fill
+        // would not order before the training kernels
+        if (hipMemsetAsync(N->m[l], 0, b4, N->stream) != hipSuccess ||
+            hipMemsetAsync(N->v[l], 0, b4, N->stream) != hipSuccess)
             return fail(VFEM_ERR_DEVICE);
     }
     if (N->alloc((void **)&N->loss, 4096 * sizeof(float))) return fail(VFEM_ERR_DEVICE);
@@ -652,6 +664,7 @@ void vfem_unet_destroy(vfem_unet *net) { delete net; }
 int vfem_unet_get_weights(vfem_unet *net, float *w)
 {
     if (!net || !w) return VFEM_ERR_ARGUMENT;
+    U_TRY(hipStreamSynchronize(net->stream));  // the copies below run on the null stream
     for (int l = 0; l < kNumLayers; l++) {
         U_TRY(hipMemcpy(w, net->w[l], net->n[l] * sizeof(float), hipMemcpyDeviceToHost));
         w += net->n[l];
@@ -663,7 +676,7 @@ int vfem_unet_set_weights(vfem_unet *net, const float *w)
 {
     if (!net || !w) return VFEM_ERR_ARGUMENT;
     for (int l = 0; l < kNumLayers; l++) {
-        U_TRY(hipMemcpy(net->w[l], w, net->n[l] * sizeof(float), hipMemcpyHostToDevice));
+        U_TRY(to_device(net->stream, net->w[l], w, net->n[l] * sizeof(float)));
         w += net->n[l];
     }
     pack_all(net);
@@ -677,7 +690,7 @@ int vfem_unet_forward(vfem_unet *net, const float *x, int32_t batch, int32_t d, 
     if (int rc = prepare(net, 2, batch, d, h, w, true)) return rc;
     Acts &A = net->acts[2];
     const int64_t nin = (int64_t)batch * 4 * A.dim[0].v(), nout = (int64_t)batch * 3 * A.dim[0].v();
-    U_TRY(hipMemcpy(A.x, x, nin * sizeof(float), hipMemcpyHostToDevice));
+    U_TRY(to_device(net->stream, A.x, x, nin * sizeof(float)));
     forward(net, A);
     U_TRY(hipStreamSynchronize(net->stream));
     U_TRY(hipMemcpy(out, A.a[OUT], nout * sizeof(float), hipMemcpyDeviceToHost));
@@ -690,7 +703,7 @@ int vfem_unet_backward(vfem_unet *net, const float *gout, float *gw, float *gx)
     Acts &A = net->acts[2];
     if (!A.grads) return VFEM_ERR_ARGUMENT;
     const int64_t nout = (int64_t)A.B * 3 * A.dim[0].v(), nin = (int64_t)A.B * 4 * A.dim[0].v();
-    U_TRY(hipMemcpy(A.ga[OUT], gout, nout * sizeof(float), hipMemcpyHostToDevice));
+    U_TRY(to_device(net->stream, A.ga[OUT], gout, nout * sizeof(float)));
     backward(net, A, gx != nullptr);
     U_TRY(hipStreamSynchronize(net->stream));
     for (int l = 0; l < kNumLayers; l++) {
@@ -729,9 +742,9 @@ int vfem_unet_train(vfem_unet *net, const float *x, const float *y, const float 
         u_err = "out of device memory";
         return VFEM_ERR_DEVICE;
     }
-    if (hipMemcpy(dx, x, n_samples * 4 * V * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dy, y, n_samples * 3 * V * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(dm, mask, n_samples * V * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    if (to_device(net->stream, dx, x, n_samples * 4 * V * sizeof(float)) != hipSuccess ||
+        to_device(net->stream, dy, y, n_samples * 3 * V * sizeof(float)) != hipSuccess ||
+        to_device(net->stream, dm, mask, n_samples * V * sizeof(float)) != hipSuccess) {
         cleanup();
         return VFEM_ERR_DEVICE;
     }
@@ -743,7 +756,7 @@ int vfem_unet_train(vfem_unet *net, const float *x, const float *y, const float 
     for (int e = 0; e < epochs; e++) {
         for (int64_t i = 0; i < n_samples; i++) perm[i] = (int32_t)i;
         std::shuffle(perm.begin(), perm.end(), rng);
-        if (hipMemcpy(didx, perm.data(), n_samples * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        if (to_device(net->stream, didx, perm.data(), n_samples * sizeof(int32_t)) != hipSuccess) {
             cleanup();
             return VFEM_ERR_DEVICE;
         }
