@@ -113,6 +113,29 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+
+// The x work of the p update that follows a z.r finalizer (the column goes on):
+// the residual update left alpha p (x_lag bit 0); with pairing on, the p update
+// keeps p in p_prev and defers the term, and the next one applies both, so x is
+// read and written every other iteration.  The flags describe x after that
+// p update (which reads xmode only), so no kernel changes what it reads.
+__device__ __forceinline__ void plan_x_update(CGState *st)
+{
+    if (!(st->x_lag & 1)) {
+        st->xmode = XM_NONE;
+    } else if (st->x_lag & 2) {
+        st->xmode = XM_PAIR;
+        st->x_lag = 0;
+    } else if (st->xpair) {
+        st->xmode = XM_STASH;
+        st->alpha_prev = st->alpha;
+        st->x_lag = 2;
+    } else {
+        st->xmode = XM_SINGLE;
+        st->x_lag = 0;
+    }
+}
+
 __device__ void finalize(CGState *st, int fin, double v, double v2)
 {
     if (st->no_stop) {
@@ -135,7 +158,7 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
     case FIN_RR_LAZY:
         st->rr = v;
         st->rzx = v2;  // r_{k+1}.z_k when the update reduced it (k_cg_update_smooth), else 0
-        st->x_lag = fin == FIN_RR_LAZY;
+        st->x_lag = (st->x_lag & 2) | (fin == FIN_RR_LAZY ? 1 : 0);
         st->last_true = 0;
         if (!isfinite(v)) st->done = 3;
         else if (v < st->threshold) st->done = 1;
@@ -144,6 +167,7 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         st->absNew = v;
         st->beta = 0;
         if (!isfinite(v)) st->done = 3;
+        st->xmode = XM_NONE;
         break;
     case FIN_RZ:
         st->absOld = st->absNew;
@@ -152,6 +176,7 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         st->iter += 1;
         if (!isfinite(st->beta)) st->done = 3;
         else if (st->iter >= st->maxit) st->done = 2;
+        if (!st->done) plan_x_update(st);
         break;
     case FIN_RHS: {
         st->rhs2 = v;
@@ -170,11 +195,12 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         st->iter += 1;
         if (!isfinite(st->beta)) st->done = 3;
         else if (st->iter >= st->maxit) st->done = 2;
+        if (!st->done) plan_x_update(st);
         break;
     case FIN_RZX:
         // a column whose x lagged was stopped before this true residual reopened
         // it: its z_k is gone, so it keeps the r_{k+1}.z_k its last update reduced
-        if (!st->x_lag) st->rzx = v;
+        if (!(st->x_lag & 1)) st->rzx = v;
         break;
     case FIN_RZ_XFLEX:
         st->absOld = st->absNew;
@@ -183,6 +209,7 @@ __device__ void finalize(CGState *st, int fin, double v, double v2)
         st->iter += 1;
         if (!isfinite(st->beta)) st->done = 3;
         else if (st->iter >= st->maxit) st->done = 2;
+        if (!st->done) plan_x_update(st);
         break;
     case FIN_RR_TRUE:
         st->rr = v;
@@ -3735,11 +3762,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_smooth(int64_t n, double *
 }
 
 // ConjugateGradient_2.comp: p = z + beta p
-// (flexible CG: also keeps z for the next Polak-Ribiere beta; x non-null: first
-// x += alpha p with the outgoing p, the update k_cg_update_smooth left)
+// (flexible CG: also keeps z for the next Polak-Ribiere beta; x non-null: the
+// x work the finalizer before this launch chose (CGState::xmode) with the
+// outgoing p: x += alpha p, or p_prev = p with the term deferred, or the
+// deferred and the current term at once, x += alpha_prev p_prev + alpha p)
 template <typename Tz, typename Tp>
 __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__restrict__ z, Tp *__restrict__ p,
-                                                        Tz *__restrict__ zold, CGState *st, double *__restrict__ x)
+                                                        Tz *__restrict__ zold, CGState *st, double *__restrict__ x,
+                                                        Tp *__restrict__ pprev)
 {
     {
         const int64_t co = col_offset(n);
@@ -3747,21 +3777,25 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__re
         p += co;
         if (zold) zold += co;
         if (x) x += co;
+        if (pprev) pprev += co;
         st = col_state(st);
     }
     if (st->done) return;
-    const double beta = st->beta, alpha = st->alpha;
+    const double beta = st->beta, alpha = st->alpha, alpha_prev = st->alpha_prev;
+    const int xm = x ? st->xmode : XM_NONE;  // uniform; no kernel of this launch changes it
+    const bool rx = xm == XM_SINGLE || xm == XM_PAIR;
     const Range R = flat_range(n);
     for (int64_t i0 = R.begin + threadIdx.x; i0 < R.end; i0 += kBlock * FLAT_U) {
         Tz zv[FLAT_U];
-        Tp pv[FLAT_U];
+        Tp pv[FLAT_U], qv[FLAT_U];
         double xv[FLAT_U];
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
             const int64_t i = min(i0 + u * kBlock, R.end - 1);
             zv[u] = z[i];
             pv[u] = p[i];
-            if (x) xv[u] = x[i];
+            if (rx) xv[u] = x[i];
+            if (xm == XM_PAIR) qv[u] = pprev[i];
         }
 #pragma unroll
         for (int u = 0; u < FLAT_U; u++) {
@@ -3769,28 +3803,37 @@ __global__ __launch_bounds__(kBlock) void k_cg_pupdate(int64_t n, const Tz *__re
             if (i >= R.end) break;
             p[i] = Tp((double)zv[u] + beta * (double)pv[u]);
             if (zold) zold[i] = zv[u];
-            if (x) x[i] = xv[u] + alpha * (double)pv[u];
+            if (xm == XM_SINGLE) x[i] = xv[u] + alpha * (double)pv[u];
+            else if (xm == XM_PAIR) x[i] = xv[u] + (alpha_prev * (double)qv[u] + alpha * (double)pv[u]);
+            else if (xm == XM_STASH) pprev[i] = pv[u];
         }
     }
-    if (x && blockIdx.x == 0 && threadIdx.x == 0) st->x_lag = 0;  // nothing else reads it in this launch
 }
 
-// x += alpha p for columns whose last update left it to a p update that did not
-// run (the column stopped); k_x_lag_clear then resets the flags
+// The terms x still lacks (x_lag): alpha_prev p_prev that a p update deferred
+// (bit 1), then alpha p that the residual update left to a p update that did not
+// run (bit 0, the column stopped); k_x_lag_clear then resets the flags
 template <typename Tp>
 __global__ __launch_bounds__(kBlock) void k_x_flush(int64_t n, double *__restrict__ x, const Tp *__restrict__ p,
-                                                     const CGState *st)
+                                                     const Tp *__restrict__ pprev, const CGState *st)
 {
     {
         const int64_t co = col_offset(n);
         x += co;
         p += co;
+        if (pprev) pprev += co;
         st = col_state(st);
     }
-    if (!st->x_lag) return;
-    const double alpha = st->alpha;
+    const int lag = st->x_lag;
+    if (!lag) return;
+    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
     const Range R = flat_range(n);
-    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) x[i] += alpha * (double)p[i];
+    for (int64_t i = R.begin + threadIdx.x; i < R.end; i += kBlock) {
+        double v = x[i];
+        if (lag & 2) v += alpha_prev * (double)pprev[i];
+        if (lag & 1) v += alpha * (double)p[i];
+        x[i] = v;
+    }
 }
 
 __global__ void k_x_lag_clear(CGState *st, int ncols)
@@ -3965,10 +4008,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_hash(int64_t n, float *__restri
     }
 }
 
-__global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int ncols)
+__global__ void k_set_state(CGState *st, int maxit, double tol, int no_stop, int ncols, int xpair)
 {
     if (blockIdx.x == 0 && (int)threadIdx.x < ncols) {
         st += threadIdx.x;
+        st->xpair = xpair;
+        st->xmode = XM_NONE;
+        st->alpha_prev = 0;
         st->no_stop = no_stop;
         st->iter = 0;
         st->done = 0;
@@ -4294,6 +4340,7 @@ struct vfem_system {
     // level-0 vectors
     double *x = nullptr, *r = nullptr, *f = nullptr, *t64 = nullptr, *p64 = nullptr, *ap64 = nullptr, *z64 = nullptr;
     float *p32 = nullptr, *ap32 = nullptr, *z32 = nullptr, *zold32 = nullptr;
+    float *pprev32 = nullptr;  // p of the iteration whose x term the p update deferred
     double *zold64 = nullptr;
     // coarse-level vectors (index = level), t = operator scratch of the damped cycle
     double *rl64[kMaxLevels] = {}, *zl64[kMaxLevels] = {}, *tl64[kMaxLevels] = {};
@@ -4848,6 +4895,7 @@ int alloc_vectors(vfem_system *S, int nb)
     if (int rc = S->alloc(&S->ap64, n3, true, V)) return rc;
     if (int rc = S->alloc(&S->z64, n3, true, V)) return rc;
     if (int rc = S->alloc(&S->p32, n3, true, V)) return rc;
+    if (int rc = S->alloc(&S->pprev32, n3, true, V)) return rc;
     if (int rc = S->alloc(&S->ap32, n3, true, V)) return rc;
     if (int rc = S->alloc(&S->z32, n3, true, V)) return rc;
     if (int rc = S->alloc(&S->zold32, n3, true, V)) return rc;
@@ -5617,6 +5665,13 @@ bool xflex(const vfem_system *S, const Plan &P)
     return flexible(S, P) && P.cycle == VFEM_CYCLE_DAMPED && !P.learned && !P.force_flex;
 }
 
+// p updates defer their x term and apply two at once (XM_STASH / XM_PAIR): the
+// FP32 plan's p update, the one that gets p_prev (k_cg_pupdate<float, float>)
+bool x_pairing(const vfem_system *S, const Plan &P)
+{
+    return P.precision == VFEM_FP32 && !P.op64 && !P.mfma64 && S->H.nlev > 1;
+}
+
 // rzx = r.z_k for a residual that was just replaced (FIN_RZX)
 void enqueue_rzx(vfem_system *S, const Plan &P)
 {
@@ -5689,19 +5744,19 @@ void enqueue_pupdate(vfem_system *S, const Plan &P)
     double *xl = S->lazy_x ? S->x : nullptr;            // x += alpha p left by the residual update
     if (P.precision == VFEM_FP64)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           keep ? S->zold64 : (double *)nullptr, S->st, xl);
+                           keep ? S->zold64 : (double *)nullptr, S->st, xl, (double *)nullptr);
     else if (P.op64 && S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p64,
-                           (double *)nullptr, S->st, xl);
+                           (double *)nullptr, S->st, xl, (double *)nullptr);
     else if (P.op64)  // FP64 direction from the FP32 preconditioned residual
         hipLaunchKernelGGL((k_cg_pupdate<float, double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p64,
-                           keep ? S->zold32 : (float *)nullptr, S->st, xl);
+                           keep ? S->zold32 : (float *)nullptr, S->st, xl, (double *)nullptr);
     else if (S->H.nlev == 1)
         hipLaunchKernelGGL((k_cg_pupdate<double, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z64, S->p32,
-                           (double *)nullptr, S->st, xl);
-    else
+                           (double *)nullptr, S->st, xl, (float *)nullptr);
+    else  // the plan that pairs x terms (x_pairing)
         hipLaunchKernelGGL((k_cg_pupdate<float, float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, n3, S->z32, S->p32,
-                           keep ? S->zold32 : (float *)nullptr, S->st, xl);
+                           keep ? S->zold32 : (float *)nullptr, S->st, xl, S->pprev32);
 }
 
 void enqueue_x_lag_clear(vfem_system *S)
@@ -5716,10 +5771,10 @@ void enqueue_x_flush(vfem_system *S, const Plan &P, bool clear = true)
     const int gf = grid_for_flat(S->n3);
     if (P.precision == VFEM_FP64 || P.op64)
         hipLaunchKernelGGL((k_x_flush<double>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->x, S->p64,
-                           S->st);
+                           (const double *)nullptr, S->st);
     else
         hipLaunchKernelGGL((k_x_flush<float>), dim3(gf, S->ncols), dim3(kBlock), 0, S->stream, S->n3, S->x, S->p32,
-                           S->st);
+                           (const float *)S->pprev32, S->st);
     if (clear) enqueue_x_lag_clear(S);
 }
 
@@ -5900,7 +5955,8 @@ int solve_resident(vfem_system *S, const vfem_options &o, vfem_stats *stats, boo
             if (int rc = get_iteration_graph(S, P, true, &g_true)) return rc;
     }
     HIP_TRY(hipEventRecord(S->ev0, S->stream));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, o.max_iter, o.tol, 0, nc);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, o.max_iter, o.tol, 0, nc,
+                       x_pairing(S, P) ? 1 : 0);
     // |f|^2 and the threshold; r = f with x = 0, or r = f - A x0
     hipLaunchKernelGGL(k_residual, dim3(gf, nc), dim3(kBlock), 0, S->stream, n3, S->f, (const double *)nullptr,
                        S->r, warm ? (double *)nullptr : S->x, S->st, S->red, (int)FIN_RHS);
@@ -6094,7 +6150,7 @@ extern "C" {
 // FP64 reduction of a . b over one level's 3 nl DOFs into the host (setup only)
 int level_dot(vfem_system *S, int64_t n3, const double *a, const double *b, double *out)
 {
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1, 0);
     hipLaunchKernelGGL((k_dot<double, double>), dim3(grid_for_flat(n3)), dim3(kBlock), 0, S->stream, n3, a, b, S->st,
                        S->red, (int)FIN_STORE, 0);
     enqueue_finalize(S, S->stream, grid_for_flat(n3), 1, FIN_STORE, 0);
@@ -6115,7 +6171,7 @@ constexpr double kLmaxSafety = 1.1;
 int estimate_lmax(vfem_system *S)
 {
     const int nl = S->H.nlev;
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, S->stream, S->st, 1, 0.0, 0, 1, 0);
     for (int l = 1; l + 1 < nl; l++) {
         const int64_t n3 = 3 * S->H.L[l].nn;
         const int gf = grid_for_flat(n3);
@@ -6548,7 +6604,7 @@ int vfem_collect_coarse_pairs(vfem_system *sys, const double *f, int32_t level, 
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(copy_to_device(sys->stream, sys->f, f, n3 * sizeof(double)));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, n_iter + 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, n_iter + 1, 0.0, 0, 1, 0);
     hipLaunchKernelGGL(k_residual, dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->f, (const double *)nullptr,
                        sys->r, sys->x, sys->st, sys->red, (int)FIN_RHS);
     enqueue_finalize(sys, sys->stream, gf, 1, FIN_RHS, 0);
@@ -6621,7 +6677,7 @@ int vfem_matvec_block(vfem_system *sys, const double *x, double *y, int32_t nrhs
     sys->ncols = nrhs;
     const int gf = grid_for_flat(n3);
     HIP_TRY(copy_to_device(sys->stream, sys->t64, x, nall * sizeof(double)));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs, 0);
     if (precision == VFEM_FP64) {
         launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0);
     } else {
@@ -6721,7 +6777,7 @@ int vfem_autotune_block(vfem_system *sys, int32_t nrhs, int32_t passes, int32_t 
     sys->ncols = nrhs;
     const int64_t nall = sys->n3 * nrhs;
     hipLaunchKernelGGL(k_fill_hash, dim3(2048), dim3(kBlock), 0, sys->stream, nall, sys->p32, 12345u);
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, nrhs, 0);
     // reference output: configuration 0
     launch_matvec_block(sys, sys->p32, sys->z32, -1, 0, passes, 0);
     std::vector<float> ref(nall), out(nall);
@@ -6799,7 +6855,7 @@ int vfem_matvec(vfem_system *sys, const double *x, double *y, int32_t precision)
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(copy_to_device(sys->stream, sys->t64, x, n3 * sizeof(double)));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1, 0);
     if (precision == VFEM_FP64 || precision == VFEM_MIXED) {
         sys->l0_mfma = precision == VFEM_MIXED;
         launch_matvec_fine<double>(sys, sys->t64, sys->ap64, -1, 0, 6);
@@ -6830,7 +6886,7 @@ int vfem_precondition(vfem_system *sys, const double *r, double *z, const vfem_o
     const int64_t n3 = sys->n3;
     const int gf = grid_for_flat(n3);
     HIP_TRY(copy_to_device(sys->stream, sys->r, r, n3 * sizeof(double)));
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1, 0.0, 0, 1, 0);
     enqueue_precond(sys, P, sys->r, FIN_STORE);
     if (P.precision != VFEM_FP64 && sys->H.nlev > 1)
         hipLaunchKernelGGL((k_convert<float, double>), dim3(gf), dim3(kBlock), 0, sys->stream, n3, sys->z32,
@@ -6881,7 +6937,7 @@ int vfem_dot_noalloc(const double *a, const double *b, int64_t n, double *out)
     Reduction red{sc->partials, nullptr, kNoAllocBlocks};
     CGState *st = &sc->st;
     // the state keeps the previous call's contents: reset it (no_stop, done) first
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, 0, st, 1, 0.0, 0, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, 0, st, 1, 0.0, 0, 1, 0);
     hipLaunchKernelGGL((k_dot<double, double>), dim3(gf), dim3(kBlock), 0, 0, n, a, b, st, red, (int)FIN_STORE, 0);
     hipLaunchKernelGGL(k_finalize, dim3(1, 1), dim3(kBlock), 0, 0, red, gf, st, (int)FIN_STORE, 0);
     HIP_TRY(hipGetLastError());
@@ -6908,7 +6964,7 @@ int vfem_time_kernel(vfem_system *sys, int32_t which, int32_t precision, int32_t
     o.cycle = VFEM_CYCLE_DAMPED;
     Plan P;
     if (int rc = check_plan(sys, o, P)) return rc;
-    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1 << 30, 0.0, 1, 1);
+    hipLaunchKernelGGL(k_set_state, dim3(1), dim3(kMaxCols), 0, sys->stream, sys->st, 1 << 30, 0.0, 1, 1, 0);
     auto one = [&]() {
         if (which == 0) {
             if (P.precision == VFEM_FP64) {

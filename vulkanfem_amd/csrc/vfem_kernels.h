@@ -40,13 +40,26 @@ struct CGState {
     double rr;         // r.r
     double result;     // scratch reduction result (vfem_dot)
     double rzx;        // r_{k+1}.z_k, reduced by the residual update (flexible beta without z_old)
+    double alpha_prev; // alpha of the deferred x term (x_lag bit 1)
     double tol;
     int iter;          // completed iterations
     int maxit;
     int done;          // 0 running, 1 converged, 2 iteration cap, 3 breakdown
     int last_true;     // 1 if the last convergence test used the FP64 true residual
     int no_stop;       // kernel timing: finalizers never end the iteration
-    int x_lag;         // x still lacks alpha p: the residual update left it to the p update
+    int x_lag;         // terms x still lacks: bit 0 alpha p (the residual update left it to the p
+                       // update), bit 1 alpha_prev p_prev (a p update deferred it to the next)
+    int xmode;         // what the next p update does with x (XM_*), set by the finalizer before it
+    int xpair;         // 1: p updates may defer their x term and apply two at once
+};
+static_assert(sizeof(CGState) == 128, "one 128-byte CG state per column");
+
+// x work of a p update (CGState::xmode)
+enum XMode {
+    XM_NONE = 0,    // x is complete
+    XM_SINGLE = 1,  // x += alpha p
+    XM_STASH = 2,   // p_prev = p, x untouched (alpha p deferred)
+    XM_PAIR = 3,    // x += alpha_prev p_prev + alpha p
 };
 
 enum Finalize {
