@@ -30,7 +30,7 @@ PEAK_HBM_GBS = 8000.0
 OMEGA = 0.5
 # The measured plans smooth the coarse levels >= 2 with two Chebyshev steps
 # (interval [lmax / 60, lmax] of D^-1 A) before and after the coarse correction
-# instead of one damped Jacobi step: 46 -> 37 iterations at 256^3 for about 8%
+# instead of one damped Jacobi step: 46 -> 36 iterations at 256^3 for about 8%
 # more time per iteration (levels >= 2 are small).  Level 0 and 1 keep damped
 # Jacobi.  CHEB_LEVEL = 0 gives damped Jacobi everywhere.
 CHEB_LEVEL = 2
@@ -411,7 +411,8 @@ def main():
                    "kept_components": p.info.get("kept_components", 1),
                    "cycle": f"damped V(1,1), omega {OMEGA} on levels 0-{CHEB_LEVEL - 1}, 2 Chebyshev steps on "
                             f"levels >= {CHEB_LEVEL}" if CHEB_LEVEL else f"damped V(1,1), omega {OMEGA}",
-                   "tol": 1e-8, "true_res_every": 20},
+                   "cg": "FP64 flexible PCG (x, r, dots, alpha, beta in FP64)", "tol": 1e-8,
+                   "true_res_every": 20},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": kernel + " (level-0 sum-factorised stencil matvec of the CG operator and the cycle, "
